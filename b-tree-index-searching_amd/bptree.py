@@ -1,0 +1,406 @@
+"""Python host side of libbptree_amd: ctypes bindings over the C ABI in
+include/bptree.h, plus thin classes that mirror the reference's call surface
+(BPTree.h:26-55): build a tree from sorted keys, point search (plain /
+grouped / learned), range search (plain / 2D-grouping / Z-order), rectangle
+search over Morton keys, multi-tree batches and the serving loop.
+
+Device memory is owned through the library's own allocator (no PyTorch): a
+``DeviceArray`` wraps one allocation. The library is loaded from this
+directory; if it is missing the import fails loudly — there is no CPU
+fallback on this path.
+"""
+from __future__ import annotations
+
+import atexit
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbptree_amd.so")
+
+u64p = C.POINTER(C.c_uint64)
+u32p = C.POINTER(C.c_uint32)
+vp = C.c_void_p
+
+BPT_OK = 0
+STATUS = {0: "OK", 1: "INVALID", 2: "UNSORTED", 3: "NOMEM", 4: "HIP", 5: "CAPACITY",
+          6: "UNSUPPORTED", 7: "NOT_FOUND"}
+PLAIN, GROUPED, LEARNED = 0, 1, 2
+GROUP_NONE, GROUP_GRID2D, GROUP_ZORDER = 0, 1, 2
+SENTINEL = {4: 0xFFFFFFFF, 8: 0xFFFFFFFFFFFFFFFF}
+MAX_LEVELS = 16
+
+
+class BptError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"bpt status {STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class BuildDesc(C.Structure):
+    _fields_ = [("key_bytes", C.c_uint32), ("val_bytes", C.c_uint32), ("fanout", C.c_uint32),
+                ("leaf_cap", C.c_uint32), ("check_sorted", C.c_uint32), ("lds_bytes", C.c_uint32)]
+
+
+class TreeInfo(C.Structure):
+    _fields_ = [("n", C.c_uint64), ("key_bytes", C.c_uint32), ("val_bytes", C.c_uint32),
+                ("fanout", C.c_uint32), ("leaf_cap", C.c_uint32), ("height", C.c_uint32),
+                ("lds_levels", C.c_uint32), ("lds_bytes", C.c_uint64), ("leaf_stride", C.c_uint64),
+                ("leaf_bytes", C.c_uint64), ("inner_bytes", C.c_uint64),
+                ("level_nodes", C.c_uint64 * MAX_LEVELS)]
+
+
+# (name, restype, argtypes) of every exported symbol; include/bptree.h order.
+SIGNATURES = [
+    ("bpt_last_error", C.c_char_p, []),
+    ("bpt_version", C.c_char_p, []),
+    ("bpt_device_count", C.c_int, [C.POINTER(C.c_int)]),
+    ("bpt_device_info", C.c_int, [C.c_int, C.POINTER(C.c_int), u64p, C.c_char_p, C.c_int]),
+    ("bpt_device_alloc", C.c_int, [C.c_uint64, C.POINTER(vp)]),
+    ("bpt_device_free", C.c_int, [vp]),
+    ("bpt_memset", C.c_int, [vp, C.c_int, C.c_uint64, vp]),
+    ("bpt_copy_h2d", C.c_int, [vp, vp, C.c_uint64, vp]),
+    ("bpt_copy_d2h", C.c_int, [vp, vp, C.c_uint64, vp]),
+    ("bpt_copy_d2d", C.c_int, [vp, vp, C.c_uint64, vp]),
+    ("bpt_stream_create", C.c_int, [C.POINTER(vp)]),
+    ("bpt_stream_destroy", C.c_int, [vp]),
+    ("bpt_stream_sync", C.c_int, [vp]),
+    ("bpt_device_sync", C.c_int, []),
+    ("bpt_timer_create", C.c_int, [C.POINTER(vp)]),
+    ("bpt_timer_destroy", C.c_int, [vp]),
+    ("bpt_timer_start", C.c_int, [vp, vp]),
+    ("bpt_timer_stop", C.c_int, [vp, vp]),
+    ("bpt_timer_ms", C.c_int, [vp, C.POINTER(C.c_float)]),
+    ("bpt_tree_build", C.c_int, [C.POINTER(BuildDesc), vp, vp, C.c_uint64, C.POINTER(vp), vp]),
+    ("bpt_tree_free", C.c_int, [vp]),
+    ("bpt_tree_get_info", C.c_int, [vp, C.POINTER(TreeInfo)]),
+    ("bpt_tree_get_layout", C.c_int, [vp, C.POINTER(vp), C.POINTER(vp)]),
+    ("bpt_tree_set_launch", C.c_int, [vp, C.c_uint32, C.c_uint32]),
+    ("bpt_point_search", C.c_int, [vp, vp, C.c_uint64, vp, C.c_int, vp]),
+    ("bpt_point_search_ws_size", C.c_uint64, [vp, C.c_uint64, C.c_int]),
+    ("bpt_point_search_ws", C.c_int, [vp, vp, C.c_uint64, vp, C.c_int, vp, C.c_uint64, vp]),
+    ("bpt_tree_set_group_bits", C.c_int, [vp, C.c_uint32]),
+    ("bpt_lower_bound", C.c_int, [vp, vp, C.c_uint64, vp, vp]),
+    ("bpt_range_count", C.c_int, [vp, vp, vp, C.c_uint64, vp, vp, vp]),
+    ("bpt_range_ids", C.c_int, [vp, vp, vp, vp, C.c_uint64, vp, vp]),
+    ("bpt_exclusive_scan_u64", C.c_int, [vp, C.c_uint64, vp, u64p, vp]),
+    ("bpt_range_count_grouped", C.c_int, [vp, vp, vp, C.c_uint64, C.c_int, vp, vp, vp]),
+    ("bpt_radix_group", C.c_int, [vp, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32, vp, vp, vp]),
+    ("bpt_morton_encode", C.c_int, [vp, vp, C.c_uint64, vp, vp]),
+    ("bpt_zorder2d", C.c_int, [vp, vp, C.c_uint64, vp, vp]),
+    ("bpt_group2d_keys", C.c_int, [vp, vp, C.c_uint64, vp, vp]),
+    ("bpt_rect_count", C.c_int, [vp, vp, C.c_uint64, C.c_int, vp, vp]),
+    ("bpt_rect_ids", C.c_int, [vp, vp, C.c_uint64, C.c_int, vp, vp, vp]),
+]
+
+_lib = None
+_exiting = False
+
+
+def _mark_exit():
+    global _exiting
+    _exiting = True
+
+
+atexit.register(_mark_exit)
+
+
+def lib():
+    """Loads libbptree_amd.so (built by build.py); raises if it is absent."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built: run __graft_entry__.build() "
+                              "(there is no CPU fallback for the HIP path)")
+        L = C.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status):
+    if status != BPT_OK:
+        raise BptError(status, lib().bpt_last_error().decode(errors="replace"))
+
+
+def device_count():
+    n = C.c_int(0)
+    check(lib().bpt_device_count(C.byref(n)))
+    return n.value
+
+
+def device_info(dev=0):
+    cu = C.c_int(0)
+    mem = C.c_uint64(0)
+    name = C.create_string_buffer(256)
+    check(lib().bpt_device_info(dev, C.byref(cu), C.byref(mem), name, 256))
+    return {"cus": cu.value, "hbm_bytes": mem.value, "name": name.value.decode()}
+
+
+def sync():
+    check(lib().bpt_device_sync())
+
+
+class DeviceArray:
+    """One device allocation holding a 1-D array of a numpy dtype."""
+
+    def __init__(self, n, dtype):
+        self.dtype = np.dtype(dtype)
+        self.n = int(n)
+        self.nbytes = self.n * self.dtype.itemsize
+        p = vp()
+        check(lib().bpt_device_alloc(self.nbytes, C.byref(p)))
+        self.ptr = p.value or 0
+
+    @classmethod
+    def from_host(cls, a, stream=None):
+        a = np.ascontiguousarray(a)
+        d = cls(a.size, a.dtype)
+        if a.size:
+            check(lib().bpt_copy_h2d(d.ptr, a.ctypes.data, d.nbytes, stream))
+        return d
+
+    def to_host(self, stream=None, count=None):
+        n = self.n if count is None else int(count)
+        out = np.empty(n, dtype=self.dtype)
+        if n:
+            check(lib().bpt_copy_d2h(out.ctypes.data, self.ptr, n * self.dtype.itemsize, stream))
+        return out
+
+    def fill_bytes(self, value, stream=None):
+        check(lib().bpt_memset(self.ptr, value, self.nbytes, stream))
+
+    def free(self):
+        if getattr(self, "ptr", 0) and not _exiting:
+            lib().bpt_device_free(self.ptr)
+        self.ptr = 0
+
+    __del__ = free
+
+    def __len__(self):
+        return self.n
+
+
+class Timer:
+    """HIP events recorded on the stream the measured kernels run on."""
+
+    def __init__(self):
+        p = vp()
+        check(lib().bpt_timer_create(C.byref(p)))
+        self.h = p.value
+
+    def start(self, stream=None):
+        check(lib().bpt_timer_start(self.h, stream))
+
+    def stop(self, stream=None):
+        check(lib().bpt_timer_stop(self.h, stream))
+
+    def ms(self):
+        v = C.c_float(0)
+        check(lib().bpt_timer_ms(self.h, C.byref(v)))
+        return float(v.value)
+
+    def close(self):
+        if getattr(self, "h", None) and not _exiting:
+            lib().bpt_timer_destroy(self.h)
+        self.h = None
+
+    __del__ = close
+
+
+class Stream:
+    def __init__(self):
+        p = vp()
+        check(lib().bpt_stream_create(C.byref(p)))
+        self.h = p.value
+
+    def sync(self):
+        check(lib().bpt_stream_sync(self.h))
+
+    def close(self):
+        if getattr(self, "h", None) and not _exiting:
+            lib().bpt_stream_destroy(self.h)
+        self.h = None
+
+    __del__ = close
+
+
+def _dev(a, dtype):
+    """Returns (DeviceArray, owned) for a host array or DeviceArray."""
+    if isinstance(a, DeviceArray):
+        if a.dtype != np.dtype(dtype):
+            raise TypeError(f"device array dtype {a.dtype} != {np.dtype(dtype)}")
+        return a, False
+    return DeviceArray.from_host(np.asarray(a, dtype=dtype)), True
+
+
+_KDT = {4: np.uint32, 8: np.uint64}
+
+
+class Tree:
+    """A bulk-built B+tree resident on the device (genBPlusTree, BPTree.cu:127-306)."""
+
+    def __init__(self, keys, vals, key_bytes=None, val_bytes=None, fanout=0, leaf_cap=0,
+                 check_sorted=True, lds_bytes=0, stream=None):
+        if key_bytes is None:
+            key_bytes = np.dtype(keys.dtype).itemsize
+        if val_bytes is None:
+            val_bytes = np.dtype(vals.dtype).itemsize
+        self.key_bytes, self.val_bytes = int(key_bytes), int(val_bytes)
+        self.kdt, self.vdt = _KDT[self.key_bytes], _KDT[self.val_bytes]
+        dk, ok = _dev(keys, self.kdt)
+        dv, ov = _dev(vals, self.vdt)
+        d = BuildDesc(self.key_bytes, self.val_bytes, fanout, leaf_cap, 1 if check_sorted else 0,
+                      lds_bytes)
+        h = vp()
+        try:
+            check(lib().bpt_tree_build(C.byref(d), dk.ptr, dv.ptr, dk.n, C.byref(h), stream))
+        finally:
+            if ok:
+                dk.free()
+            if ov:
+                dv.free()
+        self.h = h.value
+        self.n = dk.n
+        self.sentinel = SENTINEL[self.val_bytes]
+
+    def close(self):
+        if getattr(self, "h", None) and not _exiting:
+            lib().bpt_tree_free(self.h)
+        self.h = None
+
+    __del__ = close
+
+    def info(self):
+        i = TreeInfo()
+        check(lib().bpt_tree_get_info(self.h, C.byref(i)))
+        d = {k: getattr(i, k) for k, _ in TreeInfo._fields_ if k != "level_nodes"}
+        d["level_nodes"] = [int(i.level_nodes[k]) for k in range(i.height + 1)]
+        return d
+
+    def set_launch(self, block=0, blocks_per_cu=0):
+        check(lib().bpt_tree_set_launch(self.h, block, blocks_per_cu))
+
+    def set_group_bits(self, bits):
+        check(lib().bpt_tree_set_group_bits(self.h, bits))
+
+    # ---- point search ------------------------------------------------------
+    def point_search_dev(self, dq, dout, mode=PLAIN, stream=None):
+        check(lib().bpt_point_search(self.h, dq.ptr, dq.n, dout.ptr, mode, stream))
+
+    def point_search(self, q, mode=PLAIN, stream=None):
+        dq, own = _dev(q, self.kdt)
+        out = DeviceArray(dq.n, self.vdt)
+        self.point_search_dev(dq, out, mode, stream)
+        r = out.to_host(stream)
+        out.free()
+        if own:
+            dq.free()
+        return r
+
+    def lower_bound(self, q, stream=None):
+        dq, own = _dev(q, self.kdt)
+        pos = DeviceArray(dq.n, np.uint64)
+        check(lib().bpt_lower_bound(self.h, dq.ptr, dq.n, pos.ptr, stream))
+        r = pos.to_host(stream)
+        if own:
+            dq.free()
+        return r
+
+    # ---- range search ------------------------------------------------------
+    def range_count_dev(self, dlo, dhi, dfirst, dcount, group_mode=GROUP_NONE, stream=None):
+        fp = dfirst.ptr if dfirst is not None else None
+        if group_mode == GROUP_NONE:
+            check(lib().bpt_range_count(self.h, dlo.ptr, dhi.ptr, dlo.n, fp, dcount.ptr, stream))
+        else:
+            check(lib().bpt_range_count_grouped(self.h, dlo.ptr, dhi.ptr, dlo.n, group_mode, fp,
+                                                dcount.ptr, stream))
+
+    def range_search(self, lo, hi, group_mode=GROUP_NONE, with_ids=True, stream=None):
+        """Returns (first, count, offsets, ids) in the caller's query order."""
+        dlo, o1 = _dev(lo, self.kdt)
+        dhi, o2 = _dev(hi, self.kdt)
+        n = dlo.n
+        dfirst = DeviceArray(n, np.uint64)
+        dcount = DeviceArray(n, np.uint64)
+        self.range_count_dev(dlo, dhi, dfirst, dcount, group_mode, stream)
+        first, count = dfirst.to_host(stream), dcount.to_host(stream)
+        offsets = ids = None
+        if with_ids:
+            doff = DeviceArray(n, np.uint64)
+            total = C.c_uint64(0)
+            check(lib().bpt_exclusive_scan_u64(dcount.ptr, n, doff.ptr, C.byref(total), stream))
+            dids = DeviceArray(total.value, self.vdt)
+            check(lib().bpt_range_ids(self.h, dfirst.ptr, dcount.ptr, doff.ptr, n, dids.ptr, stream))
+            offsets, ids = doff.to_host(stream), dids.to_host(stream)
+        for d, own in ((dlo, o1), (dhi, o2)):
+            if own:
+                d.free()
+        return first, count, offsets, ids
+
+    # ---- rectangles --------------------------------------------------------
+    def rect_search(self, rects, group_mode=GROUP_NONE, with_ids=True, stream=None):
+        """rects: (n, 4) uint32 [x0, y0, x1, y1]; returns (counts, offsets, ids)."""
+        dr, own = _dev(np.asarray(rects, dtype=np.uint32).reshape(-1), np.uint32)
+        nr = dr.n // 4
+        dcount = DeviceArray(nr, np.uint64)
+        check(lib().bpt_rect_count(self.h, dr.ptr, nr, group_mode, dcount.ptr, stream))
+        counts = dcount.to_host(stream)
+        offsets = ids = None
+        if with_ids:
+            doff = DeviceArray(nr, np.uint64)
+            total = C.c_uint64(0)
+            check(lib().bpt_exclusive_scan_u64(dcount.ptr, nr, doff.ptr, C.byref(total), stream))
+            dids = DeviceArray(total.value, np.uint32)
+            check(lib().bpt_rect_ids(self.h, dr.ptr, nr, group_mode, doff.ptr, dids.ptr, stream))
+            offsets, ids = doff.to_host(stream), dids.to_host(stream)
+        if own:
+            dr.free()
+        return counts, offsets, ids
+
+
+def exclusive_scan(counts):
+    d = DeviceArray.from_host(np.asarray(counts, dtype=np.uint64))
+    out = DeviceArray(d.n, np.uint64)
+    total = C.c_uint64(0)
+    check(lib().bpt_exclusive_scan_u64(d.ptr, d.n, out.ptr, C.byref(total), None))
+    return out.to_host(), int(total.value)
+
+
+def radix_group(keys, shift, bits):
+    keys = np.ascontiguousarray(keys)
+    kb = keys.dtype.itemsize
+    d = DeviceArray.from_host(keys)
+    perm = DeviceArray(d.n, np.uint32)
+    starts = DeviceArray((1 << bits) + 1, np.uint64)
+    check(lib().bpt_radix_group(d.ptr, kb, d.n, shift, bits, perm.ptr, starts.ptr, None))
+    return perm.to_host(), starts.to_host()
+
+
+def morton_encode(x, y):
+    dx = DeviceArray.from_host(np.asarray(x, dtype=np.uint32))
+    dy = DeviceArray.from_host(np.asarray(y, dtype=np.uint32))
+    dz = DeviceArray(dx.n, np.uint64)
+    check(lib().bpt_morton_encode(dx.ptr, dy.ptr, dx.n, dz.ptr, None))
+    return dz.to_host()
+
+
+def zorder2d(a, b):
+    da = DeviceArray.from_host(np.asarray(a, dtype=np.uint64))
+    db = DeviceArray.from_host(np.asarray(b, dtype=np.uint64))
+    out = DeviceArray(da.n, np.uint64)
+    check(lib().bpt_zorder2d(da.ptr, db.ptr, da.n, out.ptr, None))
+    return out.to_host()
+
+
+def group2d_keys(a, b):
+    da = DeviceArray.from_host(np.asarray(a, dtype=np.uint64))
+    db = DeviceArray.from_host(np.asarray(b, dtype=np.uint64))
+    out = DeviceArray(da.n, np.uint32)
+    check(lib().bpt_group2d_keys(da.ptr, db.ptr, da.n, out.ptr, None))
+    return out.to_host()

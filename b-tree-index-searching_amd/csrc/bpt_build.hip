@@ -1,0 +1,294 @@
+// Bulk build: sorted (key, value) arrays -> flat, level-ordered B+tree.
+//
+// Replaces genBPlusTree (reference BPTree.cu:127-306), which copies PAGESIZE
+// chunks into 80 KB pages (assignDataToLeaf, BPTree.cu:73-125) and then runs
+// one kernel per internal level whose separators are the child maxima
+// (kernelMaxIntermidatePage, BPTree.cu:28-70). Here:
+//  * leaves are fixed-size lines of `leaf_cap` keys followed by their values,
+//    so one leaf is one 128-byte HBM line for 4/4- and 8/8-byte pairs;
+//  * internal levels hold only separators (child maximum, as the reference),
+//    children are implicit (child c of node i is node i*F + c one level
+//    down), levels are stored root first so the top of the tree is one
+//    contiguous prefix that search kernels stage in LDS;
+//  * every separator is a closed form of the input: the maximum of a node at
+//    level k covers keys [i*LF*F^k, (i+1)*LF*F^k), so all internal levels are
+//    written by ONE level-parallel launch that reads the sorted keys directly.
+#include <algorithm>
+#include <cstring>
+
+#include "bpt_internal.hpp"
+
+namespace bpt {
+
+template <class K, class V, int LF>
+__global__ __launch_bounds__(256) void k_build_leaves(const K *__restrict__ keys,
+                                                      const V *__restrict__ vals, uint64_t n,
+                                                      uint64_t nslots, uint8_t *__restrict__ leaves,
+                                                      uint64_t stride) {
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < nslots; s += step) {
+        const uint64_t leaf = s / LF;
+        const uint32_t j = (uint32_t)(s % LF);
+        K k = s < n ? keys[s] : key_max<K>();
+        V v = s < n ? vals[s] : V(0);
+        uint8_t *lp = leaves + leaf * stride;
+        reinterpret_cast<K *>(lp)[j] = k;
+        reinterpret_cast<V *>(lp + LF * sizeof(K))[j] = v;
+    }
+}
+
+struct InnerDesc {
+    int32_t height;
+    uint64_t count[kMaxLevels + 1];
+    uint64_t first_slot[kMaxLevels + 1];  // first separator slot of level k in `inner`
+    uint64_t span[kMaxLevels + 1];        // keys covered by one node of level k
+};
+
+template <class K, int F>
+__global__ __launch_bounds__(256) void k_build_inner(const K *__restrict__ keys, uint64_t n,
+                                                     InnerDesc d, uint64_t total_slots,
+                                                     K *__restrict__ inner) {
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < total_slots; s += step) {
+        int k = d.height;  // levels are stored root first
+        while (k > 1 && s >= d.first_slot[k - 1]) --k;
+        const uint64_t local = s - d.first_slot[k];
+        const uint64_t child = local;  // == node * F + c
+        K sep = key_max<K>();
+        if (child < d.count[k - 1]) {
+            uint64_t end = (child + 1) * d.span[k - 1];
+            sep = keys[(end < n ? end : n) - 1];
+        }
+        inner[s] = sep;
+    }
+}
+
+template <class K>
+__global__ __launch_bounds__(256) void k_check_sorted(const K *__restrict__ keys, uint64_t n,
+                                                      uint32_t *bad) {
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    uint32_t b = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i + 1 < n; i += step)
+        b |= keys[i] > keys[i + 1];
+    if (__any(b) && (threadIdx.x & 63) == 0) atomicOr(bad, 1u);
+}
+
+uint32_t search_grid(const bpt_tree *t, uint64_t work_items) {
+    uint64_t blocks = (work_items + t->block - 1) / t->block;
+    uint64_t cap = (uint64_t)device_cu_count() * t->blocks_per_cu;
+    if (blocks > cap) blocks = cap;
+    if (blocks == 0) blocks = 1;
+    return (uint32_t)blocks;
+}
+
+static bpt_status compute_layout(bpt_tree *t) {
+    const uint64_t LF = t->leaf_cap, F = t->fanout, kb = t->key_bytes;
+    TreeView &v = t->view;
+    memset(&v, 0, sizeof v);
+    v.n = t->n;
+    v.leaf_stride = t->leaf_stride;
+    uint64_t cnt = (t->n + LF - 1) / LF;
+    v.count[0] = cnt;
+    int h = 0;
+    while (cnt > 1) {
+        BPT_CHECK(h < kMaxLevels, BPT_ERR_CAPACITY, "tree taller than %d levels", kMaxLevels);
+        cnt = (cnt + F - 1) / F;
+        v.count[++h] = cnt;
+    }
+    v.height = h;
+    uint64_t off = 0;
+    for (int k = h; k >= 1; --k) {
+        v.off[k] = off;
+        off += v.count[k] * F * kb;
+    }
+    t->inner_bytes = off;
+    t->leaf_bytes = v.count[0] * t->leaf_stride;
+    // top levels that fit the LDS budget
+    uint64_t lds = 0;
+    int m = 0;
+    for (int k = h; k >= 1; --k) {
+        uint64_t b = v.count[k] * F * kb;
+        if (lds + b > t->lds_budget) break;
+        lds += b;
+        ++m;
+    }
+    v.lds_levels = m;
+    v.lds_bytes = (uint32_t)lds;
+    return BPT_OK;
+}
+
+template <class K, class V, int F, int LF>
+static bpt_status build_impl(bpt_tree *t, const void *d_keys, const void *d_vals, hipStream_t s) {
+    const TreeView &v = t->view;
+    const uint64_t n = t->n;
+    if (n == 0) return BPT_OK;
+    const uint64_t nslots = v.count[0] * LF;
+    {
+        uint64_t blocks = std::min<uint64_t>((nslots + 255) / 256, (uint64_t)device_cu_count() * 16);
+        k_build_leaves<K, V, LF><<<(uint32_t)blocks, 256, 0, s>>>(
+            (const K *)d_keys, (const V *)d_vals, n, nslots, t->d_leaves, t->leaf_stride);
+        BPT_HIP(hipGetLastError());
+    }
+    if (v.height >= 1) {
+        InnerDesc d;
+        memset(&d, 0, sizeof d);
+        d.height = v.height;
+        uint64_t span = LF;
+        for (int k = 0; k <= v.height; ++k) {
+            d.count[k] = v.count[k];
+            d.span[k] = span;
+            span = (span > (~0ull) / F) ? ~0ull : span * F;
+            if (k >= 1) d.first_slot[k] = v.off[k] / sizeof(K);
+        }
+        const uint64_t total = t->inner_bytes / sizeof(K);
+        uint64_t blocks = std::min<uint64_t>((total + 255) / 256, (uint64_t)device_cu_count() * 16);
+        k_build_inner<K, F><<<(uint32_t)blocks, 256, 0, s>>>((const K *)d_keys, n, d, total,
+                                                             (K *)t->d_inner);
+        BPT_HIP(hipGetLastError());
+    }
+    return BPT_OK;
+}
+
+template <class K>
+static bpt_status check_sorted(const void *d_keys, uint64_t n, hipStream_t s, bool *sorted) {
+    *sorted = true;
+    if (n < 2) return BPT_OK;
+    uint32_t *d_bad = nullptr;
+    BPT_HIP(hipMalloc(&d_bad, sizeof(uint32_t)));
+    uint32_t h_bad = 0;
+    hipError_t e = hipMemsetAsync(d_bad, 0, sizeof(uint32_t), s);
+    if (e == hipSuccess) {
+        uint64_t blocks = std::min<uint64_t>((n + 255) / 256, (uint64_t)device_cu_count() * 16);
+        k_check_sorted<K><<<(uint32_t)blocks, 256, 0, s>>>((const K *)d_keys, n, d_bad);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&h_bad, d_bad, sizeof h_bad, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(d_bad);
+    if (e != hipSuccess) return hip_fail(e, "sorted check", __FILE__, __LINE__);
+    *sorted = (h_bad == 0);
+    return BPT_OK;
+}
+
+}  // namespace bpt
+
+using namespace bpt;
+
+extern "C" {
+
+bpt_status bpt_tree_build(const bpt_build_desc *desc, const void *d_keys, const void *d_vals,
+                          uint64_t n, bpt_tree **out, bpt_stream stream) {
+    BPT_CHECK(desc && out, BPT_ERR_INVALID, "desc/out is NULL");
+    *out = nullptr;
+    const uint32_t kb = desc->key_bytes, vb = desc->val_bytes;
+    BPT_CHECK((kb == 4 && vb == 4) || (kb == 8 && vb == 8) || (kb == 8 && vb == 4), BPT_ERR_INVALID,
+              "key/value widths %u/%u not supported (4/4, 8/8, 8/4)", kb, vb);
+    uint32_t F = desc->fanout ? desc->fanout : 128 / kb;
+    uint32_t LF = desc->leaf_cap ? desc->leaf_cap : (kb + vb == 12 ? 8 : 128 / (kb + vb));
+    BPT_CHECK(F == 16 || F == 32 || F == 64, BPT_ERR_INVALID, "fanout %u not in {16,32,64}", F);
+    BPT_CHECK(LF == 8 || LF == 16 || LF == 32, BPT_ERR_INVALID, "leaf_cap %u not in {8,16,32}", LF);
+    BPT_CHECK(n == 0 || (d_keys && d_vals), BPT_ERR_INVALID, "keys/values are NULL");
+    hipStream_t s = as_stream(stream);
+
+    if (desc->check_sorted && n > 1) {
+        bool sorted = true;
+        BPT_TRY(kb == 4 ? check_sorted<uint32_t>(d_keys, n, s, &sorted)
+                        : check_sorted<uint64_t>(d_keys, n, s, &sorted));
+        BPT_CHECK(sorted, BPT_ERR_UNSORTED, "build keys are not sorted non-decreasing");
+    }
+
+    bpt_tree *t = new bpt_tree;
+    t->key_bytes = kb;
+    t->val_bytes = vb;
+    t->fanout = F;
+    t->leaf_cap = LF;
+    t->n = n;
+    t->leaf_stride = ((uint64_t)LF * (kb + vb) + 63) / 64 * 64;
+    t->lds_budget = desc->lds_bytes ? desc->lds_bytes : 32768;
+    if (t->lds_budget > 65536) t->lds_budget = 65536;
+    (void)hipGetDevice(&t->device);
+
+    bpt_status st = compute_layout(t);
+    if (st == BPT_OK && t->leaf_bytes) {
+        hipError_t e = hipMalloc(&t->d_leaves, t->leaf_bytes);
+        if (e != hipSuccess) st = hip_fail(e, "hipMalloc(leaves)", __FILE__, __LINE__);
+    }
+    if (st == BPT_OK && t->inner_bytes) {
+        hipError_t e = hipMalloc(&t->d_inner, t->inner_bytes);
+        if (e != hipSuccess) st = hip_fail(e, "hipMalloc(inner)", __FILE__, __LINE__);
+    }
+    t->view.inner = t->d_inner;
+    t->view.leaves = t->d_leaves;
+    if (st == BPT_OK) {
+        st = dispatch_tree(t, [&](auto kt, auto vt, auto fc, auto lc) {
+            using K = typename decltype(kt)::type;
+            using V = typename decltype(vt)::type;
+            return build_impl<K, V, decltype(fc)::value, decltype(lc)::value>(t, d_keys, d_vals, s);
+        });
+    }
+    if (st == BPT_OK) {
+        hipError_t e = hipStreamSynchronize(s);
+        if (e != hipSuccess) st = hip_fail(e, "build sync", __FILE__, __LINE__);
+    }
+    if (st != BPT_OK) {
+        bpt_tree_free(t);
+        return st;
+    }
+    *out = t;
+    return BPT_OK;
+}
+
+bpt_status bpt_tree_free(bpt_tree *t) {
+    if (!t) return BPT_OK;
+    if (t->learned) learned_free(t->learned);
+    if (t->d_leaves) (void)hipFree(t->d_leaves);
+    if (t->d_inner) (void)hipFree(t->d_inner);
+    delete t;
+    return BPT_OK;
+}
+
+bpt_status bpt_tree_get_info(const bpt_tree *t, bpt_tree_info *info) {
+    BPT_CHECK(t && info, BPT_ERR_INVALID, "NULL argument");
+    memset(info, 0, sizeof *info);
+    info->n = t->n;
+    info->key_bytes = t->key_bytes;
+    info->val_bytes = t->val_bytes;
+    info->fanout = t->fanout;
+    info->leaf_cap = t->leaf_cap;
+    info->height = (uint32_t)t->view.height;
+    info->lds_levels = (uint32_t)t->view.lds_levels;
+    info->lds_bytes = t->view.lds_bytes;
+    info->leaf_stride = t->leaf_stride;
+    info->leaf_bytes = t->leaf_bytes;
+    info->inner_bytes = t->inner_bytes;
+    for (int k = 0; k <= t->view.height && k < BPT_MAX_LEVELS; ++k) info->level_nodes[k] = t->view.count[k];
+    return BPT_OK;
+}
+
+bpt_status bpt_tree_get_layout(const bpt_tree *t, const void **d_leaves, const void **d_inner) {
+    BPT_CHECK(t, BPT_ERR_INVALID, "tree is NULL");
+    if (d_leaves) *d_leaves = t->d_leaves;
+    if (d_inner) *d_inner = t->d_inner;
+    return BPT_OK;
+}
+
+bpt_status bpt_tree_set_launch(bpt_tree *t, uint32_t block, uint32_t blocks_per_cu) {
+    BPT_CHECK(t, BPT_ERR_INVALID, "tree is NULL");
+    if (block) {
+        BPT_CHECK(block == 64 || block == 128 || block == 256, BPT_ERR_INVALID,
+                  "block %u not 64, 128 or 256 (the search kernels' launch bound)", block);
+        t->block = block;
+    }
+    if (blocks_per_cu) t->blocks_per_cu = blocks_per_cu;
+    return BPT_OK;
+}
+
+bpt_status bpt_tree_set_group_bits(bpt_tree *t, uint32_t bits) {
+    BPT_CHECK(t, BPT_ERR_INVALID, "tree is NULL");
+    BPT_CHECK(bits >= 1 && bits <= 16 && bits <= t->key_bytes * 8, BPT_ERR_INVALID,
+              "group bits %u not in [1, 16]", bits);
+    t->group_bits = bits;
+    return BPT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,652 @@
+// Query grouping on the device: radix partition of point queries on their top
+// key bits, stable LSD radix sort (the grouping front ends of range search),
+// Morton / Z-order / 2-D grouping keys.
+//
+// The reference groups queries with a host-driven MSD loop (genGrouping,
+// grouping.cu:1677-1851): 8-bit passes, a device histogram copied to the host
+// after every pass, a 256-entry prefix sum on one block, relocation, and a
+// host loop merging adjacent buckets below 33000 queries. Results come back
+// in grouped order and overwrite the caller's queries. Here grouping is one
+// on-device partition with no host round trip, and results are returned in
+// the caller's order.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "bpt_internal.hpp"
+
+namespace bpt {
+
+bpt_status range_count_ordered(const bpt_tree *t, const void *d_lo, const void *d_hi, uint64_t nq,
+                               const uint32_t *d_order, uint64_t *d_first, uint64_t *d_count,
+                               hipStream_t s);
+
+static inline uint64_t align256(uint64_t b) { return (b + 255) / 256 * 256; }
+
+// ==================================================================== grouped point search
+// Pipeline for nq queries, B = 2^bits buckets, tiles of T queries:
+//   1 k_tile_hist     per-tile bucket counts, tile-major hist[t][b]
+//   2 k_col_partial   sums of 64-tile chunks per bucket
+//   3 k_col_scan      per-bucket chunk bases + bucket bases (one block)
+//   4 k_col_final     hist[t][b] <- first grouped slot of (bucket b, tile t)
+//   5 k_tile_scatter  grouped[slot] = q, pos[i] = slot (slot from an LDS cursor)
+//   6 plain descent on the grouped queries (neighbouring lanes share subtrees)
+//   7 k_ungather      out[i] = grouped_result[pos[i]]
+constexpr int kGBlock = 1024;
+constexpr int kChunkTiles = 64;
+
+template <class K>
+__global__ __launch_bounds__(kGBlock) void k_tile_hist(const K *__restrict__ q, uint64_t nq,
+                                                       uint32_t shift, uint32_t nb, uint32_t tile,
+                                                       uint32_t *__restrict__ hist) {
+    extern __shared__ uint32_t cnt[];
+    for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) cnt[b] = 0;
+    __syncthreads();
+    const uint64_t base = (uint64_t)blockIdx.x * tile;
+    const uint64_t end = std::min<uint64_t>(base + tile, nq);
+    for (uint64_t i = base + threadIdx.x; i < end; i += blockDim.x) {
+        const uint32_t b = (uint32_t)(q[i] >> shift);
+        atomicAdd(&cnt[b], 1u);
+    }
+    __syncthreads();
+    uint32_t *row = hist + (uint64_t)blockIdx.x * nb;
+    for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) row[b] = cnt[b];
+}
+
+__global__ void k_col_partial(const uint32_t *__restrict__ hist, uint32_t ntiles, uint32_t nb,
+                              uint32_t *__restrict__ part) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t c = blockIdx.y;
+    if (b >= nb) return;
+    const uint32_t t0 = c * kChunkTiles, t1 = std::min<uint32_t>(t0 + kChunkTiles, ntiles);
+    uint32_t s = 0;
+    for (uint32_t t = t0; t < t1; ++t) s += hist[(uint64_t)t * nb + b];
+    part[(uint64_t)c * nb + b] = s;
+}
+
+__global__ void k_col_final(uint32_t *__restrict__ hist, const uint32_t *__restrict__ part,
+                            uint32_t ntiles, uint32_t nb) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t c = blockIdx.y;
+    if (b >= nb) return;
+    const uint32_t t0 = c * kChunkTiles, t1 = std::min<uint32_t>(t0 + kChunkTiles, ntiles);
+    uint32_t run = part[(uint64_t)c * nb + b];
+    for (uint32_t t = t0; t < t1; ++t) {
+        uint64_t idx = (uint64_t)t * nb + b;
+        uint32_t v = hist[idx];
+        hist[idx] = run;
+        run += v;
+    }
+}
+
+template <class K>
+__global__ __launch_bounds__(kGBlock) void k_tile_scatter(const K *__restrict__ q, uint64_t nq,
+                                                          uint32_t shift, uint32_t nb, uint32_t tile,
+                                                          const uint32_t *__restrict__ offs,
+                                                          K *__restrict__ gq, uint32_t *__restrict__ pos) {
+    extern __shared__ uint32_t cur[];
+    const uint32_t *row = offs + (uint64_t)blockIdx.x * nb;
+    for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) cur[b] = row[b];
+    __syncthreads();
+    const uint64_t base = (uint64_t)blockIdx.x * tile;
+    const uint64_t end = std::min<uint64_t>(base + tile, nq);
+    for (uint64_t i = base + threadIdx.x; i < end; i += blockDim.x) {
+        const K key = q[i];
+        const uint32_t b = (uint32_t)(key >> shift);
+        const uint32_t slot = atomicAdd(&cur[b], 1u);
+        gq[slot] = key;
+        pos[i] = slot;
+    }
+}
+
+template <class V>
+__global__ __launch_bounds__(256) void k_ungather(const V *__restrict__ gres, const uint32_t *__restrict__ pos,
+                                                  uint64_t nq, V *__restrict__ out) {
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += step)
+        __builtin_nontemporal_store(gres[__builtin_nontemporal_load(&pos[i])], &out[i]);
+}
+
+struct GroupPlan {
+    uint32_t nb, tile, ntiles, nchunks, shift;
+    uint64_t off_hist, off_part, off_sums, off_gq, off_pos, off_gres, total;
+};
+
+static GroupPlan group_plan(const bpt_tree *t, uint64_t nq) {
+    GroupPlan p{};
+    const uint32_t bits = t->group_bits;
+    p.nb = 1u << bits;
+    p.shift = t->key_bytes * 8 - bits;
+    uint64_t tile = 4096;
+    while (tile < 65536 && tile * 2048 < nq) tile *= 2;
+    p.tile = (uint32_t)tile;
+    p.ntiles = (uint32_t)std::max<uint64_t>(1, (nq + tile - 1) / tile);
+    p.nchunks = (p.ntiles + kChunkTiles - 1) / kChunkTiles;
+    uint64_t off = 0;
+    p.off_hist = off; off += align256((uint64_t)p.ntiles * p.nb * 4);
+    p.off_part = off; off += align256((uint64_t)p.nchunks * p.nb * 4);
+    p.off_sums = off; off += align256((uint64_t)p.nchunks * p.nb * 4);
+    p.off_gq = off;   off += align256(nq * t->key_bytes);
+    p.off_pos = off;  off += align256(nq * 4);
+    p.off_gres = off; off += align256(nq * t->val_bytes);
+    p.total = off;
+    return p;
+}
+
+uint64_t point_grouped_ws(const bpt_tree *t, uint64_t nq) {
+    const uint64_t chunk = std::min<uint64_t>(nq, 1ull << 31);
+    return group_plan(t, chunk).total;
+}
+
+// Single-block scan over buckets using chunk sums kept in a second array:
+// sums[c][b] (chunk partial sums) -> part[c][b] = bucket base + chunk base.
+__global__ __launch_bounds__(1024) void k_col_scan(const uint32_t *__restrict__ sums,
+                                                   uint32_t *__restrict__ part, uint32_t nchunks,
+                                                   uint32_t nb) {
+    __shared__ uint32_t wsum[16];
+    const uint32_t per = (nb + 1023) / 1024;
+    uint32_t local = 0;
+    for (uint32_t k = 0; k < per; ++k) {
+        const uint32_t b = threadIdx.x * per + k;
+        if (b >= nb) break;
+        for (uint32_t c = 0; c < nchunks; ++c) local += sums[(uint64_t)c * nb + b];
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t x = local;
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (int i = 0; i < 16; ++i) { uint32_t v = wsum[i]; wsum[i] = acc; acc += v; }
+    }
+    __syncthreads();
+    uint32_t run = x - local + wsum[w];
+    for (uint32_t k = 0; k < per; ++k) {
+        const uint32_t b = threadIdx.x * per + k;
+        if (b >= nb) break;
+        for (uint32_t c = 0; c < nchunks; ++c) {
+            const uint64_t idx = (uint64_t)c * nb + b;
+            const uint32_t v = sums[idx];
+            part[idx] = run;
+            run += v;
+        }
+    }
+}
+
+template <class K, class V>
+static bpt_status grouped_chunk(const bpt_tree *t, const K *q, uint64_t nq, V *out, uint8_t *ws,
+                                hipStream_t s) {
+    const GroupPlan p = group_plan(t, nq);
+    uint32_t *hist = reinterpret_cast<uint32_t *>(ws + p.off_hist);
+    uint32_t *part = reinterpret_cast<uint32_t *>(ws + p.off_part);
+    K *gq = reinterpret_cast<K *>(ws + p.off_gq);
+    uint32_t *pos = reinterpret_cast<uint32_t *>(ws + p.off_pos);
+    V *gres = reinterpret_cast<V *>(ws + p.off_gres);
+    uint32_t *sums_buf = reinterpret_cast<uint32_t *>(ws + p.off_sums);
+    const size_t lds = (size_t)p.nb * 4;
+
+    k_tile_hist<K><<<p.ntiles, kGBlock, lds, s>>>(q, nq, p.shift, p.nb, p.tile, hist);
+    BPT_HIP(hipGetLastError());
+    dim3 g2((p.nb + 255) / 256, p.nchunks);
+    k_col_partial<<<g2, 256, 0, s>>>(hist, p.ntiles, p.nb, sums_buf);
+    BPT_HIP(hipGetLastError());
+    k_col_scan<<<1, 1024, 0, s>>>(sums_buf, part, p.nchunks, p.nb);
+    BPT_HIP(hipGetLastError());
+    k_col_final<<<g2, 256, 0, s>>>(hist, part, p.ntiles, p.nb);
+    BPT_HIP(hipGetLastError());
+    k_tile_scatter<K><<<p.ntiles, kGBlock, lds, s>>>(q, nq, p.shift, p.nb, p.tile, hist, gq, pos);
+    BPT_HIP(hipGetLastError());
+    BPT_TRY(launch_plain(t, gq, nq, gres, s));
+    k_ungather<V><<<search_grid(t, nq), 256, 0, s>>>(gres, pos, nq, out);
+    BPT_HIP(hipGetLastError());
+    return BPT_OK;
+}
+
+bpt_status point_grouped(const bpt_tree *t, const void *d_q, uint64_t nq, void *d_out, void *d_ws,
+                         uint64_t ws_bytes, hipStream_t s) {
+    if (nq == 0) return BPT_OK;
+    if (t->n == 0) return launch_plain(t, d_q, nq, d_out, s);
+    BPT_CHECK(t->group_bits <= 13, BPT_ERR_INVALID, "grouped search supports at most 13 group bits");
+    BPT_CHECK(d_ws && ws_bytes >= point_grouped_ws(t, nq), BPT_ERR_INVALID,
+              "grouped search needs a workspace of %llu bytes",
+              (unsigned long long)point_grouped_ws(t, nq));
+    return dispatch_tree(t, [&](auto kt, auto vt, auto, auto) -> bpt_status {
+        using K = typename decltype(kt)::type;
+        using V = typename decltype(vt)::type;
+        const uint64_t chunk = 1ull << 31;
+        for (uint64_t b = 0; b < nq; b += chunk) {
+            const uint64_t m = std::min(chunk, nq - b);
+            BPT_TRY(grouped_chunk<K, V>(t, (const K *)d_q + b, m, (V *)d_out + b, (uint8_t *)d_ws, s));
+        }
+        return BPT_OK;
+    });
+}
+
+// ==================================================================== stable LSD radix sort
+// Sorts (key, u32 value) pairs by key bits [lo_bit, hi_bit), 8 bits a pass.
+// Each pass: per-tile digit histogram (tile-major), exclusive scan in
+// digit-major order, stable scatter. Stability inside a tile comes from
+// processing 256-element rows in order with per-wave digit counts.
+constexpr int kSortBlock = 256;
+constexpr int kSortRows = 16;
+constexpr uint32_t kSortTile = kSortBlock * kSortRows;  // 4096
+
+template <class KT>
+__global__ __launch_bounds__(kSortBlock) void k_sort_hist(const KT *__restrict__ keys, uint64_t n,
+                                                          uint32_t shift, uint32_t mask,
+                                                          uint32_t ntiles, uint32_t *__restrict__ hist) {
+    __shared__ uint32_t cnt[256];
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
+    for (int r = 0; r < kSortRows; ++r) {
+        const uint64_t i = base + (uint64_t)r * kSortBlock + threadIdx.x;
+        if (i < n) atomicAdd(&cnt[(uint32_t)(keys[i] >> shift) & mask], 1u);
+    }
+    __syncthreads();
+    // digit-major: hist[d * ntiles + tile]
+    hist[(uint64_t)threadIdx.x * ntiles + blockIdx.x] = cnt[threadIdx.x];
+}
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+    const uint32_t lane = threadIdx.x & 63;
+    return lane ? (~0ull >> (64 - lane)) : 0ull;
+}
+
+template <class KT>
+__global__ __launch_bounds__(kSortBlock) void k_sort_scatter(const KT *__restrict__ keys_in,
+                                                             const uint32_t *__restrict__ vals_in,
+                                                             uint64_t n, uint32_t shift, uint32_t mask,
+                                                             uint32_t nbits, uint32_t ntiles,
+                                                             const uint32_t *__restrict__ offs,
+                                                             KT *__restrict__ keys_out,
+                                                             uint32_t *__restrict__ vals_out) {
+    constexpr int W = kSortBlock / 64;
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t wcnt[W][256];
+    run[threadIdx.x] = offs[(uint64_t)threadIdx.x * ntiles + blockIdx.x];
+    const int w = threadIdx.x >> 6;
+    const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
+    for (int r = 0; r < kSortRows; ++r) {
+        for (int k = 0; k < W; ++k) wcnt[k][threadIdx.x] = 0;
+        __syncthreads();
+        const uint64_t i = base + (uint64_t)r * kSortBlock + threadIdx.x;
+        const bool ok = i < n;
+        KT key = ok ? keys_in[i] : KT(0);
+        uint32_t d = ok ? ((uint32_t)(key >> shift) & mask) : 0xFFFFFFFFu;
+        // lanes of this wave with the same digit
+        uint64_t peers = __ballot(ok);
+        for (uint32_t b = 0; b < nbits; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        const uint32_t rank = __popcll(peers & lanemask_lt());
+        if (ok && rank == 0) wcnt[w][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        // thread t owns digit t: per-wave bases, then advance the running offset
+        {
+            uint32_t acc = run[threadIdx.x];
+            for (int k = 0; k < W; ++k) {
+                const uint32_t c = wcnt[k][threadIdx.x];
+                wcnt[k][threadIdx.x] = acc;
+                acc += c;
+            }
+            run[threadIdx.x] = acc;
+        }
+        __syncthreads();
+        if (ok) {
+            const uint32_t dst = wcnt[w][d] + rank;
+            keys_out[dst] = key;
+            vals_out[dst] = vals_in ? vals_in[i] : (uint32_t)i;
+        }
+        __syncthreads();
+    }
+}
+
+uint64_t sort_ws_bytes(uint64_t n, uint32_t key_bytes) {
+    const uint64_t ntiles = (n + kSortTile - 1) / kSortTile;
+    return 2 * align256(n * key_bytes) + 2 * align256(n * 4) + align256(ntiles * 256 * 4) +
+           scan_tmp_bytes(ntiles * 256, 4);
+}
+
+// Sorted values (original indices when vals_in is NULL) end in *vals_sorted,
+// keys in *keys_sorted; both point into ws.
+template <class KT>
+bpt_status sort_pairs(const KT *keys_in, const uint32_t *vals_in, uint64_t n, uint32_t lo_bit,
+                      uint32_t hi_bit, void *ws, uint64_t ws_bytes, KT **keys_sorted,
+                      uint32_t **vals_sorted, hipStream_t s) {
+    BPT_CHECK(n < (1ull << 32), BPT_ERR_CAPACITY, "sort of more than 2^32 elements");
+    BPT_CHECK(ws_bytes >= sort_ws_bytes(n, sizeof(KT)), BPT_ERR_INVALID, "sort workspace too small");
+    const uint64_t ntiles = (n + kSortTile - 1) / kSortTile;
+    uint8_t *p = reinterpret_cast<uint8_t *>(ws);
+    KT *kbuf[2];
+    uint32_t *vbuf[2];
+    kbuf[0] = reinterpret_cast<KT *>(p); p += align256(n * sizeof(KT));
+    kbuf[1] = reinterpret_cast<KT *>(p); p += align256(n * sizeof(KT));
+    vbuf[0] = reinterpret_cast<uint32_t *>(p); p += align256(n * 4);
+    vbuf[1] = reinterpret_cast<uint32_t *>(p); p += align256(n * 4);
+    uint32_t *hist = reinterpret_cast<uint32_t *>(p); p += align256(ntiles * 256 * 4);
+    void *scan_tmp = p;
+    const uint64_t scan_bytes = scan_tmp_bytes(ntiles * 256, 4);
+    const KT *kin = keys_in;
+    const uint32_t *vin = vals_in;
+    int cur = 0;
+    if (n == 0) { *keys_sorted = kbuf[0]; *vals_sorted = vbuf[0]; return BPT_OK; }
+    for (uint32_t b = lo_bit; b < hi_bit; b += 8) {
+        const uint32_t nbits = std::min<uint32_t>(8, hi_bit - b);
+        const uint32_t mask = (1u << nbits) - 1;
+        k_sort_hist<KT><<<(uint32_t)ntiles, kSortBlock, 0, s>>>(kin, n, b, mask, (uint32_t)ntiles, hist);
+        BPT_HIP(hipGetLastError());
+        BPT_TRY(exclusive_scan<uint32_t>(hist, ntiles * 256, hist, scan_tmp, scan_bytes, s));
+        k_sort_scatter<KT><<<(uint32_t)ntiles, kSortBlock, 0, s>>>(kin, vin, n, b, mask, nbits,
+                                                                  (uint32_t)ntiles, hist, kbuf[cur], vbuf[cur]);
+        BPT_HIP(hipGetLastError());
+        kin = kbuf[cur];
+        vin = vbuf[cur];
+        cur ^= 1;
+    }
+    if (vin == nullptr) {
+        // zero passes: identity permutation
+        BPT_CHECK(false, BPT_ERR_INVALID, "empty bit range");
+    }
+    *keys_sorted = const_cast<KT *>(kin);
+    *vals_sorted = const_cast<uint32_t *>(vin);
+    return BPT_OK;
+}
+
+template bpt_status sort_pairs<uint32_t>(const uint32_t *, const uint32_t *, uint64_t, uint32_t, uint32_t,
+                                         void *, uint64_t, uint32_t **, uint32_t **, hipStream_t);
+template bpt_status sort_pairs<uint64_t>(const uint64_t *, const uint32_t *, uint64_t, uint32_t, uint32_t,
+                                         void *, uint64_t, uint64_t **, uint32_t **, hipStream_t);
+
+// ==================================================================== keys
+__device__ __host__ __forceinline__ uint64_t spread32(uint32_t v) {
+    uint64_t x = v;
+    x = (x | (x << 16)) & 0x0000FFFF0000FFFFull;
+    x = (x | (x << 8)) & 0x00FF00FF00FF00FFull;
+    x = (x | (x << 4)) & 0x0F0F0F0F0F0F0F0Full;
+    x = (x | (x << 2)) & 0x3333333333333333ull;
+    x = (x | (x << 1)) & 0x5555555555555555ull;
+    return x;
+}
+
+__global__ void k_morton(const uint32_t *__restrict__ x, const uint32_t *__restrict__ y, uint64_t n,
+                         uint64_t *__restrict__ z) {
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step)
+        z[i] = spread32(x[i]) | (spread32(y[i]) << 1);
+}
+
+template <class KT>
+__global__ void k_max2(const KT *__restrict__ a, const KT *__restrict__ b, uint64_t n,
+                       unsigned long long *__restrict__ out) {
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    unsigned long long ma = 0, mb = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) {
+        ma = std::max<unsigned long long>(ma, (unsigned long long)a[i]);
+        mb = std::max<unsigned long long>(mb, (unsigned long long)b[i]);
+    }
+    for (int d = 32; d >= 1; d >>= 1) {
+        ma = std::max<unsigned long long>(ma, __shfl_xor(ma, d, 64));
+        mb = std::max<unsigned long long>(mb, __shfl_xor(mb, d, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(&out[0], ma);
+        atomicMax(&out[1], mb);
+    }
+}
+
+template <class KT>
+__global__ void k_zorder_keys(const KT *__restrict__ a, const KT *__restrict__ b, uint64_t n,
+                              uint64_t ix, uint64_t iy, uint64_t *__restrict__ out) {
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) {
+        const uint32_t xg = (uint32_t)((uint64_t)a[i] / ix);
+        const uint32_t yg = (uint32_t)((uint64_t)b[i] / iy);
+        out[i] = spread32(xg) | (spread32(yg) << 1);
+    }
+}
+
+__device__ __forceinline__ uint32_t bucket2d(uint64_t a, uint64_t b, int pass) {
+    // grouping2d.cu:24-31: BIT_PASS 3 -> 22 passes of a 64-bit key, 4-bit nibbles
+    const int shift = 3 * (21 - pass);
+    if (pass > 15 || shift < 0 || shift >= 64) return 0;
+    return (uint32_t)(((a >> shift) & 0xF) * ((b >> shift) & 0xF));
+}
+
+template <class KT>
+__global__ void k_group2d_keys(const KT *__restrict__ a, const KT *__restrict__ b, uint64_t n,
+                               int pass0, uint32_t *__restrict__ out) {
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) {
+        const uint64_t x = a[i], y = b[i];
+        out[i] = (bucket2d(x, y, pass0) << 24) | (bucket2d(x, y, pass0 + 1) << 16) |
+                 (bucket2d(x, y, pass0 + 2) << 8) | bucket2d(x, y, pass0 + 3);
+    }
+}
+
+static int bitwidth(uint64_t v) { int b = 0; while (v) { v >>= 1; ++b; } return b; }
+
+template <class KT>
+bpt_status pair_max(const KT *a, const KT *b, uint64_t n, uint64_t *ma, uint64_t *mb, hipStream_t s) {
+    unsigned long long *d = nullptr;
+    BPT_HIP(hipMalloc(&d, 16));
+    unsigned long long h[2] = {0, 0};
+    hipError_t e = hipMemsetAsync(d, 0, 16, s);
+    if (e == hipSuccess) {
+        uint32_t g = (uint32_t)std::min<uint64_t>((n + 255) / 256, (uint64_t)device_cu_count() * 8);
+        k_max2<KT><<<std::max<uint32_t>(g, 1), 256, 0, s>>>(a, b, n, d);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(h, d, 16, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(d);
+    if (e != hipSuccess) return hip_fail(e, "pair max", __FILE__, __LINE__);
+    *ma = h[0];
+    *mb = h[1];
+    return BPT_OK;
+}
+
+// zOrder2d.cu:129-130
+static uint64_t zorder_interval(uint64_t m) {
+    float r = (float)m / 4294967296.0f;
+    int iv = (int)ceilf(r);
+    return iv < 1 ? 1 : (uint64_t)iv;
+}
+
+// grouping2d.cu:610-635
+static int group2d_start_pass(uint64_t ma, uint64_t mb) {
+    int a = 16 - (int)ceilf((float)bitwidth(ma) / 3.0f);
+    int b = 16 - (int)ceilf((float)bitwidth(mb) / 3.0f);
+    return a > b ? a : b;
+}
+
+template <class KT>
+bpt_status zorder_keys(const KT *a, const KT *b, uint64_t n, uint64_t *out, hipStream_t s) {
+    if (n == 0) return BPT_OK;
+    uint64_t ma, mb;
+    BPT_TRY(pair_max<KT>(a, b, n, &ma, &mb, s));
+    uint32_t g = (uint32_t)std::min<uint64_t>((n + 255) / 256, (uint64_t)device_cu_count() * 16);
+    k_zorder_keys<KT><<<g, 256, 0, s>>>(a, b, n, zorder_interval(ma), zorder_interval(mb), out);
+    BPT_HIP(hipGetLastError());
+    return BPT_OK;
+}
+
+template <class KT>
+bpt_status group2d_keys(const KT *a, const KT *b, uint64_t n, uint32_t *out, hipStream_t s) {
+    if (n == 0) return BPT_OK;
+    uint64_t ma, mb;
+    BPT_TRY(pair_max<KT>(a, b, n, &ma, &mb, s));
+    int p0 = group2d_start_pass(ma, mb);
+    if (p0 < 0) p0 = 0;  // passes below 0 shift by 64 or more: every bucket is 0
+    uint32_t g = (uint32_t)std::min<uint64_t>((n + 255) / 256, (uint64_t)device_cu_count() * 16);
+    k_group2d_keys<KT><<<g, 256, 0, s>>>(a, b, n, p0, out);
+    BPT_HIP(hipGetLastError());
+    return BPT_OK;
+}
+
+template bpt_status zorder_keys<uint32_t>(const uint32_t *, const uint32_t *, uint64_t, uint64_t *, hipStream_t);
+template bpt_status zorder_keys<uint64_t>(const uint64_t *, const uint64_t *, uint64_t, uint64_t *, hipStream_t);
+template bpt_status group2d_keys<uint32_t>(const uint32_t *, const uint32_t *, uint64_t, uint32_t *, hipStream_t);
+template bpt_status group2d_keys<uint64_t>(const uint64_t *, const uint64_t *, uint64_t, uint32_t *, hipStream_t);
+
+// Order of n (lo, hi) queries under a grouping mode: *order points into ws.
+uint64_t query_order_ws(uint64_t n) {
+    return align256(n * 8) + sort_ws_bytes(n, 4);
+}
+
+template <class KT>
+bpt_status query_order(const KT *lo, const KT *hi, uint64_t n, int mode, void *ws, uint64_t ws_bytes,
+                       uint32_t **order, hipStream_t s) {
+    BPT_CHECK(ws_bytes >= query_order_ws(n), BPT_ERR_INVALID, "order workspace too small");
+    uint8_t *p = reinterpret_cast<uint8_t *>(ws);
+    uint8_t *sort_ws = p + align256(n * 8);
+    uint32_t *ksorted = nullptr;
+    if (mode == BPT_GROUP_GRID2D) {
+        uint32_t *keys = reinterpret_cast<uint32_t *>(p);
+        BPT_TRY(group2d_keys<KT>(lo, hi, n, keys, s));
+        return sort_pairs<uint32_t>(keys, nullptr, n, 0, 32, sort_ws, ws_bytes - align256(n * 8), &ksorted,
+                                    order, s);
+    }
+    if (mode == BPT_GROUP_ZORDER) {
+        uint64_t *z = reinterpret_cast<uint64_t *>(p);
+        BPT_TRY(zorder_keys<KT>(lo, hi, n, z, s));
+        // the top 32 bits of the 64-bit Z key order the groups
+        uint64_t *zs = nullptr;
+        return sort_pairs<uint64_t>(z, nullptr, n, 32, 64, sort_ws, ws_bytes - align256(n * 8), &zs, order, s);
+    }
+    set_error("unknown group mode %d", mode);
+    return BPT_ERR_INVALID;
+}
+
+template bpt_status query_order<uint32_t>(const uint32_t *, const uint32_t *, uint64_t, int, void *, uint64_t,
+                                          uint32_t **, hipStream_t);
+template bpt_status query_order<uint64_t>(const uint64_t *, const uint64_t *, uint64_t, int, void *, uint64_t,
+                                          uint32_t **, hipStream_t);
+
+uint64_t query_order_ws_any(uint64_t n) {
+    return align256(n * 8) + std::max(sort_ws_bytes(n, 4), sort_ws_bytes(n, 8));
+}
+
+}  // namespace bpt
+
+using namespace bpt;
+
+extern "C" {
+
+bpt_status bpt_radix_group(const void *d_keys, uint32_t key_bytes, uint64_t n, uint32_t shift,
+                           uint32_t bits, uint32_t *d_perm, uint64_t *d_bucket_start, bpt_stream stream) {
+    BPT_CHECK(key_bytes == 4 || key_bytes == 8, BPT_ERR_INVALID, "key_bytes %u", key_bytes);
+    BPT_CHECK(bits >= 1 && bits <= 16 && shift + bits <= key_bytes * 8, BPT_ERR_INVALID,
+              "bucket bits [%u, %u) outside the key", shift, shift + bits);
+    BPT_CHECK(n == 0 || (d_keys && d_perm), BPT_ERR_INVALID, "NULL argument");
+    hipStream_t s = as_stream(stream);
+    const uint32_t nb = 1u << bits;
+    if (n == 0) {
+        if (d_bucket_start) BPT_HIP(hipMemsetAsync(d_bucket_start, 0, (nb + 1) * 8ull, s));
+        BPT_HIP(hipStreamSynchronize(s));
+        return BPT_OK;
+    }
+    const uint64_t ws_bytes = sort_ws_bytes(n, key_bytes);
+    void *ws = nullptr;
+    BPT_HIP(hipMalloc(&ws, ws_bytes));
+    bpt_status st;
+    uint64_t *bucket_keys_u64 = nullptr;
+    uint32_t *perm = nullptr;
+    if (key_bytes == 4) {
+        uint32_t *ks = nullptr;
+        st = sort_pairs<uint32_t>((const uint32_t *)d_keys, nullptr, n, shift, shift + bits, ws, ws_bytes, &ks,
+                                  &perm, s);
+    } else {
+        uint64_t *ks = nullptr;
+        st = sort_pairs<uint64_t>((const uint64_t *)d_keys, nullptr, n, shift, shift + bits, ws, ws_bytes, &ks,
+                                  &perm, s);
+        bucket_keys_u64 = ks;
+    }
+    (void)bucket_keys_u64;
+    if (st == BPT_OK) {
+        hipError_t e = hipMemcpyAsync(d_perm, perm, n * 4, hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) st = hip_fail(e, "perm copy", __FILE__, __LINE__);
+    }
+    if (st == BPT_OK && d_bucket_start) {
+        // bucket starts = lower bound of each bucket in the sorted digit sequence
+        std::vector<uint64_t> h(nb + 1, 0);
+        std::vector<uint8_t> keys_host;
+        keys_host.resize(n * key_bytes);
+        hipError_t e = hipMemcpyAsync(keys_host.data(), d_keys, n * key_bytes, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) st = hip_fail(e, "bucket count copy", __FILE__, __LINE__);
+        if (st == BPT_OK) {
+            for (uint64_t i = 0; i < n; ++i) {
+                uint64_t k = key_bytes == 4 ? ((const uint32_t *)keys_host.data())[i]
+                                            : ((const uint64_t *)keys_host.data())[i];
+                h[((k >> shift) & (nb - 1)) + 1]++;
+            }
+            for (uint32_t b = 0; b < nb; ++b) h[b + 1] += h[b];
+            e = hipMemcpyAsync(d_bucket_start, h.data(), (nb + 1) * 8ull, hipMemcpyHostToDevice, s);
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            if (e != hipSuccess) st = hip_fail(e, "bucket start copy", __FILE__, __LINE__);
+        }
+    }
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(ws);
+    return st;
+}
+
+bpt_status bpt_morton_encode(const uint32_t *d_x, const uint32_t *d_y, uint64_t n, uint64_t *d_z,
+                             bpt_stream stream) {
+    BPT_CHECK(n == 0 || (d_x && d_y && d_z), BPT_ERR_INVALID, "NULL argument");
+    if (n == 0) return BPT_OK;
+    uint32_t g = (uint32_t)std::min<uint64_t>((n + 255) / 256, (uint64_t)device_cu_count() * 16);
+    k_morton<<<g, 256, 0, as_stream(stream)>>>(d_x, d_y, n, d_z);
+    BPT_HIP(hipGetLastError());
+    return BPT_OK;
+}
+
+bpt_status bpt_zorder2d(const uint64_t *d_a, const uint64_t *d_b, uint64_t n, uint64_t *d_out,
+                        bpt_stream stream) {
+    BPT_CHECK(n == 0 || (d_a && d_b && d_out), BPT_ERR_INVALID, "NULL argument");
+    BPT_TRY(zorder_keys<uint64_t>(d_a, d_b, n, d_out, as_stream(stream)));
+    BPT_HIP(hipStreamSynchronize(as_stream(stream)));
+    return BPT_OK;
+}
+
+bpt_status bpt_group2d_keys(const uint64_t *d_a, const uint64_t *d_b, uint64_t n, uint32_t *d_out,
+                            bpt_stream stream) {
+    BPT_CHECK(n == 0 || (d_a && d_b && d_out), BPT_ERR_INVALID, "NULL argument");
+    BPT_TRY(group2d_keys<uint64_t>(d_a, d_b, n, d_out, as_stream(stream)));
+    BPT_HIP(hipStreamSynchronize(as_stream(stream)));
+    return BPT_OK;
+}
+
+bpt_status bpt_range_count_grouped(const bpt_tree *t, const void *d_lo, const void *d_hi, uint64_t nq,
+                                   int group_mode, uint64_t *d_first, uint64_t *d_count, bpt_stream stream) {
+    BPT_CHECK(t, BPT_ERR_INVALID, "tree is NULL");
+    BPT_CHECK(nq == 0 || (d_lo && d_hi && d_count), BPT_ERR_INVALID, "NULL argument");
+    hipStream_t s = as_stream(stream);
+    if (group_mode == BPT_GROUP_NONE || nq == 0)
+        return range_count_ordered(t, d_lo, d_hi, nq, nullptr, d_first, d_count, s);
+    BPT_CHECK(group_mode == BPT_GROUP_GRID2D || group_mode == BPT_GROUP_ZORDER, BPT_ERR_INVALID,
+              "unknown group mode %d", group_mode);
+    BPT_CHECK(nq < (1ull << 32), BPT_ERR_CAPACITY, "grouped range search of more than 2^32 queries");
+    const uint64_t ws_bytes = query_order_ws_any(nq);
+    void *ws = nullptr;
+    BPT_HIP(hipMalloc(&ws, ws_bytes));
+    uint32_t *order = nullptr;
+    bpt_status st = t->key_bytes == 4
+                        ? query_order<uint32_t>((const uint32_t *)d_lo, (const uint32_t *)d_hi, nq, group_mode,
+                                                ws, ws_bytes, &order, s)
+                        : query_order<uint64_t>((const uint64_t *)d_lo, (const uint64_t *)d_hi, nq, group_mode,
+                                                ws, ws_bytes, &order, s);
+    if (st == BPT_OK) st = range_count_ordered(t, d_lo, d_hi, nq, order, d_first, d_count, s);
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(ws);
+    return st;
+}
+
+}  // extern "C"

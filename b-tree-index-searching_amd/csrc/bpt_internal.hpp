@@ -1,0 +1,268 @@
+// Internal declarations shared by the HIP sources of libbptree_amd.
+// gfx950 (CDNA4) only: 64-lane wavefronts, 160 KiB LDS per CU, 128 B L2 lines.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <mutex>
+#include <string>
+#include <type_traits>
+#include <utility>
+
+#include "bptree.h"
+
+namespace bpt {
+
+// ---------------------------------------------------------------- errors
+void set_error(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
+bpt_status hip_fail(hipError_t e, const char *what, const char *file, int line);
+
+#define BPT_HIP(...)                                                         \
+    do {                                                                     \
+        hipError_t e_ = (__VA_ARGS__);                                       \
+        if (e_ != hipSuccess) return ::bpt::hip_fail(e_, #__VA_ARGS__, __FILE__, __LINE__); \
+    } while (0)
+
+#define BPT_CHECK(cond, status, ...)                                         \
+    do {                                                                     \
+        if (!(cond)) { ::bpt::set_error(__VA_ARGS__); return (status); }     \
+    } while (0)
+
+#define BPT_TRY(...)                                                         \
+    do {                                                                     \
+        bpt_status s_ = (__VA_ARGS__);                                       \
+        if (s_ != BPT_OK) return s_;                                         \
+    } while (0)
+
+inline hipStream_t as_stream(bpt_stream s) { return reinterpret_cast<hipStream_t>(s); }
+
+int device_cu_count();  // CUs of the current device (cached per device)
+
+// ---------------------------------------------------------------- layout
+constexpr int kMaxLevels = BPT_MAX_LEVELS;
+
+// What a search kernel needs to walk a tree; passed by value (kernarg).
+struct TreeView {
+    const uint8_t *inner;   // internal levels, top (root) level first
+    const uint8_t *leaves;  // leaf lines
+    uint64_t n;             // keys
+    uint64_t leaf_stride;   // bytes per leaf
+    int32_t height;         // internal levels (0: the single leaf is the root)
+    int32_t lds_levels;     // top internal levels copied into LDS
+    uint32_t lds_bytes;     // bytes of `inner` copied into LDS (a prefix)
+    uint32_t pad_;
+    uint64_t count[kMaxLevels + 1];  // nodes per level, [0] = leaves
+    uint64_t off[kMaxLevels + 1];    // byte offset of level k inside `inner`
+};
+
+struct Workspace {
+    void *ptr = nullptr;
+    uint64_t bytes = 0;
+    ~Workspace();
+    bpt_status reserve(uint64_t need);
+};
+
+struct LearnedModel;  // learned position model (bpt_learned.hip)
+
+}  // namespace bpt
+
+struct bpt_tree {
+    uint32_t key_bytes = 4, val_bytes = 4, fanout = 32, leaf_cap = 16;
+    uint64_t n = 0;
+    uint64_t leaf_stride = 128;
+    uint8_t *d_leaves = nullptr;
+    uint8_t *d_inner = nullptr;
+    uint64_t leaf_bytes = 0, inner_bytes = 0;
+    int device = 0;
+    uint32_t block = 256, blocks_per_cu = 8;
+    uint32_t group_bits = 12;
+    uint32_t lds_budget = 32768;
+    bpt::TreeView view{};
+    bpt::Workspace ws;             // grouped / learned scratch owned by the tree
+    bpt::LearnedModel *learned = nullptr;
+};
+
+namespace bpt {
+
+// ---------------------------------------------------------------- dispatch
+template <int N> using IC = std::integral_constant<int, N>;
+template <class T> struct Tag { using type = T; };
+
+// Calls fn(Tag<K>, Tag<V>, IC<F>, IC<LF>) for the tree's configuration.
+template <class Fn>
+bpt_status dispatch_tree(const bpt_tree *t, Fn &&fn) {
+    auto with_f_lf = [&](auto kt, auto vt) -> bpt_status {
+        auto with_lf = [&](auto fc) -> bpt_status {
+            switch (t->leaf_cap) {
+                case 8: return fn(kt, vt, fc, IC<8>{});
+                case 16: return fn(kt, vt, fc, IC<16>{});
+                case 32: return fn(kt, vt, fc, IC<32>{});
+            }
+            set_error("unsupported leaf_cap %u", t->leaf_cap);
+            return BPT_ERR_INVALID;
+        };
+        switch (t->fanout) {
+            case 16: return with_lf(IC<16>{});
+            case 32: return with_lf(IC<32>{});
+            case 64: return with_lf(IC<64>{});
+        }
+        set_error("unsupported fanout %u", t->fanout);
+        return BPT_ERR_INVALID;
+    };
+    if (t->key_bytes == 4 && t->val_bytes == 4) return with_f_lf(Tag<uint32_t>{}, Tag<uint32_t>{});
+    if (t->key_bytes == 8 && t->val_bytes == 8) return with_f_lf(Tag<uint64_t>{}, Tag<uint64_t>{});
+    if (t->key_bytes == 8 && t->val_bytes == 4) return with_f_lf(Tag<uint64_t>{}, Tag<uint32_t>{});
+    set_error("unsupported key/value widths %u/%u", t->key_bytes, t->val_bytes);
+    return BPT_ERR_INVALID;
+}
+
+template <class K> __host__ __device__ constexpr K key_max() { return ~K(0); }
+
+// ---------------------------------------------------------------- device helpers
+// Number of separators in an F-wide node that are < q. The node is
+// F * sizeof(K) bytes, 16-byte aligned; it is read with 128-bit loads, all
+// issued before any compare so the loads of one node overlap.
+template <class K, int F>
+__device__ __forceinline__ uint32_t node_rank(const K *__restrict__ node, K q) {
+    constexpr int NV = F * (int)sizeof(K) / 16;
+    const uint4 *p = reinterpret_cast<const uint4 *>(node);
+    uint4 w[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) w[v] = p[v];
+    uint32_t c = 0;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        if constexpr (sizeof(K) == 4) {
+            c += (w[v].x < q) + (w[v].y < q) + (w[v].z < q) + (w[v].w < q);
+        } else {
+            uint64_t a = (uint64_t)w[v].x | ((uint64_t)w[v].y << 32);
+            uint64_t b = (uint64_t)w[v].z | ((uint64_t)w[v].w << 32);
+            c += (a < q) + (b < q);
+        }
+    }
+    return c;
+}
+
+// Leaf node index of q: walks the LDS-resident top levels, then the
+// global levels. `lds` holds the first t.lds_bytes of t.inner.
+template <class K, int F>
+__device__ __forceinline__ uint64_t descend(const TreeView &t, const uint8_t *lds, K q) {
+    uint64_t node = 0;
+    int lvl = t.height;
+    const int lds_stop = t.height - t.lds_levels;
+    for (; lvl > lds_stop; --lvl) {
+        const K *nd = reinterpret_cast<const K *>(lds + t.off[lvl]) + node * F;
+        uint64_t c = node * F + node_rank<K, F>(nd, q);
+        uint64_t last = t.count[lvl - 1] - 1;
+        node = c < last ? c : last;
+    }
+    for (; lvl >= 1; --lvl) {
+        const K *nd = reinterpret_cast<const K *>(t.inner + t.off[lvl]) + node * F;
+        uint64_t c = node * F + node_rank<K, F>(nd, q);
+        uint64_t last = t.count[lvl - 1] - 1;
+        node = c < last ? c : last;
+    }
+    return node;
+}
+
+// Loads the keys of a leaf into registers.
+template <class K, int LF>
+__device__ __forceinline__ void load_leaf_keys(const uint8_t *leaf, K (&k)[LF]) {
+    constexpr int NV = LF * (int)sizeof(K) / 16;
+    const uint4 *p = reinterpret_cast<const uint4 *>(leaf);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        uint4 w = p[v];
+        if constexpr (sizeof(K) == 4) {
+            k[4 * v] = w.x; k[4 * v + 1] = w.y; k[4 * v + 2] = w.z; k[4 * v + 3] = w.w;
+        } else {
+            k[2 * v] = (uint64_t)w.x | ((uint64_t)w.y << 32);
+            k[2 * v + 1] = (uint64_t)w.z | ((uint64_t)w.w << 32);
+        }
+    }
+}
+
+template <class V, int LF>
+__device__ __forceinline__ void load_leaf_vals(const uint8_t *vals, V (&v)[LF]) {
+    constexpr int NV = LF * (int)sizeof(V) / 16;
+    const uint4 *p = reinterpret_cast<const uint4 *>(vals);
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        uint4 w = p[i];
+        if constexpr (sizeof(V) == 4) {
+            v[4 * i] = w.x; v[4 * i + 1] = w.y; v[4 * i + 2] = w.z; v[4 * i + 3] = w.w;
+        } else {
+            v[2 * i] = (uint64_t)w.x | ((uint64_t)w.y << 32);
+            v[2 * i + 1] = (uint64_t)w.z | ((uint64_t)w.w << 32);
+        }
+    }
+}
+
+// Exact lookup of q in leaf `leaf`: value of the first slot equal to q
+// (slots past n never match), else `sentinel`.
+template <class K, class V, int LF>
+__device__ __forceinline__ V leaf_lookup(const TreeView &t, uint64_t leaf, K q, V sentinel) {
+    const uint8_t *lp = t.leaves + leaf * t.leaf_stride;
+    K k[LF];
+    V v[LF];
+    load_leaf_keys<K, LF>(lp, k);
+    load_leaf_vals<V, LF>(lp + LF * sizeof(K), v);
+    const uint64_t base = leaf * LF;
+    V res = sentinel;
+#pragma unroll
+    for (int j = LF - 1; j >= 0; --j)
+        if (k[j] == q && base + j < t.n) res = v[j];
+    return res;
+}
+
+// Global lower_bound position of q inside leaf `leaf`.
+template <class K, int LF>
+__device__ __forceinline__ uint64_t leaf_lower_bound(const TreeView &t, uint64_t leaf, K q) {
+    const uint8_t *lp = t.leaves + leaf * t.leaf_stride;
+    K k[LF];
+    load_leaf_keys<K, LF>(lp, k);
+    uint32_t c = 0;
+#pragma unroll
+    for (int j = 0; j < LF; ++j) c += (k[j] < q);
+    uint64_t p = leaf * LF + c;
+    return p < t.n ? p : t.n;
+}
+
+// Slot i of the sorted key / value sequence, read from the leaf layout.
+template <class K, int LF>
+__device__ __forceinline__ K slot_key(const TreeView &t, uint64_t i) {
+    return *reinterpret_cast<const K *>(t.leaves + (i / LF) * t.leaf_stride + (i % LF) * sizeof(K));
+}
+template <class K, class V, int LF>
+__device__ __forceinline__ V slot_val(const TreeView &t, uint64_t i) {
+    return *reinterpret_cast<const V *>(t.leaves + (i / LF) * t.leaf_stride + LF * sizeof(K) +
+                                        (i % LF) * sizeof(V));
+}
+
+// Copies the LDS prefix of the internal levels into shared memory.
+__device__ __forceinline__ void stage_top_levels(const TreeView &t, uint8_t *lds) {
+    const uint4 *src = reinterpret_cast<const uint4 *>(t.inner);
+    uint4 *dst = reinterpret_cast<uint4 *>(lds);
+    for (uint32_t i = threadIdx.x; i < t.lds_bytes / 16; i += blockDim.x) dst[i] = src[i];
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------- host helpers
+uint32_t search_grid(const bpt_tree *t, uint64_t work_items);
+bpt_status launch_plain(const bpt_tree *t, const void *d_q, uint64_t nq, void *d_out,
+                        hipStream_t s);
+template <class T>
+bpt_status exclusive_scan(const T *d_in, uint64_t n, T *d_out, void *d_tmp, uint64_t tmp_bytes,
+                          hipStream_t s);
+uint64_t scan_tmp_bytes(uint64_t n, uint64_t elem_bytes);
+bpt_status point_grouped(const bpt_tree *t, const void *d_q, uint64_t nq, void *d_out,
+                         void *d_ws, uint64_t ws_bytes, hipStream_t s);
+uint64_t point_grouped_ws(const bpt_tree *t, uint64_t nq);
+bpt_status point_learned(const bpt_tree *t, const void *d_q, uint64_t nq, void *d_out,
+                         void *d_ws, uint64_t ws_bytes, hipStream_t s);
+uint64_t point_learned_ws(const bpt_tree *t, uint64_t nq);
+void learned_free(LearnedModel *m);
+
+}  // namespace bpt

@@ -1,0 +1,192 @@
+// Error reporting, device plumbing and timers of the C ABI.
+#include <cstdarg>
+#include <cstring>
+#include <vector>
+
+#include "bpt_internal.hpp"
+
+namespace bpt {
+
+static thread_local std::string g_last_error;
+
+void set_error(const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+}
+
+bpt_status hip_fail(hipError_t e, const char *what, const char *file, int line) {
+    set_error("%s failed: %s (%d) at %s:%d", what, hipGetErrorString(e), (int)e, file, line);
+    (void)hipGetLastError();  // clear the sticky non-fatal error
+    return (e == hipErrorOutOfMemory) ? BPT_ERR_NOMEM : BPT_ERR_HIP;
+}
+
+int device_cu_count() {
+    static std::mutex mu;
+    static std::vector<int> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    std::lock_guard<std::mutex> g(mu);
+    if ((int)cache.size() <= dev) cache.resize(dev + 1, 0);
+    if (cache[dev] == 0) {
+        int cu = 0;
+        if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cu <= 0)
+            cu = 256;
+        cache[dev] = cu;
+    }
+    return cache[dev];
+}
+
+Workspace::~Workspace() {
+    if (ptr) (void)hipFree(ptr);
+}
+
+bpt_status Workspace::reserve(uint64_t need) {
+    if (need <= bytes) return BPT_OK;
+    if (ptr) {
+        BPT_HIP(hipDeviceSynchronize());
+        BPT_HIP(hipFree(ptr));
+        ptr = nullptr;
+        bytes = 0;
+    }
+    BPT_HIP(hipMalloc(&ptr, need));
+    bytes = need;
+    return BPT_OK;
+}
+
+}  // namespace bpt
+
+struct bpt_timer {
+    hipEvent_t a = nullptr, b = nullptr;
+};
+
+extern "C" {
+
+const char *bpt_last_error(void) { return bpt::g_last_error.c_str(); }
+const char *bpt_version(void) { return "bptree_amd 0.1 (gfx950)"; }
+
+bpt_status bpt_device_count(int *count) {
+    BPT_CHECK(count, BPT_ERR_INVALID, "count is NULL");
+    BPT_HIP(hipGetDeviceCount(count));
+    return BPT_OK;
+}
+
+bpt_status bpt_device_info(int device, int *cu_count, uint64_t *hbm_bytes, char *name, int name_len) {
+    hipDeviceProp_t p;
+    BPT_HIP(hipGetDeviceProperties(&p, device));
+    if (cu_count) *cu_count = p.multiProcessorCount;
+    if (hbm_bytes) *hbm_bytes = p.totalGlobalMem;
+    if (name && name_len > 0) {
+        snprintf(name, (size_t)name_len, "%s (%s)", p.name, p.gcnArchName);
+    }
+    return BPT_OK;
+}
+
+bpt_status bpt_device_alloc(uint64_t bytes, void **d_ptr) {
+    BPT_CHECK(d_ptr, BPT_ERR_INVALID, "d_ptr is NULL");
+    *d_ptr = nullptr;
+    if (bytes == 0) return BPT_OK;
+    BPT_HIP(hipMalloc(d_ptr, bytes));
+    return BPT_OK;
+}
+
+bpt_status bpt_device_free(void *d_ptr) {
+    if (d_ptr) BPT_HIP(hipFree(d_ptr));
+    return BPT_OK;
+}
+
+bpt_status bpt_memset(void *d_ptr, int value, uint64_t bytes, bpt_stream s) {
+    if (bytes == 0) return BPT_OK;
+    BPT_HIP(hipMemsetAsync(d_ptr, value, bytes, bpt::as_stream(s)));
+    return BPT_OK;
+}
+
+bpt_status bpt_copy_h2d(void *d_dst, const void *h_src, uint64_t bytes, bpt_stream s) {
+    if (bytes == 0) return BPT_OK;
+    BPT_HIP(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, bpt::as_stream(s)));
+    BPT_HIP(hipStreamSynchronize(bpt::as_stream(s)));
+    return BPT_OK;
+}
+
+bpt_status bpt_copy_d2h(void *h_dst, const void *d_src, uint64_t bytes, bpt_stream s) {
+    if (bytes == 0) return BPT_OK;
+    BPT_HIP(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, bpt::as_stream(s)));
+    BPT_HIP(hipStreamSynchronize(bpt::as_stream(s)));
+    return BPT_OK;
+}
+
+bpt_status bpt_copy_d2d(void *d_dst, const void *d_src, uint64_t bytes, bpt_stream s) {
+    if (bytes == 0) return BPT_OK;
+    BPT_HIP(hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, bpt::as_stream(s)));
+    BPT_HIP(hipStreamSynchronize(bpt::as_stream(s)));
+    return BPT_OK;
+}
+
+bpt_status bpt_stream_create(bpt_stream *s) {
+    BPT_CHECK(s, BPT_ERR_INVALID, "s is NULL");
+    hipStream_t h;
+    BPT_HIP(hipStreamCreateWithFlags(&h, hipStreamNonBlocking));
+    *s = reinterpret_cast<bpt_stream>(h);
+    return BPT_OK;
+}
+
+bpt_status bpt_stream_destroy(bpt_stream s) {
+    if (s) BPT_HIP(hipStreamDestroy(bpt::as_stream(s)));
+    return BPT_OK;
+}
+
+bpt_status bpt_stream_sync(bpt_stream s) {
+    BPT_HIP(hipStreamSynchronize(bpt::as_stream(s)));
+    return BPT_OK;
+}
+
+bpt_status bpt_device_sync(void) {
+    BPT_HIP(hipDeviceSynchronize());
+    return BPT_OK;
+}
+
+bpt_status bpt_timer_create(bpt_timer **t) {
+    BPT_CHECK(t, BPT_ERR_INVALID, "t is NULL");
+    auto *x = new bpt_timer;
+    hipError_t e = hipEventCreate(&x->a);
+    if (e == hipSuccess) e = hipEventCreate(&x->b);
+    if (e != hipSuccess) {
+        if (x->a) (void)hipEventDestroy(x->a);
+        delete x;
+        return bpt::hip_fail(e, "hipEventCreate", __FILE__, __LINE__);
+    }
+    *t = x;
+    return BPT_OK;
+}
+
+bpt_status bpt_timer_destroy(bpt_timer *t) {
+    if (!t) return BPT_OK;
+    (void)hipEventDestroy(t->a);
+    (void)hipEventDestroy(t->b);
+    delete t;
+    return BPT_OK;
+}
+
+bpt_status bpt_timer_start(bpt_timer *t, bpt_stream s) {
+    BPT_CHECK(t, BPT_ERR_INVALID, "timer is NULL");
+    BPT_HIP(hipEventRecord(t->a, bpt::as_stream(s)));
+    return BPT_OK;
+}
+
+bpt_status bpt_timer_stop(bpt_timer *t, bpt_stream s) {
+    BPT_CHECK(t, BPT_ERR_INVALID, "timer is NULL");
+    BPT_HIP(hipEventRecord(t->b, bpt::as_stream(s)));
+    return BPT_OK;
+}
+
+bpt_status bpt_timer_ms(bpt_timer *t, float *ms) {
+    BPT_CHECK(t && ms, BPT_ERR_INVALID, "NULL argument");
+    BPT_HIP(hipEventSynchronize(t->b));
+    BPT_HIP(hipEventElapsedTime(ms, t->a, t->b));
+    return BPT_OK;
+}
+
+}  // extern "C"

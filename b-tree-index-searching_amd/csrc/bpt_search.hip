@@ -1,0 +1,359 @@
+// Plain point search, lower bound, 1-D range count / ids, exclusive scan.
+//
+// Plain search replaces kernelSearchingNaiveOneStep (reference
+// BPTreeExactSearch.cu:25-105): one query per lane, root-to-leaf descent over
+// child maxima, exact match in the leaf, sentinel on a miss. The reference
+// binary-searches 5000-key pages with ~40 dependent 8-byte probes per query;
+// here a node is one 128-byte line read with eight independent 16-byte loads
+// and ranked with branch-free compares, the top levels come from LDS, and a
+// lookup costs one dependent line per global level plus one leaf line.
+#include <algorithm>
+
+#include "bpt_internal.hpp"
+
+namespace bpt {
+
+template <class K, class V, int F, int LF>
+__global__ __launch_bounds__(256) void k_point_plain(TreeView t, const K *__restrict__ q,
+                                                     V *__restrict__ out, uint64_t nq, V sentinel) {
+    extern __shared__ uint4 lds_raw[];
+    uint8_t *lds = reinterpret_cast<uint8_t *>(lds_raw);
+    stage_top_levels(t, lds);
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += step) {
+        const K key = __builtin_nontemporal_load(&q[i]);
+        const uint64_t leaf = descend<K, F>(t, lds, key);
+        const V r = leaf_lookup<K, V, LF>(t, leaf, key, sentinel);
+        __builtin_nontemporal_store(r, &out[i]);
+    }
+}
+
+template <class K, int F, int LF>
+__global__ __launch_bounds__(256) void k_lower_bound(TreeView t, const K *__restrict__ q,
+                                                     uint64_t *__restrict__ pos, uint64_t nq) {
+    extern __shared__ uint4 lds_raw[];
+    uint8_t *lds = reinterpret_cast<uint8_t *>(lds_raw);
+    stage_top_levels(t, lds);
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += step) {
+        const K key = q[i];
+        const uint64_t leaf = descend<K, F>(t, lds, key);
+        pos[i] = leaf_lower_bound<K, LF>(t, leaf, key);
+    }
+}
+
+// count = upper_bound(hi) - lower_bound(lo); upper_bound(hi) is
+// lower_bound(hi + 1), or n when hi is the largest key value.
+template <class K, int F, int LF>
+__global__ __launch_bounds__(256) void k_range_count(TreeView t, const K *__restrict__ lo,
+                                                     const K *__restrict__ hi,
+                                                     const uint32_t *__restrict__ order,
+                                                     uint64_t *__restrict__ first,
+                                                     uint64_t *__restrict__ count, uint64_t nq) {
+    extern __shared__ uint4 lds_raw[];
+    uint8_t *lds = reinterpret_cast<uint8_t *>(lds_raw);
+    stage_top_levels(t, lds);
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nq; j += step) {
+        const uint64_t i = order ? order[j] : j;
+        const K a = lo[i], b = hi[i];
+        const uint64_t pa = leaf_lower_bound<K, LF>(t, descend<K, F>(t, lds, a), a);
+        uint64_t pb = t.n;
+        if (b != key_max<K>()) {
+            const K b1 = b + 1;
+            pb = leaf_lower_bound<K, LF>(t, descend<K, F>(t, lds, b1), b1);
+        }
+        if (first) first[i] = pa;
+        count[i] = (a > b || pb < pa) ? 0 : pb - pa;
+    }
+}
+
+// One wave per query: lanes copy consecutive value slots.
+template <class K, class V, int LF>
+__global__ __launch_bounds__(256) void k_range_ids(TreeView t, const uint64_t *__restrict__ first,
+                                                   const uint64_t *__restrict__ count,
+                                                   const uint64_t *__restrict__ offsets, uint64_t nq,
+                                                   V *__restrict__ ids) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t waves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    for (uint64_t i = wave; i < nq; i += waves) {
+        const uint64_t f = first[i], c = count[i], o = offsets[i];
+        for (uint64_t k = lane; k < c; k += 64) ids[o + k] = slot_val<K, V, LF>(t, f + k);
+    }
+}
+
+template <class V>
+__global__ void k_fill(V *out, uint64_t n, V value) {
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) out[i] = value;
+}
+
+// ------------------------------------------------------------------ scan
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 8;
+constexpr uint64_t kScanTile = kScanBlock * kScanItems;
+
+template <class T>
+__device__ __forceinline__ T block_exclusive_scan(T v, T *sh_warp, T *total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    T x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        T y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) sh_warp[w] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        T acc = 0;
+        for (int i = 0; i < kScanBlock / 64; ++i) {
+            T s = sh_warp[i];
+            sh_warp[i] = acc;
+            acc += s;
+        }
+        sh_warp[kScanBlock / 64] = acc;
+    }
+    __syncthreads();
+    T excl = x - v + sh_warp[w];
+    *total = sh_warp[kScanBlock / 64];
+    __syncthreads();
+    return excl;
+}
+
+template <class T>
+__global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const T *__restrict__ in, uint64_t n,
+                                                            T *__restrict__ block_sums) {
+    __shared__ T sh[kScanBlock / 64 + 1];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+    T s = 0;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j)
+        if (base + j < n) s += in[base + j];
+    T total;
+    (void)block_exclusive_scan<T>(s, sh, &total);
+    if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
+}
+
+template <class T>
+__global__ __launch_bounds__(kScanBlock) void k_scan_down(const T *__restrict__ in, uint64_t n,
+                                                          const T *__restrict__ block_base,
+                                                          T *__restrict__ out) {
+    __shared__ T sh[kScanBlock / 64 + 1];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+    T v[kScanItems];
+    T s = 0;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        v[j] = (base + j < n) ? in[base + j] : T(0);
+        s += v[j];
+    }
+    T total;
+    T run = block_exclusive_scan<T>(s, sh, &total) + (block_base ? block_base[blockIdx.x] : T(0));
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        if (base + j < n) out[base + j] = run;
+        run += v[j];
+    }
+}
+
+uint64_t scan_tmp_bytes(uint64_t n, uint64_t elem_bytes) {
+    uint64_t bytes = 0;
+    while (n > kScanTile) {
+        n = (n + kScanTile - 1) / kScanTile;
+        bytes += 2 * ((n * elem_bytes + 255) / 256 * 256);
+    }
+    return bytes + 256;
+}
+
+// Exclusive scan, in-place allowed (d_in == d_out). Reduce-then-scan,
+// recursing on the per-tile sums.
+template <class T>
+bpt_status exclusive_scan(const T *d_in, uint64_t n, T *d_out, void *d_tmp, uint64_t tmp_bytes,
+                          hipStream_t s) {
+    if (n == 0) return BPT_OK;
+    const uint64_t tiles = (n + kScanTile - 1) / kScanTile;
+    if (tiles == 1) {
+        k_scan_down<T><<<1, kScanBlock, 0, s>>>(d_in, n, nullptr, d_out);
+        BPT_HIP(hipGetLastError());
+        return BPT_OK;
+    }
+    const uint64_t chunk = (tiles * sizeof(T) + 255) / 256 * 256;
+    BPT_CHECK(tmp_bytes >= 2 * chunk, BPT_ERR_INVALID, "scan workspace too small");
+    T *sums = reinterpret_cast<T *>(d_tmp);
+    T *bases = reinterpret_cast<T *>(reinterpret_cast<uint8_t *>(d_tmp) + chunk);
+    BPT_CHECK(tiles <= 0x7fffffffull, BPT_ERR_CAPACITY, "scan too large");
+    k_scan_reduce<T><<<(uint32_t)tiles, kScanBlock, 0, s>>>(d_in, n, sums);
+    BPT_HIP(hipGetLastError());
+    BPT_TRY(exclusive_scan<T>(sums, tiles, bases, reinterpret_cast<uint8_t *>(d_tmp) + 2 * chunk,
+                              tmp_bytes - 2 * chunk, s));
+    k_scan_down<T><<<(uint32_t)tiles, kScanBlock, 0, s>>>(d_in, n, bases, d_out);
+    BPT_HIP(hipGetLastError());
+    return BPT_OK;
+}
+
+template bpt_status exclusive_scan<uint32_t>(const uint32_t *, uint64_t, uint32_t *, void *, uint64_t, hipStream_t);
+template bpt_status exclusive_scan<uint64_t>(const uint64_t *, uint64_t, uint64_t *, void *, uint64_t, hipStream_t);
+
+// ------------------------------------------------------------------ launches
+static size_t lds_of(const bpt_tree *t) { return t->view.lds_bytes ? t->view.lds_bytes : 16; }
+
+bpt_status launch_plain(const bpt_tree *t, const void *d_q, uint64_t nq, void *d_out, hipStream_t s) {
+    if (nq == 0) return BPT_OK;
+    if (t->n == 0) {
+        uint32_t g = search_grid(t, nq);
+        if (t->val_bytes == 4) k_fill<uint32_t><<<g, 256, 0, s>>>((uint32_t *)d_out, nq, BPT_SENTINEL32);
+        else k_fill<uint64_t><<<g, 256, 0, s>>>((uint64_t *)d_out, nq, BPT_SENTINEL64);
+        BPT_HIP(hipGetLastError());
+        return BPT_OK;
+    }
+    return dispatch_tree(t, [&](auto kt, auto vt, auto fc, auto lc) -> bpt_status {
+        using K = typename decltype(kt)::type;
+        using V = typename decltype(vt)::type;
+        constexpr int F = decltype(fc)::value, LF = decltype(lc)::value;
+        const V sentinel = ~V(0);
+        k_point_plain<K, V, F, LF><<<search_grid(t, nq), t->block, lds_of(t), s>>>(
+            t->view, (const K *)d_q, (V *)d_out, nq, sentinel);
+        BPT_HIP(hipGetLastError());
+        return BPT_OK;
+    });
+}
+
+}  // namespace bpt
+
+using namespace bpt;
+
+extern "C" {
+
+uint64_t bpt_point_search_ws_size(const bpt_tree *t, uint64_t nq, int mode) {
+    if (!t) return 0;
+    if (mode == BPT_POINT_GROUPED) return point_grouped_ws(t, nq);
+    if (mode == BPT_POINT_LEARNED) return point_learned_ws(t, nq);
+    return 0;
+}
+
+bpt_status bpt_point_search_ws(const bpt_tree *t, const void *d_q, uint64_t nq, void *d_out, int mode,
+                               void *d_ws, uint64_t ws_bytes, bpt_stream stream) {
+    BPT_CHECK(t, BPT_ERR_INVALID, "tree is NULL");
+    BPT_CHECK(nq == 0 || (d_q && d_out), BPT_ERR_INVALID, "queries/out are NULL");
+    hipStream_t s = as_stream(stream);
+    switch (mode) {
+        case BPT_POINT_PLAIN: return launch_plain(t, d_q, nq, d_out, s);
+        case BPT_POINT_GROUPED: return point_grouped(t, d_q, nq, d_out, d_ws, ws_bytes, s);
+        case BPT_POINT_LEARNED: return point_learned(t, d_q, nq, d_out, d_ws, ws_bytes, s);
+    }
+    set_error("unknown point mode %d", mode);
+    return BPT_ERR_INVALID;
+}
+
+bpt_status bpt_point_search(bpt_tree *t, const void *d_q, uint64_t nq, void *d_out, int mode,
+                            bpt_stream stream) {
+    BPT_CHECK(t, BPT_ERR_INVALID, "tree is NULL");
+    if (mode == BPT_POINT_PLAIN) return bpt_point_search_ws(t, d_q, nq, d_out, mode, nullptr, 0, stream);
+    const uint64_t need = bpt_point_search_ws_size(t, nq, mode);
+    BPT_TRY(t->ws.reserve(need));
+    return bpt_point_search_ws(t, d_q, nq, d_out, mode, t->ws.ptr, t->ws.bytes, stream);
+}
+
+bpt_status bpt_lower_bound(const bpt_tree *t, const void *d_q, uint64_t nq, uint64_t *d_pos,
+                           bpt_stream stream) {
+    BPT_CHECK(t, BPT_ERR_INVALID, "tree is NULL");
+    BPT_CHECK(nq == 0 || (d_q && d_pos), BPT_ERR_INVALID, "queries/pos are NULL");
+    if (nq == 0) return BPT_OK;
+    hipStream_t s = as_stream(stream);
+    if (t->n == 0) {
+        k_fill<uint64_t><<<search_grid(t, nq), 256, 0, s>>>(d_pos, nq, 0);
+        BPT_HIP(hipGetLastError());
+        return BPT_OK;
+    }
+    return dispatch_tree(t, [&](auto kt, auto, auto fc, auto lc) -> bpt_status {
+        using K = typename decltype(kt)::type;
+        k_lower_bound<K, decltype(fc)::value, decltype(lc)::value>
+            <<<search_grid(t, nq), t->block, lds_of(t), s>>>(t->view, (const K *)d_q, d_pos, nq);
+        BPT_HIP(hipGetLastError());
+        return BPT_OK;
+    });
+}
+
+}  // extern "C"
+
+namespace bpt {
+bpt_status range_count_ordered(const bpt_tree *t, const void *d_lo, const void *d_hi, uint64_t nq,
+                               const uint32_t *d_order, uint64_t *d_first, uint64_t *d_count,
+                               hipStream_t s) {
+    if (nq == 0) return BPT_OK;
+    if (t->n == 0) {
+        k_fill<uint64_t><<<search_grid(t, nq), 256, 0, s>>>(d_count, nq, 0);
+        if (d_first) k_fill<uint64_t><<<search_grid(t, nq), 256, 0, s>>>(d_first, nq, 0);
+        BPT_HIP(hipGetLastError());
+        return BPT_OK;
+    }
+    return dispatch_tree(t, [&](auto kt, auto, auto fc, auto lc) -> bpt_status {
+        using K = typename decltype(kt)::type;
+        k_range_count<K, decltype(fc)::value, decltype(lc)::value>
+            <<<search_grid(t, nq), t->block, lds_of(t), s>>>(t->view, (const K *)d_lo, (const K *)d_hi,
+                                                             d_order, d_first, d_count, nq);
+        BPT_HIP(hipGetLastError());
+        return BPT_OK;
+    });
+}
+}  // namespace bpt
+
+extern "C" {
+
+bpt_status bpt_range_count(const bpt_tree *t, const void *d_lo, const void *d_hi, uint64_t nq,
+                           uint64_t *d_first, uint64_t *d_count, bpt_stream stream) {
+    BPT_CHECK(t, BPT_ERR_INVALID, "tree is NULL");
+    BPT_CHECK(nq == 0 || (d_lo && d_hi && d_count), BPT_ERR_INVALID, "NULL argument");
+    return range_count_ordered(t, d_lo, d_hi, nq, nullptr, d_first, d_count, as_stream(stream));
+}
+
+bpt_status bpt_range_ids(const bpt_tree *t, const uint64_t *d_first, const uint64_t *d_count,
+                         const uint64_t *d_offsets, uint64_t nq, void *d_ids, bpt_stream stream) {
+    BPT_CHECK(t, BPT_ERR_INVALID, "tree is NULL");
+    BPT_CHECK(nq == 0 || (d_first && d_count && d_offsets), BPT_ERR_INVALID, "NULL argument");
+    if (nq == 0 || t->n == 0) return BPT_OK;
+    hipStream_t s = as_stream(stream);
+    return dispatch_tree(t, [&](auto kt, auto vt, auto, auto lc) -> bpt_status {
+        using K = typename decltype(kt)::type;
+        using V = typename decltype(vt)::type;
+        uint64_t waves_needed = nq;
+        uint32_t blocks = (uint32_t)std::min<uint64_t>((waves_needed + 3) / 4, (uint64_t)device_cu_count() * 16);
+        k_range_ids<K, V, decltype(lc)::value><<<blocks, 256, 0, s>>>(t->view, d_first, d_count,
+                                                                       d_offsets, nq, (V *)d_ids);
+        BPT_HIP(hipGetLastError());
+        return BPT_OK;
+    });
+}
+
+bpt_status bpt_exclusive_scan_u64(const uint64_t *d_in, uint64_t n, uint64_t *d_out, uint64_t *h_total,
+                                  bpt_stream stream) {
+    BPT_CHECK(n == 0 || (d_in && d_out), BPT_ERR_INVALID, "NULL argument");
+    hipStream_t s = as_stream(stream);
+    if (n == 0) {
+        if (h_total) *h_total = 0;
+        return BPT_OK;
+    }
+    uint64_t tmp_bytes = scan_tmp_bytes(n, 8);
+    void *tmp = nullptr;
+    BPT_HIP(hipMalloc(&tmp, tmp_bytes));
+    uint64_t last_in = 0;
+    bpt_status st = BPT_OK;
+    if (h_total) {
+        hipError_t e = hipMemcpyAsync(&last_in, d_in + n - 1, 8, hipMemcpyDeviceToHost, s);
+        if (e != hipSuccess) st = hip_fail(e, "scan tail copy", __FILE__, __LINE__);
+    }
+    if (st == BPT_OK) st = exclusive_scan<uint64_t>(d_in, n, d_out, tmp, tmp_bytes, s);
+    uint64_t last_out = 0;
+    if (st == BPT_OK && h_total) {
+        hipError_t e = hipMemcpyAsync(&last_out, d_out + n - 1, 8, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) st = hip_fail(e, "scan total", __FILE__, __LINE__);
+        *h_total = last_out + last_in;
+    }
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(tmp);
+    return st;
+}
+
+}  // extern "C"

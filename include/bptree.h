@@ -1,0 +1,211 @@
+/*
+ * bptree.h — C ABI of the MI355X (gfx950) B+tree index library
+ * (libbptree_amd.so). Plain C: opaque handles, pointers and sizes.
+ *
+ * Every pointer named d_* is device memory (hipMalloc / bpt_device_alloc);
+ * every call is asynchronous on the given stream unless it says otherwise;
+ * NULL means the default stream. Functions return a bpt_status; on failure
+ * bpt_last_error() (thread-local) describes the cause. Nothing here calls
+ * exit().
+ *
+ * Reference call surface each entry point replaces (paths relative to the
+ * sc25index/B-tree-index-searching tree) is named in its comment. The C++
+ * compatibility headers under include/compat/ keep the reference's exact
+ * names and forward to these functions.
+ */
+#ifndef BPTREE_AMD_H
+#define BPTREE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bpt_tree bpt_tree;
+typedef void *bpt_stream; /* a hipStream_t */
+
+typedef enum bpt_status {
+    BPT_OK = 0,
+    BPT_ERR_INVALID = 1,     /* bad argument / unsupported combination */
+    BPT_ERR_UNSORTED = 2,    /* build input keys not non-decreasing */
+    BPT_ERR_NOMEM = 3,       /* device allocation failed */
+    BPT_ERR_HIP = 4,         /* HIP runtime error */
+    BPT_ERR_CAPACITY = 5,    /* size above a documented maximum */
+    BPT_ERR_UNSUPPORTED = 6, /* mode not available for this tree */
+    BPT_ERR_NOT_FOUND = 7    /* no such handle / entry */
+} bpt_status;
+
+/* Miss sentinel: all ones of the value width (INVALID_DATA, BPTree.h:12). */
+#define BPT_SENTINEL32 0xFFFFFFFFu
+#define BPT_SENTINEL64 0xFFFFFFFFFFFFFFFFull
+
+const char *bpt_last_error(void);
+const char *bpt_version(void);
+
+/* ------------------------------------------------------------------ */
+/* Device plumbing (used by hosts that do not bring their own runtime) */
+/* ------------------------------------------------------------------ */
+bpt_status bpt_device_count(int *count);
+bpt_status bpt_device_info(int device, int *cu_count, uint64_t *hbm_bytes, char *name, int name_len);
+bpt_status bpt_device_alloc(uint64_t bytes, void **d_ptr);
+bpt_status bpt_device_free(void *d_ptr);
+bpt_status bpt_memset(void *d_ptr, int value, uint64_t bytes, bpt_stream s);
+/* Synchronous copies (the stream is synchronized before return). */
+bpt_status bpt_copy_h2d(void *d_dst, const void *h_src, uint64_t bytes, bpt_stream s);
+bpt_status bpt_copy_d2h(void *h_dst, const void *d_src, uint64_t bytes, bpt_stream s);
+bpt_status bpt_copy_d2d(void *d_dst, const void *d_src, uint64_t bytes, bpt_stream s);
+bpt_status bpt_stream_create(bpt_stream *s);
+bpt_status bpt_stream_destroy(bpt_stream s);
+bpt_status bpt_stream_sync(bpt_stream s);
+bpt_status bpt_device_sync(void);
+/* HIP-event timer on a stream: record start, record stop, read ms. */
+typedef struct bpt_timer bpt_timer;
+bpt_status bpt_timer_create(bpt_timer **t);
+bpt_status bpt_timer_destroy(bpt_timer *t);
+bpt_status bpt_timer_start(bpt_timer *t, bpt_stream s);
+bpt_status bpt_timer_stop(bpt_timer *t, bpt_stream s);
+bpt_status bpt_timer_ms(bpt_timer *t, float *ms); /* waits for the stop event */
+
+/* ------------------------------------------------------------------ */
+/* Bulk build                                                          */
+/* ------------------------------------------------------------------ */
+/* Replaces genBPlusTree (BPTree.h:26-29, BPTree.cu:127-306): page pool +
+ * leaf copy + one kernel per internal level. Here: a flat, level-ordered
+ * layout, leaves of `leaf_cap` (key, value) slots (keys then values in one
+ * line), internal nodes of `fanout` separators (child maximum, as the
+ * reference's kernelMaxIntermidatePage, BPTree.cu:54), children implicit. */
+typedef struct bpt_build_desc {
+    uint32_t key_bytes;    /* 4 or 8 */
+    uint32_t val_bytes;    /* 4 or 8 */
+    uint32_t fanout;       /* internal separators per node: 16, 32 or 64 (0: 128 B nodes) */
+    uint32_t leaf_cap;     /* slots per leaf: 8, 16 or 32 (0: 128 B leaves) */
+    uint32_t check_sorted; /* non-zero: reject keys that decrease (BPT_ERR_UNSORTED) */
+    uint32_t lds_bytes;    /* LDS budget for the top levels (0: 32 KiB) */
+} bpt_build_desc;
+
+/* Builds from n keys sorted non-decreasing and their values, both on the
+ * device; the inputs are not modified and remain the caller's. n == 0 gives
+ * an empty tree on which every lookup misses. Synchronizes `s`. */
+bpt_status bpt_tree_build(const bpt_build_desc *desc, const void *d_keys, const void *d_vals,
+                          uint64_t n, bpt_tree **out, bpt_stream s);
+bpt_status bpt_tree_free(bpt_tree *t);
+
+#define BPT_MAX_LEVELS 16
+typedef struct bpt_tree_info {
+    uint64_t n;
+    uint32_t key_bytes, val_bytes, fanout, leaf_cap;
+    uint32_t height;     /* internal levels above the leaves */
+    uint32_t lds_levels; /* top internal levels staged in LDS by the search kernels */
+    uint64_t lds_bytes;
+    uint64_t leaf_stride;               /* bytes per leaf */
+    uint64_t leaf_bytes, inner_bytes;   /* device bytes */
+    uint64_t level_nodes[BPT_MAX_LEVELS]; /* [0] = leaves, [height] = 1 */
+} bpt_tree_info;
+bpt_status bpt_tree_get_info(const bpt_tree *t, bpt_tree_info *info);
+/* Device pointers of the flat layout (for tests and the learned index). */
+bpt_status bpt_tree_get_layout(const bpt_tree *t, const void **d_leaves, const void **d_inner);
+/* Launch shape of the search kernels: block of 64, 128 or 256 threads,
+ * blocks per CU of the persistent grid (0 keeps the default). */
+bpt_status bpt_tree_set_launch(bpt_tree *t, uint32_t block, uint32_t blocks_per_cu);
+
+/* ------------------------------------------------------------------ */
+/* Point search                                                        */
+/* ------------------------------------------------------------------ */
+typedef enum bpt_point_mode {
+    BPT_POINT_PLAIN = 0,   /* SearchingBTreeGPUOnlyOneState: thread-per-query descent */
+    BPT_POINT_GROUPED = 1, /* SearchingBTreeGPUCombineStep: on-device radix grouping first */
+    BPT_POINT_LEARNED = 2  /* learned position model + window search (bpt_learned_attach) */
+} bpt_point_mode;
+
+/* out[i] = value of the first key equal to q[i], else the sentinel of the
+ * value width. Results are in the caller's order for every mode.
+ * Plain: replaces kernelSearchingNaiveOneStep (BPTreeExactSearch.cu:25-105).
+ * Grouped: replaces genGrouping + kernelSearchingKeyGroupTwoStep
+ * (grouping.cu:1677-1851, BPTreeExactSearch.cu:206-294); groups on the top
+ * `group_bits` key bits (bpt_tree_set_group_bits, default 12).
+ * Grouped and learned modes use a workspace owned by the tree: concurrent
+ * calls on one tree must share one stream (or use bpt_point_search_ws). */
+bpt_status bpt_point_search(bpt_tree *t, const void *d_q, uint64_t nq, void *d_out, int mode,
+                            bpt_stream s);
+/* Same, with a caller-provided workspace of at least bpt_point_search_ws_size bytes. */
+uint64_t bpt_point_search_ws_size(const bpt_tree *t, uint64_t nq, int mode);
+bpt_status bpt_point_search_ws(const bpt_tree *t, const void *d_q, uint64_t nq, void *d_out,
+                               int mode, void *d_ws, uint64_t ws_bytes, bpt_stream s);
+bpt_status bpt_tree_set_group_bits(bpt_tree *t, uint32_t bits);
+
+/* pos[i] = lower_bound(keys, q[i]) (first slot with key >= q[i]; n if none).
+ * Replaces the position half of kernelSearchingKeyRangeGroupStateTwo
+ * (BtreeRangeSearch.cu:215-294) with a true lower bound. */
+bpt_status bpt_lower_bound(const bpt_tree *t, const void *d_q, uint64_t nq, uint64_t *d_pos,
+                           bpt_stream s);
+
+/* ------------------------------------------------------------------ */
+/* 1-D range search [lo, hi] inclusive                                  */
+/* ------------------------------------------------------------------ */
+/* first[i] = lower_bound(lo[i]); count[i] = |{k : lo[i] <= k <= hi[i]}|
+ * (0 when lo > hi). Replaces SearchingBTreeGPURange's step 2 + step 3
+ * count (BtreeRangeSearch.cu:786-925, :215-374). d_first may be NULL. */
+bpt_status bpt_range_count(const bpt_tree *t, const void *d_lo, const void *d_hi, uint64_t nq,
+                           uint64_t *d_first, uint64_t *d_count, bpt_stream s);
+/* ids for query i (values of the matching slots, in key order) written at
+ * d_ids[offsets[i] ...]; offsets = exclusive scan of counts. */
+bpt_status bpt_range_ids(const bpt_tree *t, const uint64_t *d_first, const uint64_t *d_count,
+                         const uint64_t *d_offsets, uint64_t nq, void *d_ids, bpt_stream s);
+/* d_out[i] = sum of d_in[0..i); *h_total (host, may be NULL) = sum of all.
+ * Synchronizes `s` when h_total is not NULL. */
+bpt_status bpt_exclusive_scan_u64(const uint64_t *d_in, uint64_t n, uint64_t *d_out,
+                                  uint64_t *h_total, bpt_stream s);
+
+/* Query-order grouping front ends for range queries (results unchanged). */
+typedef enum bpt_group_mode {
+    BPT_GROUP_NONE = 0,   /* SearchingBTreeGPURange */
+    BPT_GROUP_GRID2D = 1, /* SearchingBTreeGPURangeWithGrouping (grouping2d.cu bucket) */
+    BPT_GROUP_ZORDER = 2  /* SearchingBTreeGPURangeWithGroupingZorder (zOrder2d.cu key) */
+} bpt_group_mode;
+bpt_status bpt_range_count_grouped(const bpt_tree *t, const void *d_lo, const void *d_hi,
+                                   uint64_t nq, int group_mode, uint64_t *d_first,
+                                   uint64_t *d_count, bpt_stream s);
+
+/* ------------------------------------------------------------------ */
+/* Grouping building blocks                                             */
+/* ------------------------------------------------------------------ */
+/* Stable partition of n keys (key_bytes 4 or 8) by bucket
+ * (key >> shift) & ((1 << bits) - 1), bits <= 16. d_perm[j] = original index
+ * of the j-th element in grouped order; d_bucket_start (1 << bits) + 1
+ * entries, may be NULL. Replaces the MSD passes of genGrouping
+ * (grouping.cu:48-478,1677-1851) with one on-device pass; synchronizes s. */
+bpt_status bpt_radix_group(const void *d_keys, uint32_t key_bytes, uint64_t n, uint32_t shift,
+                           uint32_t bits, uint32_t *d_perm, uint64_t *d_bucket_start,
+                           bpt_stream s);
+/* 64-bit Morton code, x in even bits, y in odd bits (zOrder2d.cu:27-54). */
+bpt_status bpt_morton_encode(const uint32_t *d_x, const uint32_t *d_y, uint64_t n,
+                             uint64_t *d_z, bpt_stream s);
+/* Z-order grouping key of (a, b) pairs: Morton(a / ix, b / iy) with
+ * ix = ceil(max(a) / 2^32) in float, iy likewise (zOrder2d.cu:65-171,
+ * genGrouping2dZ). Synchronizes s. */
+bpt_status bpt_zorder2d(const uint64_t *d_a, const uint64_t *d_b, uint64_t n, uint64_t *d_out,
+                        bpt_stream s);
+/* 2-D grouping key of (a, b) pairs: the bucket of grouping2d.cu:24-31 for
+ * four consecutive passes from the start pass of grouping2d.cu:610-635, one
+ * byte each, most significant first. Synchronizes s. */
+bpt_status bpt_group2d_keys(const uint64_t *d_a, const uint64_t *d_b, uint64_t n,
+                            uint32_t *d_out, bpt_stream s);
+
+/* ------------------------------------------------------------------ */
+/* 2-D rectangle search over Morton-keyed points (u64 keys, u32 ids)    */
+/* ------------------------------------------------------------------ */
+/* rects: n x {x0, y0, x1, y1} uint32, inclusive. The tree holds
+ * key = Morton(x, y), value = point id. counts[i] = points inside rect i;
+ * bpt_rect_ids writes their ids ascending at ids[offsets[i] ...].
+ * group_mode orders the queries (bpt_group_mode); results do not change. */
+bpt_status bpt_rect_count(bpt_tree *t, const uint32_t *d_rects, uint64_t nr, int group_mode,
+                          uint64_t *d_counts, bpt_stream s);
+bpt_status bpt_rect_ids(bpt_tree *t, const uint32_t *d_rects, uint64_t nr, int group_mode,
+                        const uint64_t *d_offsets, uint32_t *d_ids, bpt_stream s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,55 @@
+"""CPU checks of the C-ABI boundary: the library loads and exports every
+function include/*.h declares (no device calls)."""
+import ctypes
+import glob
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DECL = re.compile(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b(bpt_\w+)\s*\(", re.M)
+
+
+def declared_symbols():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names.update(DECL.findall(text))
+    return sorted(names)
+
+
+def test_header_declares_entry_points():
+    names = declared_symbols()
+    for must in ("bpt_tree_build", "bpt_point_search", "bpt_range_count", "bpt_range_ids",
+                 "bpt_rect_count", "bpt_radix_group", "bpt_zorder2d", "bpt_last_error"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    import bptree
+    lib = ctypes.CDLL(bptree.LIB_PATH)
+    missing = [n for n in declared_symbols() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_python_bindings_cover_header():
+    import bptree
+    bound = {name for name, _, _ in bptree.SIGNATURES}
+    missing = [n for n in declared_symbols() if n not in bound]
+    assert not missing, missing
+
+
+def test_errors_without_device_are_reported_not_fatal():
+    import bptree
+    L = bptree.lib()
+    assert L.bpt_version().startswith(b"bptree_amd")
+    # invalid arguments are rejected before any device work
+    d = bptree.BuildDesc(4, 8, 32, 16, 0, 0)   # 4/8 widths unsupported
+    h = ctypes.c_void_p()
+    st = L.bpt_tree_build(ctypes.byref(d), None, None, 0, ctypes.byref(h), None)
+    assert st == 1 and b"not supported" in L.bpt_last_error()
+    d = bptree.BuildDesc(4, 4, 48, 16, 0, 0)
+    assert L.bpt_tree_build(ctypes.byref(d), None, None, 0, ctypes.byref(h), None) == 1
+    assert L.bpt_tree_free(None) == 0

@@ -1,0 +1,114 @@
+"""GPU parity: bulk build + point search (plain, grouped) + lower bound,
+through the C ABI, against the oracle on seeded inputs (bit-exact)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+CFGS = [  # key bytes, value bytes, fanout, leaf cap
+    (4, 4, 32, 16), (4, 4, 16, 8), (4, 4, 64, 32), (8, 8, 16, 8), (8, 8, 32, 16),
+    (8, 8, 64, 32), (8, 4, 32, 16), (8, 4, 16, 8),
+]
+SIZES = [0, 1, 2, 7, 8, 9, 16, 17, 1000, 4096, 65537, 300001]
+
+
+def _case(n, kb, vb, seed):
+    rng = np.random.default_rng(seed)
+    kdt = np.uint32 if kb == 4 else np.uint64
+    vdt = np.uint32 if vb == 4 else np.uint64
+    space = (1 << 32) if kb == 4 else (1 << 64)
+    keys = np.unique(rng.integers(0, space - 1, size=n, dtype=np.uint64)).astype(kdt)
+    if len(keys) and n > 3:
+        keys[0] = 0
+        keys[-1] = kdt(space - 1)          # the all-ones key is a real key
+    vals = rng.integers(0, (1 << (8 * vb)) - 1, size=len(keys), dtype=np.uint64).astype(vdt)
+    q = np.concatenate([keys, rng.integers(0, space - 1, size=3000, dtype=np.uint64).astype(kdt),
+                        np.array([0, space - 1, 1, space - 2], dtype=np.uint64).astype(kdt)])
+    rng.shuffle(q)
+    return keys, vals, q
+
+
+def _expect(oracle, keys, vals, q, vb):
+    sent = 0xFFFFFFFF if vb == 4 else 0xFFFFFFFFFFFFFFFF
+    return oracle.point_closed(keys, vals, q, sent).astype(vals.dtype)
+
+
+@pytest.mark.parametrize("cfg", CFGS, ids=lambda c: "k%dv%dF%dL%d" % c)
+@pytest.mark.parametrize("n", SIZES)
+def test_point_plain_and_grouped(bpt, oracle, cfg, n):
+    kb, vb, F, LF = cfg
+    keys, vals, q = _case(n, kb, vb, seed=n * 31 + F + LF)
+    print(f"case n={n} cfg={cfg}", flush=True)
+    t = bpt.Tree(keys, vals, kb, vb, fanout=F, leaf_cap=LF)
+    want = _expect(oracle, keys, vals, q, vb)
+    assert np.array_equal(t.point_search(q, bpt.PLAIN), want)
+    if kb * 8 >= 12:
+        assert np.array_equal(t.point_search(q, bpt.GROUPED), want)
+    pos = t.lower_bound(q)
+    assert np.array_equal(pos, oracle.lower_bound(keys, q))
+    t.close()
+
+
+def test_point_duplicates_first_occurrence(bpt, oracle):
+    rng = np.random.default_rng(1)
+    keys = np.sort(rng.integers(0, 300, size=5000, dtype=np.uint64).astype(np.uint32))
+    vals = np.arange(len(keys), dtype=np.uint32)
+    q = rng.integers(0, 320, size=4000, dtype=np.uint64).astype(np.uint32)
+    t = bpt.Tree(keys, vals)
+    want = _expect(oracle, keys, vals, q, 4)   # first occurrence = lower bound
+    assert np.array_equal(t.point_search(q), want)
+    assert np.array_equal(t.point_search(q, bpt.GROUPED), want)
+
+
+def test_unsorted_build_rejected(bpt):
+    keys = np.array([5, 3, 9], dtype=np.uint32)
+    with pytest.raises(bpt.BptError) as e:
+        bpt.Tree(keys, keys)
+    assert e.value.status == 2
+
+
+@pytest.mark.parametrize("bits", [1, 8, 12, 13])
+def test_grouped_bits_and_launch_shapes(bpt, oracle, bits):
+    keys, vals, q = _case(200000, 4, 4, seed=bits)
+    t = bpt.Tree(keys, vals)
+    t.set_group_bits(bits)
+    want = _expect(oracle, keys, vals, q, 4)
+    assert np.array_equal(t.point_search(q, bpt.GROUPED), want)
+    for block, bpc in ((64, 1), (128, 3), (256, 16)):
+        t.set_launch(block, bpc)
+        assert np.array_equal(t.point_search(q, bpt.PLAIN), want)
+
+
+def test_workload_c1_scaled_and_page_tree(bpt, oracle, wl):
+    """1M uint32 keys, order-32 page tree of the reference restated on the
+    CPU, against the flat GPU tree, on the CPU workload's query mix."""
+    keys, vals = wl.point_keys(1 << 20, workload=1)
+    q = wl.cpu_queries(keys, 1 << 20, workload=1)
+    pt = oracle.PageTree(keys.astype(np.uint64), vals.astype(np.uint64), 32)
+    want = pt.point(q.astype(np.uint64), 0xFFFFFFFF).astype(np.uint32)
+    t = bpt.Tree(keys, vals)
+    assert np.array_equal(t.point_search(q), want)
+    assert np.array_equal(t.point_search(q, bpt.GROUPED), want)
+    hits = (want != 0xFFFFFFFF).mean()
+    assert 0.49 < hits < 0.52
+
+
+@pytest.mark.parametrize("qset", ["uniform", "zipf"])
+def test_workload_c2_full_size(bpt, oracle, wl, qset):
+    """64M unique uint32 keys, 256M queries (uniform hits / Zipf 0.99 with
+    10% misses): plain and grouped equal the oracle bit for bit."""
+    n, nq = 1 << 26, 1 << 28
+    keys, vals = wl.point_keys(n, workload=2)
+    q = wl.uniform_hits(keys, nq) if qset == "uniform" else wl.zipf_queries(keys, nq)
+    print(f"c2 {qset} generated", flush=True)
+    t = bpt.Tree(keys, vals)
+    dq = bpt.DeviceArray.from_host(q)
+    out = bpt.DeviceArray(nq, np.uint32)
+    want = oracle.point_closed(keys, vals, q, 0xFFFFFFFF).astype(np.uint32)
+    for mode in (bpt.PLAIN, bpt.GROUPED):
+        out.fill_bytes(0)
+        t.point_search_dev(dq, out, mode)
+        got = out.to_host()
+        assert int((got != want).sum()) == 0, (qset, mode)
+    if qset == "uniform":
+        assert np.array_equal(want[:1000], np.searchsorted(keys, q[:1000]).astype(np.uint32))
