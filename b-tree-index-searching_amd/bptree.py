@@ -78,6 +78,7 @@ SIGNATURES = [
     ("bpt_tree_get_info", C.c_int, [vp, C.POINTER(TreeInfo)]),
     ("bpt_tree_get_layout", C.c_int, [vp, C.POINTER(vp), C.POINTER(vp)]),
     ("bpt_tree_set_launch", C.c_int, [vp, C.c_uint32, C.c_uint32]),
+    ("bpt_tree_set_option", C.c_int, [vp, C.c_char_p, C.c_int64]),
     ("bpt_point_search", C.c_int, [vp, vp, C.c_uint64, vp, C.c_int, vp]),
     ("bpt_point_search_ws_size", C.c_uint64, [vp, C.c_uint64, C.c_int]),
     ("bpt_point_search_ws", C.c_int, [vp, vp, C.c_uint64, vp, C.c_int, vp, C.c_uint64, vp]),
@@ -285,6 +286,9 @@ class Tree:
 
     def set_launch(self, block=0, blocks_per_cu=0):
         check(lib().bpt_tree_set_launch(self.h, block, blocks_per_cu))
+
+    def set_option(self, key, value):
+        check(lib().bpt_tree_set_option(self.h, key.encode(), int(value)))
 
     def set_group_bits(self, bits):
         check(lib().bpt_tree_set_group_bits(self.h, bits))

@@ -283,6 +283,20 @@ bpt_status bpt_tree_set_launch(bpt_tree *t, uint32_t block, uint32_t blocks_per_
     return BPT_OK;
 }
 
+bpt_status bpt_tree_set_option(bpt_tree *t, const char *key, int64_t value) {
+    BPT_CHECK(t && key, BPT_ERR_INVALID, "NULL argument");
+    if (strcmp(key, "coop") == 0) {
+        t->coop = value ? 1 : 0;
+        return BPT_OK;
+    }
+    if (strcmp(key, "leaf_nt") == 0) {
+        t->leaf_nt = value ? 1 : 0;
+        return BPT_OK;
+    }
+    set_error("unknown tree option '%s'", key);
+    return BPT_ERR_NOT_FOUND;
+}
+
 bpt_status bpt_tree_set_group_bits(bpt_tree *t, uint32_t bits) {
     BPT_CHECK(t, BPT_ERR_INVALID, "tree is NULL");
     BPT_CHECK(bits >= 1 && bits <= 16 && bits <= t->key_bytes * 8, BPT_ERR_INVALID,

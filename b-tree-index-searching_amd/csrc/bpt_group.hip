@@ -24,111 +24,177 @@ bpt_status range_count_ordered(const bpt_tree *t, const void *d_lo, const void *
 static inline uint64_t align256(uint64_t b) { return (b + 255) / 256 * 256; }
 
 // ==================================================================== grouped point search
-// Pipeline for nq queries, B = 2^bits buckets, tiles of T queries:
-//   1 k_tile_hist     per-tile bucket counts, tile-major hist[t][b]
-//   2 k_col_partial   sums of 64-tile chunks per bucket
-//   3 k_col_scan      per-bucket chunk bases + bucket bases (one block)
-//   4 k_col_final     hist[t][b] <- first grouped slot of (bucket b, tile t)
-//   5 k_tile_scatter  grouped[slot] = q, pos[i] = slot (slot from an LDS cursor)
-//   6 plain descent on the grouped queries (neighbouring lanes share subtrees)
-//   7 k_ungather      out[i] = grouped_result[pos[i]]
-constexpr int kGBlock = 1024;
-constexpr int kChunkTiles = 64;
+// Queries are partitioned on their top `bits` key bits (default 12) in MSD
+// passes of at most 8 bits (8 + 4 for 12 bits): no pass scatters into more
+// than 256 runs per tile, so the partially written lines of a block stay in
+// L2 and merge there before reaching HBM. A pass over segments is
+//   k_pass_hist     digit counts per (segment, tile), laid out [segment][digit][tile]
+//   exclusive scan  of that flat array = the first slot of every run
+//   k_pass_scatter  key -> its slot; slot of every input position -> pos
+// The search walks the grouped queries (lanes of a wave share subtrees and a
+// bucket's leaves stay in L2), then one gather per pass, in reverse, brings
+// the results back to the caller's order with the same run locality.
+constexpr int kPBlock = 512;
+constexpr uint32_t kPTile = 16384;
+
+struct PassDesc {
+    uint32_t shift, bits, nseg, pad_;
+    const uint32_t *seg_begin;  // nseg + 1 entries
+    const uint32_t *tile_base;  // nseg + 1 entries: first tile of each segment
+};
+
+__device__ __forceinline__ uint32_t find_seg(const PassDesc &p, uint32_t tile) {
+    uint32_t lo = 0, hi = p.nseg;  // largest s with tile_base[s] <= tile
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (p.tile_base[mid] <= tile) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint64_t lanemask_lt64() {
+    const uint32_t lane = threadIdx.x & 63;
+    return lane ? (~0ull >> (64 - lane)) : 0ull;
+}
 
 template <class K>
-__global__ __launch_bounds__(kGBlock) void k_tile_hist(const K *__restrict__ q, uint64_t nq,
-                                                       uint32_t shift, uint32_t nb, uint32_t tile,
+__global__ __launch_bounds__(kPBlock) void k_pass_hist(const K *__restrict__ keys, PassDesc p,
                                                        uint32_t *__restrict__ hist) {
-    extern __shared__ uint32_t cnt[];
-    for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) cnt[b] = 0;
+    __shared__ uint32_t cnt[256];
+    const uint32_t tile = blockIdx.x;
+    if (tile >= p.tile_base[p.nseg]) return;  // whole block
+    const uint32_t s = find_seg(p, tile);
+    const uint32_t tiles_s = p.tile_base[s + 1] - p.tile_base[s], tl = tile - p.tile_base[s];
+    const uint64_t lo = (uint64_t)p.seg_begin[s] + (uint64_t)tl * kPTile;
+    const uint64_t hi = std::min<uint64_t>(lo + kPTile, p.seg_begin[s + 1]);
+    const uint32_t D = 1u << p.bits, mask = D - 1;
+    for (uint32_t d = threadIdx.x; d < D; d += blockDim.x) cnt[d] = 0;
     __syncthreads();
-    const uint64_t base = (uint64_t)blockIdx.x * tile;
-    const uint64_t end = std::min<uint64_t>(base + tile, nq);
-    for (uint64_t i = base + threadIdx.x; i < end; i += blockDim.x) {
-        const uint32_t b = (uint32_t)(q[i] >> shift);
-        atomicAdd(&cnt[b], 1u);
-    }
+    for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x)
+        atomicAdd(&cnt[(uint32_t)(keys[i] >> p.shift) & mask], 1u);
     __syncthreads();
-    uint32_t *row = hist + (uint64_t)blockIdx.x * nb;
-    for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) row[b] = cnt[b];
-}
-
-__global__ void k_col_partial(const uint32_t *__restrict__ hist, uint32_t ntiles, uint32_t nb,
-                              uint32_t *__restrict__ part) {
-    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t c = blockIdx.y;
-    if (b >= nb) return;
-    const uint32_t t0 = c * kChunkTiles, t1 = std::min<uint32_t>(t0 + kChunkTiles, ntiles);
-    uint32_t s = 0;
-    for (uint32_t t = t0; t < t1; ++t) s += hist[(uint64_t)t * nb + b];
-    part[(uint64_t)c * nb + b] = s;
-}
-
-__global__ void k_col_final(uint32_t *__restrict__ hist, const uint32_t *__restrict__ part,
-                            uint32_t ntiles, uint32_t nb) {
-    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t c = blockIdx.y;
-    if (b >= nb) return;
-    const uint32_t t0 = c * kChunkTiles, t1 = std::min<uint32_t>(t0 + kChunkTiles, ntiles);
-    uint32_t run = part[(uint64_t)c * nb + b];
-    for (uint32_t t = t0; t < t1; ++t) {
-        uint64_t idx = (uint64_t)t * nb + b;
-        uint32_t v = hist[idx];
-        hist[idx] = run;
-        run += v;
-    }
+    uint32_t *row = hist + (uint64_t)D * p.tile_base[s] + tl;
+    for (uint32_t d = threadIdx.x; d < D; d += blockDim.x) row[(uint64_t)d * tiles_s] = cnt[d];
 }
 
 template <class K>
-__global__ __launch_bounds__(kGBlock) void k_tile_scatter(const K *__restrict__ q, uint64_t nq,
-                                                          uint32_t shift, uint32_t nb, uint32_t tile,
+__global__ __launch_bounds__(kPBlock) void k_pass_scatter(const K *__restrict__ keys_in, PassDesc p,
                                                           const uint32_t *__restrict__ offs,
-                                                          K *__restrict__ gq, uint32_t *__restrict__ pos) {
-    extern __shared__ uint32_t cur[];
-    const uint32_t *row = offs + (uint64_t)blockIdx.x * nb;
-    for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) cur[b] = row[b];
+                                                          K *__restrict__ keys_out,
+                                                          uint32_t *__restrict__ pos_out) {
+    __shared__ uint32_t cur[256];
+    const uint32_t tile = blockIdx.x;
+    if (tile >= p.tile_base[p.nseg]) return;
+    const uint32_t s = find_seg(p, tile);
+    const uint32_t tiles_s = p.tile_base[s + 1] - p.tile_base[s], tl = tile - p.tile_base[s];
+    const uint64_t lo = (uint64_t)p.seg_begin[s] + (uint64_t)tl * kPTile;
+    const uint64_t hi = std::min<uint64_t>(lo + kPTile, p.seg_begin[s + 1]);
+    const uint32_t D = 1u << p.bits, mask = D - 1;
+    const uint32_t *row = offs + (uint64_t)D * p.tile_base[s] + tl;
+    for (uint32_t d = threadIdx.x; d < D; d += blockDim.x) cur[d] = row[(uint64_t)d * tiles_s];
     __syncthreads();
-    const uint64_t base = (uint64_t)blockIdx.x * tile;
-    const uint64_t end = std::min<uint64_t>(base + tile, nq);
-    for (uint64_t i = base + threadIdx.x; i < end; i += blockDim.x) {
-        const K key = q[i];
-        const uint32_t b = (uint32_t)(key >> shift);
-        const uint32_t slot = atomicAdd(&cur[b], 1u);
-        gq[slot] = key;
-        pos[i] = slot;
+    for (uint64_t b = lo; b < hi; b += blockDim.x) {
+        const uint64_t i = b + threadIdx.x;
+        const bool ok = i < hi;
+        const K key = ok ? keys_in[i] : K(0);
+        const uint32_t d = (uint32_t)(key >> p.shift) & mask;
+        // lanes of the wave with the same digit claim consecutive slots at once
+        uint64_t peers = __ballot(ok);
+        for (uint32_t bit = 0; bit < p.bits; ++bit) {
+            const bool x = (d >> bit) & 1u;
+            const uint64_t m = __ballot(x);
+            peers &= x ? m : ~m;
+        }
+        const uint32_t rank = (uint32_t)__popcll(peers & lanemask_lt64());
+        const int leader = __ffsll((long long)peers) - 1;
+        uint32_t base = 0;
+        if (ok && rank == 0) base = atomicAdd(&cur[d], (uint32_t)__popcll(peers));
+        base = __shfl(base, leader < 0 ? 0 : leader, 64);
+        if (ok) {
+            const uint32_t slot = base + rank;
+            keys_out[slot] = key;
+            pos_out[i] = slot;
+        }
+    }
+}
+
+// One segment [0, n) split into tiles.
+__global__ void k_pass_init_single(uint32_t *seg_begin, uint32_t *tile_base, uint64_t n) {
+    seg_begin[0] = 0;
+    seg_begin[1] = (uint32_t)n;
+    tile_base[0] = 0;
+    tile_base[1] = (uint32_t)((n + kPTile - 1) / kPTile);
+}
+
+// Segments of pass 2 = digit ranges of pass 1 (first slot of each digit in
+// the scanned pass-1 histogram), tiled.
+__global__ __launch_bounds__(256) void k_pass_init_segments(const uint32_t *__restrict__ scanned1,
+                                                            uint32_t ntiles1, uint32_t nseg, uint64_t n,
+                                                            uint32_t *__restrict__ seg_begin,
+                                                            uint32_t *__restrict__ tile_base) {
+    __shared__ uint32_t sh[256];
+    __shared__ uint32_t wsum[4];
+    const uint32_t d = threadIdx.x;
+    const uint32_t begin = d < nseg ? scanned1[(uint64_t)d * ntiles1] : (uint32_t)n;
+    sh[d] = begin;
+    __syncthreads();
+    const uint32_t end = (d + 1 < nseg) ? sh[d + 1] : (uint32_t)n;
+    const uint32_t tiles = d < nseg ? (end - begin + kPTile - 1) / kPTile : 0;
+    // block exclusive scan of tiles
+    const int lane = d & 63, w = d >> 6;
+    uint32_t x = tiles;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    uint32_t add = 0;
+    for (int k = 0; k < w; ++k) add += wsum[k];
+    const uint32_t excl = x - tiles + add;
+    if (d < nseg) {
+        seg_begin[d] = begin;
+        tile_base[d] = excl;
+    }
+    if (d == nseg - 1) {
+        seg_begin[nseg] = (uint32_t)n;
+        tile_base[nseg] = excl + tiles;
     }
 }
 
 template <class V>
-__global__ __launch_bounds__(256) void k_ungather(const V *__restrict__ gres, const uint32_t *__restrict__ pos,
-                                                  uint64_t nq, V *__restrict__ out) {
+__global__ __launch_bounds__(256) void k_gather(const V *__restrict__ src, const uint32_t *__restrict__ idx,
+                                                uint64_t n, V *__restrict__ dst) {
     const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += step)
-        __builtin_nontemporal_store(gres[__builtin_nontemporal_load(&pos[i])], &out[i]);
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step)
+        __builtin_nontemporal_store(src[__builtin_nontemporal_load(&idx[i])], &dst[i]);
 }
 
 struct GroupPlan {
-    uint32_t nb, tile, ntiles, nchunks, shift;
-    uint64_t off_hist, off_part, off_sums, off_gq, off_pos, off_gres, total;
+    uint32_t bits1, bits2, ntiles, maxtiles2;
+    uint64_t off_key1, off_pos1, off_key2, off_pos2, off_res, off_resa, off_hist, off_seg, off_scan;
+    uint64_t hist_entries, scan_bytes, total;
 };
 
 static GroupPlan group_plan(const bpt_tree *t, uint64_t nq) {
     GroupPlan p{};
     const uint32_t bits = t->group_bits;
-    p.nb = 1u << bits;
-    p.shift = t->key_bytes * 8 - bits;
-    uint64_t tile = 4096;
-    while (tile < 65536 && tile * 2048 < nq) tile *= 2;
-    p.tile = (uint32_t)tile;
-    p.ntiles = (uint32_t)std::max<uint64_t>(1, (nq + tile - 1) / tile);
-    p.nchunks = (p.ntiles + kChunkTiles - 1) / kChunkTiles;
+    p.bits1 = bits > 8 ? 8 : bits;
+    p.bits2 = bits - p.bits1;
+    p.ntiles = (uint32_t)std::max<uint64_t>(1, (nq + kPTile - 1) / kPTile);
+    p.maxtiles2 = p.ntiles + (1u << p.bits1);
+    p.hist_entries = 256ull * p.maxtiles2;
+    p.scan_bytes = scan_tmp_bytes(p.hist_entries, 4);
     uint64_t off = 0;
-    p.off_hist = off; off += align256((uint64_t)p.ntiles * p.nb * 4);
-    p.off_part = off; off += align256((uint64_t)p.nchunks * p.nb * 4);
-    p.off_sums = off; off += align256((uint64_t)p.nchunks * p.nb * 4);
-    p.off_gq = off;   off += align256(nq * t->key_bytes);
-    p.off_pos = off;  off += align256(nq * 4);
-    p.off_gres = off; off += align256(nq * t->val_bytes);
+    p.off_key1 = off; off += align256(nq * t->key_bytes);
+    p.off_pos1 = off; off += align256(nq * 4);
+    p.off_key2 = off; off += align256(nq * t->key_bytes);
+    p.off_pos2 = off; off += align256(nq * 4);
+    p.off_res = off;  off += align256(nq * t->val_bytes);
+    p.off_resa = off; off += align256(nq * t->val_bytes);
+    p.off_hist = off; off += align256(p.hist_entries * 4);
+    p.off_seg = off;  off += align256(4 * 4 * 260);
+    p.off_scan = off; off += align256(p.scan_bytes);
     p.total = off;
     return p;
 }
@@ -138,70 +204,54 @@ uint64_t point_grouped_ws(const bpt_tree *t, uint64_t nq) {
     return group_plan(t, chunk).total;
 }
 
-// Single-block scan over buckets using chunk sums kept in a second array:
-// sums[c][b] (chunk partial sums) -> part[c][b] = bucket base + chunk base.
-__global__ __launch_bounds__(1024) void k_col_scan(const uint32_t *__restrict__ sums,
-                                                   uint32_t *__restrict__ part, uint32_t nchunks,
-                                                   uint32_t nb) {
-    __shared__ uint32_t wsum[16];
-    const uint32_t per = (nb + 1023) / 1024;
-    uint32_t local = 0;
-    for (uint32_t k = 0; k < per; ++k) {
-        const uint32_t b = threadIdx.x * per + k;
-        if (b >= nb) break;
-        for (uint32_t c = 0; c < nchunks; ++c) local += sums[(uint64_t)c * nb + b];
-    }
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    uint32_t x = local;
-    for (int d = 1; d < 64; d <<= 1) {
-        uint32_t y = __shfl_up(x, d, 64);
-        if (lane >= d) x += y;
-    }
-    if (lane == 63) wsum[w] = x;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t acc = 0;
-        for (int i = 0; i < 16; ++i) { uint32_t v = wsum[i]; wsum[i] = acc; acc += v; }
-    }
-    __syncthreads();
-    uint32_t run = x - local + wsum[w];
-    for (uint32_t k = 0; k < per; ++k) {
-        const uint32_t b = threadIdx.x * per + k;
-        if (b >= nb) break;
-        for (uint32_t c = 0; c < nchunks; ++c) {
-            const uint64_t idx = (uint64_t)c * nb + b;
-            const uint32_t v = sums[idx];
-            part[idx] = run;
-            run += v;
-        }
-    }
-}
-
 template <class K, class V>
 static bpt_status grouped_chunk(const bpt_tree *t, const K *q, uint64_t nq, V *out, uint8_t *ws,
                                 hipStream_t s) {
     const GroupPlan p = group_plan(t, nq);
+    K *key1 = reinterpret_cast<K *>(ws + p.off_key1);
+    K *key2 = reinterpret_cast<K *>(ws + p.off_key2);
+    uint32_t *pos1 = reinterpret_cast<uint32_t *>(ws + p.off_pos1);
+    uint32_t *pos2 = reinterpret_cast<uint32_t *>(ws + p.off_pos2);
+    V *res = reinterpret_cast<V *>(ws + p.off_res);
+    V *resa = reinterpret_cast<V *>(ws + p.off_resa);
     uint32_t *hist = reinterpret_cast<uint32_t *>(ws + p.off_hist);
-    uint32_t *part = reinterpret_cast<uint32_t *>(ws + p.off_part);
-    K *gq = reinterpret_cast<K *>(ws + p.off_gq);
-    uint32_t *pos = reinterpret_cast<uint32_t *>(ws + p.off_pos);
-    V *gres = reinterpret_cast<V *>(ws + p.off_gres);
-    uint32_t *sums_buf = reinterpret_cast<uint32_t *>(ws + p.off_sums);
-    const size_t lds = (size_t)p.nb * 4;
+    uint32_t *seg = reinterpret_cast<uint32_t *>(ws + p.off_seg);
+    uint32_t *seg1 = seg, *tb1 = seg + 260, *seg2 = seg + 520, *tb2 = seg + 780;
+    void *scan_tmp = ws + p.off_scan;
+    const uint32_t kbits = t->key_bytes * 8;
 
-    k_tile_hist<K><<<p.ntiles, kGBlock, lds, s>>>(q, nq, p.shift, p.nb, p.tile, hist);
+    // pass 1: top bits1 bits, one segment
+    PassDesc d1{kbits - p.bits1, p.bits1, 1, 0, seg1, tb1};
+    k_pass_init_single<<<1, 1, 0, s>>>(seg1, tb1, nq);
     BPT_HIP(hipGetLastError());
-    dim3 g2((p.nb + 255) / 256, p.nchunks);
-    k_col_partial<<<g2, 256, 0, s>>>(hist, p.ntiles, p.nb, sums_buf);
+    k_pass_hist<K><<<p.ntiles, kPBlock, 0, s>>>(q, d1, hist);
     BPT_HIP(hipGetLastError());
-    k_col_scan<<<1, 1024, 0, s>>>(sums_buf, part, p.nchunks, p.nb);
+    BPT_TRY(exclusive_scan<uint32_t>(hist, (uint64_t)p.ntiles << p.bits1, hist, scan_tmp, p.scan_bytes, s));
+    k_pass_scatter<K><<<p.ntiles, kPBlock, 0, s>>>(q, d1, hist, key1, pos1);
     BPT_HIP(hipGetLastError());
-    k_col_final<<<g2, 256, 0, s>>>(hist, part, p.ntiles, p.nb);
-    BPT_HIP(hipGetLastError());
-    k_tile_scatter<K><<<p.ntiles, kGBlock, lds, s>>>(q, nq, p.shift, p.nb, p.tile, hist, gq, pos);
-    BPT_HIP(hipGetLastError());
-    BPT_TRY(launch_plain(t, gq, nq, gres, s));
-    k_ungather<V><<<search_grid(t, nq), 256, 0, s>>>(gres, pos, nq, out);
+    const K *grouped = key1;
+    if (p.bits2) {
+        // pass 2: next bits2 bits inside each pass-1 digit
+        const uint32_t nseg = 1u << p.bits1;
+        k_pass_init_segments<<<1, 256, 0, s>>>(hist, p.ntiles, nseg, nq, seg2, tb2);
+        BPT_HIP(hipGetLastError());
+        PassDesc d2{kbits - p.bits1 - p.bits2, p.bits2, nseg, 0, seg2, tb2};
+        k_pass_hist<K><<<p.maxtiles2, kPBlock, 0, s>>>(key1, d2, hist);
+        BPT_HIP(hipGetLastError());
+        BPT_TRY(exclusive_scan<uint32_t>(hist, (uint64_t)p.maxtiles2 << p.bits2, hist, scan_tmp, p.scan_bytes, s));
+        k_pass_scatter<K><<<p.maxtiles2, kPBlock, 0, s>>>(key1, d2, hist, key2, pos2);
+        BPT_HIP(hipGetLastError());
+        grouped = key2;
+    }
+    BPT_TRY(launch_plain(t, grouped, nq, res, s));
+    const uint32_t g = search_grid(t, nq);
+    if (p.bits2) {
+        k_gather<V><<<g, 256, 0, s>>>(res, pos2, nq, resa);  // pass-2 order -> pass-1 order
+        BPT_HIP(hipGetLastError());
+        k_gather<V><<<g, 256, 0, s>>>(resa, pos1, nq, out);  // pass-1 order -> caller order
+    } else {
+        k_gather<V><<<g, 256, 0, s>>>(res, pos1, nq, out);
+    }
     BPT_HIP(hipGetLastError());
     return BPT_OK;
 }
@@ -210,7 +260,6 @@ bpt_status point_grouped(const bpt_tree *t, const void *d_q, uint64_t nq, void *
                          uint64_t ws_bytes, hipStream_t s) {
     if (nq == 0) return BPT_OK;
     if (t->n == 0) return launch_plain(t, d_q, nq, d_out, s);
-    BPT_CHECK(t->group_bits <= 13, BPT_ERR_INVALID, "grouped search supports at most 13 group bits");
     BPT_CHECK(d_ws && ws_bytes >= point_grouped_ws(t, nq), BPT_ERR_INVALID,
               "grouped search needs a workspace of %llu bytes",
               (unsigned long long)point_grouped_ws(t, nq));

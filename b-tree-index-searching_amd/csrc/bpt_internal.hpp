@@ -76,9 +76,11 @@ struct bpt_tree {
     uint8_t *d_inner = nullptr;
     uint64_t leaf_bytes = 0, inner_bytes = 0;
     int device = 0;
-    uint32_t block = 256, blocks_per_cu = 8;
+    uint32_t block = 256, blocks_per_cu = 32;
     uint32_t group_bits = 12;
     uint32_t lds_budget = 32768;
+    uint32_t coop = 1;               // cooperative 8-lane descent when widths match
+    uint32_t leaf_nt = 1;            // non-temporal leaf loads
     bpt::TreeView view{};
     bpt::Workspace ws;             // grouped / learned scratch owned by the tree
     bpt::LearnedModel *learned = nullptr;
@@ -247,6 +249,100 @@ __device__ __forceinline__ void stage_top_levels(const TreeView &t, uint8_t *lds
     uint4 *dst = reinterpret_cast<uint4 *>(lds);
     for (uint32_t i = threadIdx.x; i < t.lds_bytes / 16; i += blockDim.x) dst[i] = src[i];
     __syncthreads();
+}
+
+// ---------------------------------------------------------------- cooperative descent
+// Eight lanes share one node: lane s of a group holds bytes [s*NB, (s+1)*NB)
+// of the node, so one load instruction of a wave reads eight whole nodes
+// (eight fully used 128-byte lines) instead of one 16-byte piece of 64 lines.
+template <int BYTES>
+struct Chunk {
+    static constexpr int W = BYTES / 4;
+    uint32_t w[W];
+};
+
+template <int BYTES>
+__device__ __forceinline__ void load_chunk(const uint8_t *p, Chunk<BYTES> &c) {
+    if constexpr (BYTES == 8) {
+        const uint2 v = *reinterpret_cast<const uint2 *>(p);
+        c.w[0] = v.x; c.w[1] = v.y;
+    } else {
+#pragma unroll
+        for (int i = 0; i < BYTES / 16; ++i) {
+            const uint4 v = reinterpret_cast<const uint4 *>(p)[i];
+            c.w[4 * i] = v.x; c.w[4 * i + 1] = v.y; c.w[4 * i + 2] = v.z; c.w[4 * i + 3] = v.w;
+        }
+    }
+}
+
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
+
+// Same as load_chunk with the non-temporal hint: lines read once (leaves)
+// are marked streaming so they do not displace the reused upper levels.
+template <int BYTES>
+__device__ __forceinline__ void load_chunk_nt(const uint8_t *p, Chunk<BYTES> &c) {
+    if constexpr (BYTES == 8) {
+        const u32x2_t v = __builtin_nontemporal_load(reinterpret_cast<const u32x2_t *>(p));
+        c.w[0] = v.x; c.w[1] = v.y;
+    } else {
+#pragma unroll
+        for (int i = 0; i < BYTES / 16; ++i) {
+            const u32x4_t v = __builtin_nontemporal_load(reinterpret_cast<const u32x4_t *>(p) + i);
+            c.w[4 * i] = v.x; c.w[4 * i + 1] = v.y; c.w[4 * i + 2] = v.z; c.w[4 * i + 3] = v.w;
+        }
+    }
+}
+
+template <class K, int BYTES>
+__device__ __forceinline__ K chunk_elem(const Chunk<BYTES> &c, uint32_t i) {
+    constexpr int N = BYTES / (int)sizeof(K);
+    K r = 0;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        K v;
+        if constexpr (sizeof(K) == 4) v = c.w[j];
+        else v = (uint64_t)c.w[2 * j] | ((uint64_t)c.w[2 * j + 1] << 32);
+        r = (i == (uint32_t)j) ? v : r;
+    }
+    return r;
+}
+
+template <class K, int BYTES>
+__device__ __forceinline__ uint32_t chunk_count_lt(const Chunk<BYTES> &c, K q) {
+    constexpr int N = BYTES / (int)sizeof(K);
+    uint32_t n = 0;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        K v;
+        if constexpr (sizeof(K) == 4) v = c.w[j];
+        else v = (uint64_t)c.w[2 * j] | ((uint64_t)c.w[2 * j + 1] << 32);
+        n += (v < q);
+    }
+    return n;
+}
+
+// Sum over the 8 lanes of a group with DPP lane moves (no LDS traffic):
+// quad_perm [1,0,3,2], quad_perm [2,3,0,1], then row_half_mirror, which
+// pairs lane i with lane 7-i of the same 8-lane half row (the other quad).
+__device__ __forceinline__ uint32_t group8_sum(uint32_t x) {
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xF, 0xF, false);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x4E, 0xF, 0xF, false);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x141, 0xF, 0xF, false);
+    return x;
+}
+
+// Branch-free lower bound over n sorted keys (n >= 1): the loop trip count
+// depends only on n, so the lanes of a wave stay converged.
+template <class K>
+__device__ __forceinline__ uint32_t lower_bound_flat(const K *__restrict__ a, uint32_t n, K q) {
+    uint32_t base = 0;
+    while (n > 1) {
+        const uint32_t half = n >> 1;
+        base = (a[base + half] < q) ? base + half : base;
+        n -= half;
+    }
+    return base + (a[base] < q);
 }
 
 // ---------------------------------------------------------------- host helpers

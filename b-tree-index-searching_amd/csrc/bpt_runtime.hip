@@ -59,6 +59,18 @@ bpt_status Workspace::reserve(uint64_t need) {
 
 }  // namespace bpt
 
+namespace bpt {
+// Device-to-device copy with 16-byte loads/stores (also the known-size
+// reference kernel for HBM byte counters: it reads and writes `bytes`).
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_copy_u4(const u32x4 *__restrict__ src, u32x4 *__restrict__ dst,
+                                                 uint64_t n16) {
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += step)
+        __builtin_nontemporal_store(__builtin_nontemporal_load(&src[i]), &dst[i]);
+}
+}  // namespace bpt
+
 struct bpt_timer {
     hipEvent_t a = nullptr, b = nullptr;
 };
@@ -120,8 +132,19 @@ bpt_status bpt_copy_d2h(void *h_dst, const void *d_src, uint64_t bytes, bpt_stre
 
 bpt_status bpt_copy_d2d(void *d_dst, const void *d_src, uint64_t bytes, bpt_stream s) {
     if (bytes == 0) return BPT_OK;
-    BPT_HIP(hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, bpt::as_stream(s)));
-    BPT_HIP(hipStreamSynchronize(bpt::as_stream(s)));
+    hipStream_t h = bpt::as_stream(s);
+    const bool aligned = ((reinterpret_cast<uintptr_t>(d_dst) | reinterpret_cast<uintptr_t>(d_src) | bytes) & 15) == 0;
+    if (aligned) {
+        const uint64_t n16 = bytes / 16;
+        uint64_t blocks = (n16 + 255) / 256;
+        const uint64_t cap = (uint64_t)bpt::device_cu_count() * 16;
+        if (blocks > cap) blocks = cap;
+        bpt::k_copy_u4<<<(uint32_t)blocks, 256, 0, h>>>((const bpt::u32x4 *)d_src, (bpt::u32x4 *)d_dst, n16);
+        BPT_HIP(hipGetLastError());
+    } else {
+        BPT_HIP(hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, h));
+    }
+    BPT_HIP(hipStreamSynchronize(h));
     return BPT_OK;
 }
 

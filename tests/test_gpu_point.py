@@ -42,6 +42,9 @@ def test_point_plain_and_grouped(bpt, oracle, cfg, n):
     t = bpt.Tree(keys, vals, kb, vb, fanout=F, leaf_cap=LF)
     want = _expect(oracle, keys, vals, q, vb)
     assert np.array_equal(t.point_search(q, bpt.PLAIN), want)
+    t.set_option("coop", 0)   # one lane per query
+    assert np.array_equal(t.point_search(q, bpt.PLAIN), want)
+    t.set_option("coop", 1)
     if kb * 8 >= 12:
         assert np.array_equal(t.point_search(q, bpt.GROUPED), want)
     pos = t.lower_bound(q)
