@@ -1,0 +1,129 @@
+"""GPU parity: 1-D inclusive range search (count, first, ids) in every
+grouping mode, and 2-D rectangle search over Morton keys, against the oracle."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+MODES = (0, 1, 2)  # none, 2D grouping, Z-order
+
+
+def _ids_expected(vals, first, cnt):
+    if len(first) == 0:
+        return np.empty(0, dtype=vals.dtype)
+    return np.concatenate([vals[int(f):int(f) + int(c)] for f, c in zip(first, cnt)] + [vals[:0]])
+
+
+@pytest.mark.parametrize("kb,vb", [(4, 4), (8, 8), (8, 4)])
+@pytest.mark.parametrize("n", [0, 1, 5, 1000, 100003])
+def test_range_matches_oracle(bpt, oracle, kb, vb, n):
+    rng = np.random.default_rng(n + kb)
+    kdt = np.uint32 if kb == 4 else np.uint64
+    vdt = np.uint32 if vb == 4 else np.uint64
+    space = (1 << 32) if kb == 4 else (1 << 64)
+    keys = np.sort(rng.integers(0, space - 1, size=n, dtype=np.uint64)).astype(kdt)
+    if n > 10:
+        keys[3:8] = keys[3]       # duplicates spanning slots
+        keys[-1] = kdt(space - 1)
+    vals = rng.integers(0, (1 << (8 * vb)) - 1, size=n, dtype=np.uint64).astype(vdt)
+    nq = 5000
+    lo = rng.integers(0, space - 1, size=nq, dtype=np.uint64).astype(kdt)
+    w = np.floor(np.exp2(rng.uniform(0, 8 * kb - 4, size=nq))).astype(np.uint64).astype(kdt)
+    hi = (lo + w).astype(kdt)
+    hi = np.where(hi < lo, kdt(space - 1), hi)   # clamp wrapped upper bounds
+    lo[:4] = [0, 5, 0, space - 1]
+    hi[:4] = [space - 1, 4, 0, space - 1]        # full range, lo > hi, point ranges
+    if n:
+        lo[4], hi[4] = keys[0], keys[-1]
+    t = bpt.Tree(keys, vals, kb, vb)
+    want_first, want_cnt = oracle.range_closed(keys, lo, hi)
+    want_ids = _ids_expected(vals, want_first, want_cnt)
+    for mode in MODES:
+        first, cnt, offs, ids = t.range_search(lo, hi, group_mode=mode)
+        assert np.array_equal(cnt, want_cnt), mode
+        assert np.array_equal(first, want_first), mode
+        assert np.array_equal(ids, want_ids), mode
+
+
+def test_rect_golden_fixture(bpt):
+    d = np.load(os.path.join(GOLDEN, "rect_small.npz"))
+    t = bpt.Tree(d["keys"], d["kid"], 8, 4)
+    for mode in MODES:
+        c, o, ids = t.rect_search(d["rects"], group_mode=mode)
+        assert np.array_equal(c, d["count"]) and np.array_equal(ids, d["ids"]), mode
+
+
+@pytest.mark.parametrize("npts,nr,lo_exp,hi_exp", [(1 << 16, 3000, 8, 32), (1 << 20, 20000, 8, 28),
+                                                    (1 << 20, 2000, 20, 32)])
+def test_rect_matches_oracle(bpt, oracle, wl, npts, nr, lo_exp, hi_exp):
+    x, y, ids, keys, kid = wl.points_2d(npts, seed=npts)
+    r = wl.rects(nr, seed=nr, lo_exp=lo_exp, hi_exp=hi_exp)
+    r[0] = [0, 0, 0xFFFFFFFF, 0xFFFFFFFF]        # everything
+    r[1] = [10, 10, 5, 20]                         # x0 > x1: empty
+    r[2] = [x[7], y[7], x[7], y[7]]                # a single point
+    want_c, want_o, want_ids = oracle.rect_morton(keys, kid, r)
+    # brute force pins the Morton oracle on a sample
+    bc, _, bids = oracle.rect_bruteforce(x, y, ids, r[:50])
+    assert np.array_equal(bc, want_c[:50])
+    t = bpt.Tree(keys, kid, 8, 4)
+    for mode in MODES:
+        c, o, got = t.rect_search(r, group_mode=mode)
+        assert np.array_equal(c, want_c), mode
+        assert np.array_equal(got, want_ids), mode
+
+
+def test_rect_duplicate_points(bpt, oracle, wl):
+    rng = np.random.default_rng(9)
+    x = rng.integers(0, 64, size=5000, dtype=np.uint64).astype(np.uint32)   # many coincident points
+    y = rng.integers(0, 64, size=5000, dtype=np.uint64).astype(np.uint32)
+    ids = np.arange(5000, dtype=np.uint32)
+    z = wl.morton_np(x, y)
+    order = np.argsort(z, kind="stable")
+    r = np.array([[0, 0, 63, 63], [3, 3, 3, 3], [10, 20, 30, 40], [63, 0, 63, 63]], dtype=np.uint32)
+    want_c, _, want_ids = oracle.rect_bruteforce(x, y, ids, r)
+    t = bpt.Tree(z[order], ids[order], 8, 4)
+    c, o, got = t.rect_search(r)
+    assert np.array_equal(c, want_c) and np.array_equal(got, want_ids)
+
+
+def test_grouping_keys_match_reference_functions(bpt, oracle):
+    rng = np.random.default_rng(10)
+    a = rng.integers(0, 1 << 64, size=3000, dtype=np.uint64)
+    b = rng.integers(0, 1 << 40, size=3000, dtype=np.uint64)
+    assert np.array_equal(bpt.zorder2d(a, b), oracle.zorder_keys(a, b))
+    g = bpt.group2d_keys(a, b)
+    p0 = max(0, oracle.group2d_start_pass(int(a.max()), int(b.max())))
+    for i in range(0, 3000, 97):
+        want = 0
+        for k in range(4):
+            want = (want << 8) | (oracle.bucket2d(int(a[i]), int(b[i]), p0 + k) if p0 + k <= 15 else 0)
+        assert int(g[i]) == want
+    x = rng.integers(0, 1 << 32, size=1000, dtype=np.uint64).astype(np.uint32)
+    y = rng.integers(0, 1 << 32, size=1000, dtype=np.uint64).astype(np.uint32)
+    z = bpt.morton_encode(x, y)
+    assert all(int(z[i]) == oracle.morton2d(int(x[i]), int(y[i])) for i in range(0, 1000, 7))
+
+
+@pytest.mark.parametrize("kb,shift,bits", [(4, 20, 12), (4, 0, 8), (8, 52, 12), (8, 10, 16)])
+def test_radix_group_is_stable_partition(bpt, kb, shift, bits):
+    rng = np.random.default_rng(bits + shift)
+    kdt = np.uint32 if kb == 4 else np.uint64
+    keys = rng.integers(0, (1 << (8 * kb)) - 1, size=70000, dtype=np.uint64).astype(kdt)
+    keys[:1000] = keys[0]
+    perm, starts = bpt.radix_group(keys, shift, bits)
+    digit = ((keys.astype(np.uint64) >> np.uint64(shift)) & np.uint64((1 << bits) - 1)).astype(np.int64)
+    want = np.argsort(digit, kind="stable")
+    assert np.array_equal(perm.astype(np.int64), want)
+    cnt = np.bincount(digit, minlength=1 << bits)
+    assert np.array_equal(starts, np.concatenate([[0], np.cumsum(cnt)]).astype(np.uint64))
+
+
+def test_exclusive_scan(bpt):
+    rng = np.random.default_rng(11)
+    for n in (1, 7, 2048, 2049, 5_000_003):
+        c = rng.integers(0, 1000, size=n, dtype=np.uint64)
+        off, total = bpt.exclusive_scan(c)
+        assert total == int(c.sum())
+        assert np.array_equal(off, np.concatenate([[0], np.cumsum(c)[:-1]]).astype(np.uint64))
