@@ -89,6 +89,9 @@ SIGNATURES = [
     ("bpt_exclusive_scan_u64", C.c_int, [vp, C.c_uint64, vp, u64p, vp]),
     ("bpt_range_count_grouped", C.c_int, [vp, vp, vp, C.c_uint64, C.c_int, vp, vp, vp]),
     ("bpt_radix_group", C.c_int, [vp, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32, vp, vp, vp]),
+    ("bpt_sort_perm", C.c_int, [vp, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32, vp, vp]),
+    ("bpt_gather", C.c_int, [vp, C.c_uint32, vp, C.c_uint64, vp, vp]),
+    ("bpt_sort_pairs", C.c_int, [vp, vp, C.c_uint32, C.c_uint32, C.c_uint64, vp]),
     ("bpt_morton_encode", C.c_int, [vp, vp, C.c_uint64, vp, vp]),
     ("bpt_zorder2d", C.c_int, [vp, vp, C.c_uint64, vp, vp]),
     ("bpt_group2d_keys", C.c_int, [vp, vp, C.c_uint64, vp, vp]),
@@ -408,3 +411,14 @@ def group2d_keys(a, b):
     out = DeviceArray(da.n, np.uint32)
     check(lib().bpt_group2d_keys(da.ptr, db.ptr, da.n, out.ptr, None))
     return out.to_host()
+
+
+def sort_pairs(keys, vals=None):
+    """Sorts (key, value) pairs by key on the device; returns host arrays."""
+    keys = np.ascontiguousarray(keys)
+    dk = DeviceArray.from_host(keys)
+    dv = DeviceArray.from_host(np.ascontiguousarray(vals)) if vals is not None else None
+    vb = dv.dtype.itemsize if dv is not None else 4
+    check(lib().bpt_sort_pairs(dk.ptr, dv.ptr if dv is not None else None, keys.dtype.itemsize, vb,
+                               dk.n, None))
+    return dk.to_host(), (dv.to_host() if dv is not None else None)

@@ -52,6 +52,14 @@ def build(force: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-6000:]}")
     os.replace(tmp, LIB)
+    # the drop-in driver with the reference's command line, linked to the library
+    exe = os.path.join(HERE, "bptree_main")
+    cmd = [HIPCC, *FLAGS, os.path.join(HERE, "examples", "bptree_main.cpp"), "-o", exe + ".tmp",
+           "-L", HERE, "-lbptree_amd", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"bptree_main failed to build:\n{r.stderr[-6000:]}")
+    os.replace(exe + ".tmp", exe)
     return LIB
 
 

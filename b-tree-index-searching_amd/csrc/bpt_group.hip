@@ -647,6 +647,79 @@ bpt_status bpt_radix_group(const void *d_keys, uint32_t key_bytes, uint64_t n, u
     return st;
 }
 
+bpt_status bpt_sort_perm(const void *d_keys, uint32_t key_bytes, uint64_t n, uint32_t lo_bit, uint32_t hi_bit,
+                         uint32_t *d_perm, bpt_stream stream) {
+    BPT_CHECK(key_bytes == 4 || key_bytes == 8, BPT_ERR_INVALID, "key_bytes %u", key_bytes);
+    BPT_CHECK(lo_bit < hi_bit && hi_bit <= key_bytes * 8, BPT_ERR_INVALID, "bit range [%u, %u)", lo_bit, hi_bit);
+    BPT_CHECK(n < (1ull << 32), BPT_ERR_CAPACITY, "sort of more than 2^32 elements");
+    BPT_CHECK(n == 0 || (d_keys && d_perm), BPT_ERR_INVALID, "NULL argument");
+    if (n == 0) return BPT_OK;
+    hipStream_t s = as_stream(stream);
+    const uint64_t ws_bytes = sort_ws_bytes(n, key_bytes);
+    void *ws = nullptr;
+    BPT_HIP(hipMalloc(&ws, ws_bytes));
+    uint32_t *perm = nullptr;
+    bpt_status st;
+    if (key_bytes == 4) {
+        uint32_t *ks = nullptr;
+        st = sort_pairs<uint32_t>((const uint32_t *)d_keys, nullptr, n, lo_bit, hi_bit, ws, ws_bytes, &ks, &perm, s);
+    } else {
+        uint64_t *ks = nullptr;
+        st = sort_pairs<uint64_t>((const uint64_t *)d_keys, nullptr, n, lo_bit, hi_bit, ws, ws_bytes, &ks, &perm, s);
+    }
+    if (st == BPT_OK) {
+        hipError_t e = hipMemcpyAsync(d_perm, perm, n * 4, hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) st = hip_fail(e, "sort perm copy", __FILE__, __LINE__);
+    }
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(ws);
+    return st;
+}
+
+bpt_status bpt_gather(const void *d_src, uint32_t elem_bytes, const uint32_t *d_idx, uint64_t n, void *d_dst,
+                      bpt_stream stream) {
+    BPT_CHECK(elem_bytes == 4 || elem_bytes == 8, BPT_ERR_INVALID, "elem_bytes %u", elem_bytes);
+    BPT_CHECK(n == 0 || (d_src && d_idx && d_dst), BPT_ERR_INVALID, "NULL argument");
+    if (n == 0) return BPT_OK;
+    hipStream_t s = as_stream(stream);
+    const uint32_t g = (uint32_t)std::min<uint64_t>((n + 255) / 256, (uint64_t)device_cu_count() * 16);
+    if (elem_bytes == 4) k_gather<uint32_t><<<g, 256, 0, s>>>((const uint32_t *)d_src, d_idx, n, (uint32_t *)d_dst);
+    else k_gather<uint64_t><<<g, 256, 0, s>>>((const uint64_t *)d_src, d_idx, n, (uint64_t *)d_dst);
+    BPT_HIP(hipGetLastError());
+    return BPT_OK;
+}
+
+bpt_status bpt_sort_pairs(void *d_keys, void *d_vals, uint32_t key_bytes, uint32_t val_bytes, uint64_t n,
+                          bpt_stream stream) {
+    BPT_CHECK(val_bytes == 4 || val_bytes == 8, BPT_ERR_INVALID, "val_bytes %u", val_bytes);
+    if (n < 2) return BPT_OK;
+    hipStream_t s = as_stream(stream);
+    uint32_t *perm = nullptr;
+    void *tmp = nullptr;
+    BPT_HIP(hipMalloc(&perm, n * 4));
+    hipError_t e = hipMalloc(&tmp, n * std::max(key_bytes, val_bytes));
+    if (e != hipSuccess) {
+        (void)hipFree(perm);
+        return hip_fail(e, "hipMalloc(sort tmp)", __FILE__, __LINE__);
+    }
+    bpt_status st = bpt_sort_perm(d_keys, key_bytes, n, 0, key_bytes * 8, perm, stream);
+    if (st == BPT_OK) st = bpt_gather(d_keys, key_bytes, perm, n, tmp, stream);
+    if (st == BPT_OK) {
+        e = hipMemcpyAsync(d_keys, tmp, n * key_bytes, hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) st = hip_fail(e, "sort keys copy", __FILE__, __LINE__);
+    }
+    if (st == BPT_OK && d_vals) st = bpt_gather(d_vals, val_bytes, perm, n, tmp, stream);
+    if (st == BPT_OK && d_vals) {
+        e = hipMemcpyAsync(d_vals, tmp, n * val_bytes, hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) st = hip_fail(e, "sort values copy", __FILE__, __LINE__);
+    }
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(perm);
+    (void)hipFree(tmp);
+    return st;
+}
+
 bpt_status bpt_morton_encode(const uint32_t *d_x, const uint32_t *d_y, uint64_t n, uint64_t *d_z,
                              bpt_stream stream) {
     BPT_CHECK(n == 0 || (d_x && d_y && d_z), BPT_ERR_INVALID, "NULL argument");

@@ -184,6 +184,19 @@ bpt_status bpt_range_count_grouped(const bpt_tree *t, const void *d_lo, const vo
 bpt_status bpt_radix_group(const void *d_keys, uint32_t key_bytes, uint64_t n, uint32_t shift,
                            uint32_t bits, uint32_t *d_perm, uint64_t *d_bucket_start,
                            bpt_stream s);
+/* Stable sort permutation of n keys by bits [lo_bit, hi_bit): d_perm[j] =
+ * index of the j-th element in sorted order (LSD radix, 8 bits a pass).
+ * Synchronizes s. */
+bpt_status bpt_sort_perm(const void *d_keys, uint32_t key_bytes, uint64_t n, uint32_t lo_bit,
+                         uint32_t hi_bit, uint32_t *d_perm, bpt_stream s);
+/* d_dst[i] = d_src[d_idx[i]] for 4- or 8-byte elements. */
+bpt_status bpt_gather(const void *d_src, uint32_t elem_bytes, const uint32_t *d_idx, uint64_t n,
+                      void *d_dst, bpt_stream s);
+/* In-place stable sort of (key, value) pairs by key on the device, the input
+ * preparation the reference does with cub::DeviceRadixSort::SortPairs
+ * (main.cu:106-118). d_vals may be NULL. Synchronizes s. */
+bpt_status bpt_sort_pairs(void *d_keys, void *d_vals, uint32_t key_bytes, uint32_t val_bytes,
+                          uint64_t n, bpt_stream s);
 /* 64-bit Morton code, x in even bits, y in odd bits (zOrder2d.cu:27-54). */
 bpt_status bpt_morton_encode(const uint32_t *d_x, const uint32_t *d_y, uint64_t n,
                              uint64_t *d_z, bpt_stream s);

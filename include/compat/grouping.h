@@ -1,0 +1,28 @@
+/* Compatibility header: the reference's grouping.h (grouping.h:6-28).
+ * The partitions run on the device; group lists follow the reference's
+ * host bookkeeping (grouping.cu:855-872, 1333-1352, 1746-1791). Within a
+ * group the order is by key (the reference's is arbitrary). */
+#ifndef __INCLUDED_GROUPING_H__
+#define __INCLUDED_GROUPING_H__
+
+#include <vector>
+
+#include "utils.h"
+
+/* grouping.cu:25-29 */
+int bucketIdentify(ulli input, int nPass);
+
+/* grouping.h:17-18: host input; returns a device pointer (hipMalloc) to the
+ * grouped keys that the caller frees; fills the group lists. */
+void *genGrouping(ulli *input, ulli inputSize, std::vector<ulli> &groupList,
+                  std::vector<ulli> &groupPrefix, ulli Local_Group_Size);
+
+/* grouping.h:20-28: host inputs; new[]'d host outputs. */
+void genGroupingWithOnedata(ulli *input, ulli *input_data, ulli inputSize, ulli *&output,
+                            ulli *&output_data, std::vector<ulli> &groupList,
+                            std::vector<ulli> &groupPrefix, ulli Local_Group_Size);
+void genGroupingWithTwodata(ulli *input, ulli *input_data, ulli *input_data2, ulli inputSize,
+                            ulli *&output, ulli *&output_data, ulli *&output_data2,
+                            std::vector<ulli> &groupList, std::vector<ulli> &groupPrefix,
+                            ulli Local_Group_Size);
+#endif

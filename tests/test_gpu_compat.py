@@ -1,0 +1,64 @@
+"""GPU: the reference-named C++ call surface (include/compat/*.h) through the
+drop-in driver bptree_main, which takes the reference's command line and raw
+uint64 files (main.cu:19-360). Unsorted input exercises the on-device
+pre-sort; every search form is checked against the oracle."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "b-tree-index-searching_amd", "bptree_main")
+SENT = 0xFFFFFFFFFFFFFFFF
+
+
+def _write(path, a):
+    np.ascontiguousarray(a, dtype=np.uint64).tofile(path)
+
+
+def test_driver_point_and_range(tmp_path, oracle):
+    rng = np.random.default_rng(21)
+    n = 300_000
+    keys = np.unique(rng.integers(1, 890_000_000_000, size=n, dtype=np.uint64))
+    n = len(keys)
+    data = rng.integers(0, 1 << 62, size=n, dtype=np.uint64)
+    perm = rng.permutation(n)                         # unsorted on disk, as the reference reads it
+    _write(tmp_path / "k", keys[perm])
+    _write(tmp_path / "d", data[perm])
+    nq = 100_000
+    q = np.concatenate([keys[rng.integers(0, n, nq // 2)], rng.integers(0, 890_000_000_000, nq - nq // 2,
+                                                                          dtype=np.uint64)])
+    _write(tmp_path / "q", q)
+    out = str(tmp_path / "res")
+    r = subprocess.run([EXE, str(n), str(tmp_path / "k"), str(tmp_path / "d"), str(nq), str(tmp_path / "q"),
+                        "-o", out], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "Found:" in r.stdout
+    want = oracle.point_closed(keys, data, q, SENT)
+    for form in ("plain", "grouped"):
+        got = np.fromfile(f"{out}.{form}.bin", dtype=np.uint64)
+        assert np.array_equal(got, want), form
+
+    s = rng.integers(0, 890_000_000_000, size=nq, dtype=np.uint64)
+    e = s + np.floor(np.exp2(rng.uniform(0, 36, size=nq))).astype(np.uint64)
+    _write(tmp_path / "s", s)
+    _write(tmp_path / "e", e)
+    r = subprocess.run([EXE, str(n), str(tmp_path / "k"), str(tmp_path / "d"), str(nq), str(tmp_path / "s"),
+                        str(tmp_path / "e"), "-o", out], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    _, want_c = oracle.range_closed(keys, s, e)
+    for form in ("range", "range2d", "rangez"):
+        got = np.fromfile(f"{out}.{form}.bin", dtype=np.uint64)
+        assert np.array_equal(got, want_c), form
+
+
+def test_sort_pairs(bpt):
+    rng = np.random.default_rng(22)
+    for dt in (np.uint32, np.uint64):
+        k = rng.integers(0, 1 << 20, size=200_001, dtype=np.uint64).astype(dt)
+        v = np.arange(len(k), dtype=np.uint64)
+        sk, sv = bpt.sort_pairs(k, v)
+        order = np.argsort(k, kind="stable")
+        assert np.array_equal(sk, k[order]) and np.array_equal(sv, v[order])
