@@ -95,6 +95,18 @@ SIGNATURES = [
     ("bpt_morton_encode", C.c_int, [vp, vp, C.c_uint64, vp, vp]),
     ("bpt_zorder2d", C.c_int, [vp, vp, C.c_uint64, vp, vp]),
     ("bpt_group2d_keys", C.c_int, [vp, vp, C.c_uint64, vp, vp]),
+    ("bpt_forest_build", C.c_int, [vp, C.c_uint32, vp, vp, u64p, C.POINTER(vp), vp]),
+    ("bpt_forest_free", C.c_int, [vp]),
+    ("bpt_forest_info", C.c_int, [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), u64p, u64p]),
+    ("bpt_batch_prepare", C.c_int, [vp, C.c_uint32, C.c_uint32, vp, u64p, vp, vp, vp, u64p, vp, vp,
+                                    C.POINTER(vp)]),
+    ("bpt_batch_run", C.c_int, [vp, C.c_int, vp]),
+    ("bpt_batch_free", C.c_int, [vp]),
+    ("bpt_batch_info", C.c_int, [vp, C.POINTER(C.c_uint32), u64p, u64p]),
+    ("bpt_forest_autotune", C.c_int, [C.c_uint32, vp, vp, u64p, vp, u64p, vp, vp, vp, u64p, vp, vp, C.c_int,
+                                      vp, C.c_char_p, C.c_uint64]),
+    ("bpt_synth_forest", C.c_int, [C.c_uint64, C.c_uint32, u64p, C.c_uint32, C.c_uint32, u64p, C.c_uint32,
+                                   C.c_uint32, vp, vp, vp, vp, vp, vp]),
     ("bpt_rect_count", C.c_int, [vp, vp, C.c_uint64, C.c_int, vp, vp]),
     ("bpt_rect_ids", C.c_int, [vp, vp, C.c_uint64, C.c_int, vp, vp, vp]),
 ]
@@ -422,3 +434,93 @@ def sort_pairs(keys, vals=None):
     check(lib().bpt_sort_pairs(dk.ptr, dv.ptr if dv is not None else None, keys.dtype.itemsize, vb,
                                dk.n, None))
     return dk.to_host(), (dv.to_host() if dv is not None else None)
+
+
+class BatchConfig(C.Structure):
+    _fields_ = [("fanout", C.c_uint32), ("leaf_cap", C.c_uint32), ("lds_bytes", C.c_uint32),
+                ("chunk", C.c_uint32), ("blocks_per_cu", C.c_uint32)]
+
+
+def _offp(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.uint64).ctypes.data_as(u64p)
+
+
+class Forest:
+    """Thousands of 8-byte-key trees in one device arena (bpt_forest_*)."""
+
+    def __init__(self, dkeys, dvals, key_off, fanout=16, lds_bytes=32768, stream=None):
+        self.key_off = np.ascontiguousarray(key_off, dtype=np.uint64)
+        self.ntrees = len(self.key_off) - 1
+        cfg = BatchConfig(fanout, 8, lds_bytes, 0, 0)
+        h = vp()
+        check(lib().bpt_forest_build(C.byref(cfg), self.ntrees, dkeys.ptr, dvals.ptr,
+                                     _offp(self.key_off), C.byref(h), stream))
+        self.h = h.value
+        self.fanout = fanout
+
+    def close(self):
+        if getattr(self, "h", None) and not _exiting:
+            lib().bpt_forest_free(self.h)
+        self.h = None
+
+    __del__ = close
+
+
+class Batch:
+    """A prepared persistent-launch batch over a forest (bpt_batch_*)."""
+
+    def __init__(self, forest, dq=None, q_off=None, dout=None, dlo=None, dhi=None, r_off=None,
+                 dfirst=None, dcount=None, chunk=0, blocks_per_cu=0):
+        self._keep = (np.ascontiguousarray(q_off, dtype=np.uint64) if q_off is not None else None,
+                      np.ascontiguousarray(r_off, dtype=np.uint64) if r_off is not None else None)
+        h = vp()
+        P = lambda d: d.ptr if d is not None else None  # noqa: E731
+        check(lib().bpt_batch_prepare(forest.h, chunk, blocks_per_cu, P(dq), _offp(self._keep[0]), P(dout),
+                                      P(dlo), P(dhi), _offp(self._keep[1]), P(dfirst), P(dcount), C.byref(h)))
+        self.h = h.value
+        self.forest = forest
+
+    def run(self, graph=False, stream=None):
+        check(lib().bpt_batch_run(self.h, 1 if graph else 0, stream))
+
+    def info(self):
+        ni, npt, nr = C.c_uint32(), C.c_uint64(), C.c_uint64()
+        check(lib().bpt_batch_info(self.h, C.byref(ni), C.byref(npt), C.byref(nr)))
+        return {"items": ni.value, "points": npt.value, "ranges": nr.value}
+
+    def close(self):
+        if getattr(self, "h", None) and not _exiting:
+            lib().bpt_batch_free(self.h)
+        self.h = None
+
+    __del__ = close
+
+
+def synth_forest(seed, key_off, qpk=4, hit_pct=80, r_off=None, w_lo_exp=4, w_hi_exp=16):
+    """Device-generated forest workload; returns device arrays."""
+    key_off = np.ascontiguousarray(key_off, dtype=np.uint64)
+    nt = len(key_off) - 1
+    total = int(key_off[-1])
+    dk = DeviceArray(total, np.uint64)
+    dv = DeviceArray(total, np.uint64)
+    dq = DeviceArray(total * qpk, np.uint64) if qpk else None
+    nr = int(r_off[-1]) if r_off is not None else 0
+    dlo = DeviceArray(nr, np.uint64) if nr else None
+    dhi = DeviceArray(nr, np.uint64) if nr else None
+    check(lib().bpt_synth_forest(seed, nt, _offp(key_off), qpk, hit_pct, _offp(r_off), w_lo_exp, w_hi_exp,
+                                 dk.ptr, dv.ptr, dq.ptr if dq else None, dlo.ptr if dlo else None,
+                                 dhi.ptr if dhi else None, None))
+    return dk, dv, dq, dlo, dhi
+
+
+def forest_autotune(dk, dv, key_off, dq, q_off, dout, dlo=None, dhi=None, r_off=None, dfirst=None,
+                    dcount=None, reps=2):
+    best = BatchConfig()
+    rep = C.create_string_buffer(1 << 16)
+    key_off = np.ascontiguousarray(key_off, dtype=np.uint64)
+    P = lambda d: d.ptr if d is not None else None  # noqa: E731
+    check(lib().bpt_forest_autotune(len(key_off) - 1, dk.ptr, dv.ptr, _offp(key_off), P(dq), _offp(q_off),
+                                    P(dout), P(dlo), P(dhi), _offp(r_off), P(dfirst), P(dcount), reps,
+                                    C.byref(best), rep, len(rep)))
+    import json as _json
+    return ({k: getattr(best, k) for k, _ in BatchConfig._fields_}, _json.loads(rep.value.decode()))

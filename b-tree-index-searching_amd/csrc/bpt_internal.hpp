@@ -345,6 +345,89 @@ __device__ __forceinline__ uint32_t lower_bound_flat(const K *__restrict__ a, ui
     return base + (a[base] < q);
 }
 
+// ---------------------------------------------------------------- cooperative lookup
+// One wave looks up 64 queries (one per lane, `myq`) in tree t.
+// Phase A (one lane per query): the LDS-resident top levels are walked as a
+// single branch-free lower bound over the flattened separators of the lowest
+// LDS level -- the separators of one level, concatenated in node order, are
+// sorted (pads only trail the last node), and the first separator >= q there
+// is exactly the child the root-to-that-level descent reaches.
+// Phase B (eight lanes per query): group g (lanes 8g..8g+7) walks queries
+// 8g..8g+7 through the global levels and the leaf, all eight in flight at
+// each level; lane s of the group reads bytes [s*NB, (s+1)*NB) of a node, so
+// one load instruction fetches eight whole nodes. Leaf: lanes 0-3 hold the
+// keys, lanes 4-7 the values (sizeof(K) == sizeof(V)).
+// OP kOpValue returns the value of the first key equal to the lane's query
+// (else `sentinel`); kOpLowerBound returns the global lower-bound position.
+constexpr int kOpValue = 0;
+constexpr int kOpLowerBound = 1;
+
+template <class K, class V, int F, int LF, bool NT, int OP>
+__device__ __forceinline__ typename std::conditional<OP == kOpValue, V, uint64_t>::type
+coop_lookup64(const TreeView &t, const uint8_t *lds, K myq, V sentinel) {
+    static_assert(sizeof(K) == sizeof(V), "cooperative leaf split needs equal key/value widths");
+    constexpr int NB = F * (int)sizeof(K) / 8;   // node bytes per lane
+    constexpr int LB = LF * (int)sizeof(K) / 4;  // leaf bytes per lane (keys in lanes 0-3)
+    constexpr int KPC = LB / (int)sizeof(K);     // keys per leaf chunk
+    constexpr uint32_t NODE = F * sizeof(K);
+    using R = typename std::conditional<OP == kOpValue, V, uint64_t>::type;
+    const uint32_t lane = threadIdx.x & 63, g = lane >> 3, sub = lane & 7;
+    const int flat_lvl = t.height - t.lds_levels + 1;
+    const bool has_flat = t.lds_levels > 0;
+    uint32_t mynode = 0;
+    if (has_flat) {
+        const K *flat = reinterpret_cast<const K *>(lds + t.off[flat_lvl]);
+        const uint32_t flat_n = (uint32_t)t.count[flat_lvl - 1];
+        const uint32_t c = lower_bound_flat<K>(flat, flat_n, myq);
+        mynode = c < flat_n - 1 ? c : flat_n - 1;
+    }
+    const int top = has_flat ? flat_lvl - 1 : t.height;  // first level walked cooperatively
+    K qk[8];
+    uint32_t node[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        qk[k] = __shfl(myq, (int)(g * 8 + k), 64);
+        node[k] = (uint32_t)__shfl((int)mynode, (int)(g * 8 + k), 64);
+    }
+    for (int lvl = top; lvl >= 1; --lvl) {
+        const uint8_t *lv = t.inner + t.off[lvl] + sub * NB;
+        const uint32_t last = (uint32_t)t.count[lvl - 1] - 1;
+        Chunk<NB> c[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) load_chunk<NB>(lv + (uint64_t)node[k] * NODE, c[k]);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const uint32_t ch = node[k] * F + group8_sum(chunk_count_lt<K, NB>(c[k], qk[k]));
+            node[k] = ch < last ? ch : last;
+        }
+    }
+    Chunk<LB> c[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const uint8_t *lp = t.leaves + (uint64_t)node[k] * t.leaf_stride + sub * LB;
+        if constexpr (NT) load_chunk_nt<LB>(lp, c[k]);
+        else load_chunk<LB>(lp, c[k]);
+    }
+    R res = OP == kOpValue ? (R)sentinel : (R)0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const uint32_t part = sub < 4 ? chunk_count_lt<K, LB>(c[k], qk[k]) : 0u;
+        const uint32_t r = group8_sum(part);  // lower bound inside the leaf, 0..LF
+        if constexpr (OP == kOpValue) {
+            const uint32_t rl = r < LF ? r : LF - 1;
+            const uint32_t src = g * 8 + rl / KPC, comp = rl % KPC;
+            const K key = __shfl(chunk_elem<K, LB>(c[k], comp), (int)src, 64);
+            const V val = __shfl((V)chunk_elem<K, LB>(c[k], comp), (int)(src + 4), 64);
+            const bool hit = r < LF && (uint64_t)node[k] * LF + r < t.n && key == qk[k];
+            if (sub == (uint32_t)k) res = hit ? val : sentinel;
+        } else {
+            const uint64_t p = (uint64_t)node[k] * LF + r;
+            if (sub == (uint32_t)k) res = p < t.n ? p : t.n;
+        }
+    }
+    return res;
+}
+
 // ---------------------------------------------------------------- host helpers
 uint32_t search_grid(const bpt_tree *t, uint64_t work_items);
 bpt_status launch_plain(const bpt_tree *t, const void *d_q, uint64_t nq, void *d_out,

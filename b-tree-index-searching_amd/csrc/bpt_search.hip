@@ -28,83 +28,21 @@ __global__ __launch_bounds__(256) void k_point_plain(TreeView t, const K *__rest
     }
 }
 
-// Cooperative form. A wave takes 64 consecutive queries, one per lane.
-// Phase A (one lane per query): the LDS-resident top levels are walked as a
-// single branch-free lower bound over the flattened separators of the lowest
-// LDS level -- the separators of one level, concatenated in node order, are
-// sorted (pads only trail the last node), and the first separator >= q there
-// is exactly the child the root-to-that-level descent reaches.
-// Phase B (eight lanes per query): group g (lanes 8g..8g+7) walks queries
-// 8g..8g+7 through the global levels and the leaf, all eight in flight at
-// each level; lane s of the group reads bytes [s*NB, (s+1)*NB) of the node,
-// so one load instruction fetches eight whole 128-byte nodes. Leaf: lanes
-// 0-3 hold the keys, lanes 4-7 the values (sizeof(K) == sizeof(V)).
+// Cooperative single-tree kernel: every wave walks 64 consecutive queries
+// at a time with coop_lookup64 (bpt_internal.hpp).
 template <class K, class V, int F, int LF, bool NT>
 __global__ __launch_bounds__(256) void k_point_coop(TreeView t, const K *__restrict__ q,
                                                     V *__restrict__ out, uint64_t nq, V sentinel) {
-    static_assert(sizeof(K) == sizeof(V), "cooperative leaf split needs equal key/value widths");
-    constexpr int NB = F * (int)sizeof(K) / 8;   // node bytes per lane
-    constexpr int LB = LF * (int)sizeof(K) / 4;  // leaf bytes per lane (keys in lanes 0-3)
-    constexpr int KPC = LB / (int)sizeof(K);     // keys per leaf chunk
-    constexpr uint32_t NODE = F * sizeof(K);
     extern __shared__ uint4 lds_raw[];
     uint8_t *lds = reinterpret_cast<uint8_t *>(lds_raw);
     stage_top_levels(t, lds);
-    const uint32_t lane = threadIdx.x & 63, g = lane >> 3, sub = lane & 7;
+    const uint32_t lane = threadIdx.x & 63;
     const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-    // lowest LDS level and its real separator count
-    const int flat_lvl = t.height - t.lds_levels + 1;
-    const bool has_flat = t.lds_levels > 0;
-    const K *flat = reinterpret_cast<const K *>(lds + (has_flat ? t.off[flat_lvl] : 0));
-    const uint32_t flat_n = has_flat ? (uint32_t)t.count[flat_lvl - 1] : 1u;
-    const int top = has_flat ? flat_lvl - 1 : t.height;  // first level walked cooperatively
     for (uint64_t base = wave * 64; base < nq; base += nwaves * 64) {
         const uint64_t my = base + lane;
         const K myq = my < nq ? __builtin_nontemporal_load(&q[my]) : K(0);
-        uint32_t mynode = 0;
-        if (has_flat) {
-            const uint32_t c = lower_bound_flat<K>(flat, flat_n, myq);
-            mynode = c < flat_n - 1 ? c : flat_n - 1;
-        }
-        K qk[8];
-        uint32_t node[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            qk[k] = __shfl(myq, (int)(g * 8 + k), 64);
-            node[k] = (uint32_t)__shfl((int)mynode, (int)(g * 8 + k), 64);
-        }
-        for (int lvl = top; lvl >= 1; --lvl) {
-            const uint8_t *lv = t.inner + t.off[lvl] + sub * NB;
-            const uint32_t last = (uint32_t)t.count[lvl - 1] - 1;
-            Chunk<NB> c[8];
-#pragma unroll
-            for (int k = 0; k < 8; ++k) load_chunk<NB>(lv + (uint64_t)node[k] * NODE, c[k]);
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const uint32_t ch = node[k] * F + group8_sum(chunk_count_lt<K, NB>(c[k], qk[k]));
-                node[k] = ch < last ? ch : last;
-            }
-        }
-        Chunk<LB> c[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const uint8_t *lp = t.leaves + (uint64_t)node[k] * t.leaf_stride + sub * LB;
-            if constexpr (NT) load_chunk_nt<LB>(lp, c[k]);
-            else load_chunk<LB>(lp, c[k]);
-        }
-        V res = sentinel;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const uint32_t part = sub < 4 ? chunk_count_lt<K, LB>(c[k], qk[k]) : 0u;
-            const uint32_t r = group8_sum(part);           // lower bound inside the leaf, 0..LF
-            const uint32_t rl = r < LF ? r : LF - 1;
-            const uint32_t src = g * 8 + rl / KPC, comp = rl % KPC;
-            const K key = __shfl(chunk_elem<K, LB>(c[k], comp), (int)src, 64);
-            const V val = __shfl((V)chunk_elem<K, LB>(c[k], comp), (int)(src + 4), 64);
-            const bool hit = r < LF && (uint64_t)node[k] * LF + r < t.n && key == qk[k];
-            if (sub == (uint32_t)k) res = hit ? val : sentinel;
-        }
+        const V res = coop_lookup64<K, V, F, LF, NT, kOpValue>(t, lds, myq, sentinel);
         if (my < nq) __builtin_nontemporal_store(res, &out[my]);
     }
 }

@@ -149,3 +149,43 @@ def rects(nr: int, seed: int = 0, workload: int = 3, lo_exp: float = 8, hi_exp: 
     x1 = np.clip(x0 + w - 1, 0, U32_SPACE - 1)
     y1 = np.clip(y0 + h - 1, 0, U32_SPACE - 1)
     return np.stack([x0, y0, x1, y1], axis=1).astype(np.uint32)
+
+
+# ---------------------------------------------------------------- forest workload
+# numpy restatement of the device generator (bpt_synth_forest): splitmix64
+# (Steele, Lea, Flood 2014) in a counter-based chain h(seed, stream, tree, i).
+M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def splitmix64(x):
+    x = np.asarray(x, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = x + np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def hmix(seed, stream, tree, i):
+    a = splitmix64(np.uint64(seed) ^ splitmix64(np.uint64(stream)))
+    return splitmix64(splitmix64(a ^ np.uint64(tree)) ^ np.asarray(i, dtype=np.uint64))
+
+
+def synth_tree_keys(seed, tree, n):
+    stride = np.uint64((2 ** 64 - 1) // n)
+    i = np.arange(n, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        return i * stride + hmix(seed, 1, tree, i) % stride
+
+
+def forest_sizes(ntrees, lo_exp=10, hi_exp=20, seed=0, workload=4):
+    """Tree sizes floor(2^U), U uniform in [lo_exp, hi_exp] (log-uniform)."""
+    rng = rng_for(seed, workload, 0)
+    return np.floor(np.exp2(rng.uniform(lo_exp, hi_exp, size=ntrees))).astype(np.uint64)
+
+
+def forest_offsets(sizes, per_key=1, ranges_div=16):
+    key_off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    rc = np.maximum(np.asarray(sizes, dtype=np.uint64) // np.uint64(ranges_div), np.uint64(1))
+    r_off = np.concatenate([[0], np.cumsum(rc)]).astype(np.uint64)
+    return key_off, r_off

@@ -223,6 +223,60 @@ bpt_status bpt_rect_count(bpt_tree *t, const uint32_t *d_rects, uint64_t nr, int
 bpt_status bpt_rect_ids(bpt_tree *t, const uint32_t *d_rects, uint64_t nr, int group_mode,
                         const uint64_t *d_offsets, uint32_t *d_ids, bpt_stream s);
 
+/* ------------------------------------------------------------------ */
+/* Multi-tree batches (no reference counterpart)                        */
+/* ------------------------------------------------------------------ */
+/* A forest: ntrees independent trees of 8-byte keys and values in one
+ * device arena. Tree t's keys (sorted, unique or not) and values are
+ * d_keys/d_vals[h_key_off[t] .. h_key_off[t+1]) (host offsets). */
+typedef struct bpt_forest bpt_forest;
+typedef struct bpt_batch bpt_batch;
+typedef struct bpt_batch_config {
+    uint32_t fanout;        /* 16, 32 or 64 (0: 16) */
+    uint32_t leaf_cap;      /* 8 (one 128-byte line of keys + values) */
+    uint32_t lds_bytes;     /* LDS budget for each tree's top levels (0: 32 KiB) */
+    uint32_t chunk;         /* queries per work item (0: 8192) */
+    uint32_t blocks_per_cu; /* persistent grid (0: 8) */
+} bpt_batch_config;
+bpt_status bpt_forest_build(const bpt_batch_config *cfg, uint32_t ntrees, const uint64_t *d_keys,
+                            const uint64_t *d_vals, const uint64_t *h_key_off, bpt_forest **out,
+                            bpt_stream s);
+bpt_status bpt_forest_free(bpt_forest *f);
+bpt_status bpt_forest_info(const bpt_forest *f, uint32_t *ntrees, uint32_t *fanout, uint64_t *leaf_bytes,
+                           uint64_t *inner_bytes);
+/* A batch over a forest: point queries of tree t at d_q[h_q_off[t] ..
+ * h_q_off[t+1]) (results at the same positions of d_out; value or all-ones)
+ * and range queries [d_lo, d_hi] of tree t at h_r_off[t] .. h_r_off[t+1]
+ * (d_first = lower bound, d_count = inclusive count). Either part may be
+ * absent (NULL offsets). The work list is built once. */
+bpt_status bpt_batch_prepare(bpt_forest *f, uint32_t chunk, uint32_t blocks_per_cu, const uint64_t *d_q,
+                             const uint64_t *h_q_off, uint64_t *d_out, const uint64_t *d_lo,
+                             const uint64_t *d_hi, const uint64_t *h_r_off, uint64_t *d_first,
+                             uint64_t *d_count, bpt_batch **out);
+/* One persistent launch over every work item; use_graph != 0 replays a
+ * hipGraph captured on the first call. */
+bpt_status bpt_batch_run(bpt_batch *b, int use_graph, bpt_stream s);
+bpt_status bpt_batch_free(bpt_batch *b);
+bpt_status bpt_batch_info(const bpt_batch *b, uint32_t *nitems, uint64_t *npoints, uint64_t *nranges);
+/* Times fanout {16,32,64} x LDS budget {16,32,64} KiB x chunk {2K,8K,32K}
+ * on the given batch (reps timed runs after one warm-up each) and returns
+ * the fastest configuration; `report` receives a JSON array of all timings. */
+bpt_status bpt_forest_autotune(uint32_t ntrees, const uint64_t *d_keys, const uint64_t *d_vals,
+                               const uint64_t *h_key_off, const uint64_t *d_q, const uint64_t *h_q_off,
+                               uint64_t *d_out, const uint64_t *d_lo, const uint64_t *d_hi,
+                               const uint64_t *h_r_off, uint64_t *d_first, uint64_t *d_count, int reps,
+                               bpt_batch_config *best, char *report, uint64_t report_len);
+/* Synthetic forest workload from a counter-based generator (splitmix64):
+ * tree t's key i = i*stride + h(seed,1,t,i) mod stride with
+ * stride = (2^64-1) / n_t (sorted, unique, uniform per stride), value = i;
+ * qpk point queries per key, hit_pct% of them keys of the tree; range j of
+ * tree t starts at a uniform rank and spans 2^e + (r mod 2^e) ranks,
+ * e uniform in [w_lo_exp, w_hi_exp). h_key_off[0] must be 0. Synchronizes s. */
+bpt_status bpt_synth_forest(uint64_t seed, uint32_t ntrees, const uint64_t *h_key_off, uint32_t qpk,
+                            uint32_t hit_pct, const uint64_t *h_r_off, uint32_t w_lo_exp, uint32_t w_hi_exp,
+                            uint64_t *d_keys, uint64_t *d_vals, uint64_t *d_q, uint64_t *d_lo,
+                            uint64_t *d_hi, bpt_stream s);
+
 #ifdef __cplusplus
 }
 #endif
