@@ -91,6 +91,190 @@ def cpu_baseline(keys, vals, q, gpu_out, target_s=10.0):
     }
 
 
+def common_line(metric, value, unit, args, ms, dtype, config, roofline, extra):
+    line = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": dtype, "data": "synthetic (seeded)", "config": config,
+            "roofline": roofline}
+    line.update(extra)
+    return line
+
+
+def bench_c3(args):
+    """2-D range search over 32M Morton-keyed points, 4M rectangles, three
+    query orders (basic / 2-D grouping / Z-order); count + sorted ids."""
+    npts, nr = 1 << 25, 1 << 22
+    t0 = time.perf_counter()
+    x, y, ids, keys, kid = workloads.points_2d(npts)
+    rects = workloads.rects(nr)
+    log(f"generated {npts} points, {nr} rectangles in {time.perf_counter() - t0:.1f} s")
+    tree = bptree.Tree(keys, kid, 8, 4)
+    dr = bptree.DeviceArray.from_host(rects.reshape(-1))
+    dcount = bptree.DeviceArray(nr, np.uint64)
+    doff = bptree.DeviceArray(nr, np.uint64)
+    L = bptree.lib()
+    forms, results = {}, {}
+    for mode, name in ((0, "basic"), (1, "grid2d"), (2, "zorder")):
+        def step():
+            bptree.check(L.bpt_rect_count(tree.h, dr.ptr, nr, mode, dcount.ptr, None))
+            tot = bptree.C.c_uint64(0)
+            bptree.check(L.bpt_exclusive_scan_u64(dcount.ptr, nr, doff.ptr, bptree.C.byref(tot), None))
+            dids = bptree.DeviceArray(tot.value, np.uint32)
+            bptree.check(L.bpt_rect_ids(tree.h, dr.ptr, nr, mode, doff.ptr, dids.ptr, None))
+            return dids
+        for _ in range(args.warmup):
+            step().free()
+        bptree.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            d = step()
+            bptree.sync()
+            d.free()
+        dt = (time.perf_counter() - t0) / args.steps
+        d = step()
+        results[name] = (dcount.to_host(), d.to_host())
+        d.free()
+        forms[name] = {"rects_per_s": nr / dt, "ms": dt * 1e3}
+        log(f"c3 {name}: {dt * 1e3:.2f} ms, {nr / dt / 1e6:.1f} M rects/s")
+    base = results["basic"]
+    for name in ("grid2d", "zorder"):
+        forms[name]["identical_to_basic"] = bool(np.array_equal(base[0], results[name][0]) and
+                                                 np.array_equal(base[1], results[name][1]))
+    cnt = base[0]
+    extra = {"forms": forms, "hits_total": int(cnt.sum()), "hits_mean": float(cnt.mean())}
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        threads = min(16, len(os.sched_getaffinity(0)))
+        sample = 1 << 16
+        t0 = time.perf_counter()
+        c, o, sids = oracle.rect_morton(keys, kid, rects[:sample], threads=threads)
+        dt = time.perf_counter() - t0
+        offs = np.concatenate([[0], np.cumsum(cnt)]).astype(np.int64)
+        mism = int((c != cnt[:sample]).sum()) + int(not np.array_equal(sids, base[1][:offs[sample]]))
+        extra["cpu_baseline"] = {"value": sample / dt, "unit": "rects/s", "cores": threads, "kind": "port",
+                                 "sample": f"first {sample} rectangles, quadtree Morton decomposition + filter "
+                                           f"+ sort (oracle), {dt:.1f} s", "mismatches_vs_gpu": mism}
+    best = forms["basic"]
+    line = common_line("2-D rectangle queries/s @32M points (count + sorted ids)", nr * 1e3 / best["ms"],
+                       "rects/s", args, best["ms"], "u64",
+                       {"workload": "2-D range search: 32M points (half uniform, half 16-centre Gaussian "
+                                    "mixture, sigma 2^24) keyed by 64-bit Morton code; 4M rectangles, centres "
+                                    "uniform, sides log-uniform in [2^8, 2^20]", "points": npts, "rects": nr},
+                       {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
+                        "traffic": None, "note": "reported, not gated: bytes depend on the decomposition"},
+                       extra)
+    print(json.dumps(line), flush=True)
+
+
+def bench_c4(args):
+    """4096 independent uint64 trees (sizes log-uniform in [1K, 1M]), 4x
+    size point queries per tree (80% hits) + size/16 ranges per tree, one
+    persistent hipGraph-captured launch, autotuned configuration."""
+    ntrees = 4096
+    sizes = workloads.forest_sizes(ntrees, 10, 20)
+    key_off, r_off = workloads.forest_offsets(sizes)
+    q_off = key_off * np.uint64(4)
+    t0 = time.perf_counter()
+    dk, dv, dq, dlo, dhi = bptree.synth_forest(0, key_off, qpk=4, hit_pct=80, r_off=r_off)
+    nq, nr = int(q_off[-1]), int(r_off[-1])
+    log(f"c4: {ntrees} trees, {int(key_off[-1])} keys, {nq} point + {nr} range queries generated in "
+        f"{time.perf_counter() - t0:.1f} s")
+    dout = bptree.DeviceArray(nq, np.uint64)
+    dfirst = bptree.DeviceArray(nr, np.uint64)
+    dcount = bptree.DeviceArray(nr, np.uint64)
+    t0 = time.perf_counter()
+    best, report = bptree.forest_autotune(dk, dv, key_off, dq, q_off, dout, dlo, dhi, r_off, dfirst, dcount,
+                                          reps=1)
+    log(f"autotune {time.perf_counter() - t0:.1f} s -> {best}")
+    forest = bptree.Forest(dk, dv, key_off, fanout=best["fanout"], lds_bytes=best["lds_bytes"])
+    batch = bptree.Batch(forest, dq, q_off, dout, dlo, dhi, r_off, dfirst, dcount, chunk=best["chunk"])
+    for _ in range(args.warmup):
+        batch.run(graph=True)
+    bptree.sync()
+    tm = [bptree.Timer() for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        tm[k].start()
+        batch.run(graph=True)
+        tm[k].stop()
+    bptree.sync()
+    wall = (time.perf_counter() - t0) / args.steps
+    kms = float(np.mean([t.ms() for t in tm]))
+    # points only, for the roofline of the point lookups
+    batch_p = bptree.Batch(forest, dq, q_off, dout, chunk=best["chunk"])
+    batch_p.run(graph=True)
+    bptree.sync()
+    tp = bptree.Timer()
+    tp.start()
+    batch_p.run(graph=True)
+    tp.stop()
+    pms = tp.ms()
+    # baseline: one plain launch per tree (same trees, separate handles)
+    trees = []
+    for t in range(ntrees):
+        a, b = int(key_off[t]), int(key_off[t + 1])
+        d = bptree.BuildDesc(8, 8, 16, 8, 0, 0)
+        h = bptree.vp()
+        bptree.check(bptree.lib().bpt_tree_build(bptree.C.byref(d), dk.ptr + 8 * a, dv.ptr + 8 * a, b - a,
+                                                 bptree.C.byref(h), None))
+        trees.append(h.value)
+    L = bptree.lib()
+    bptree.sync()
+    t0 = time.perf_counter()
+    for t in range(ntrees):
+        qa, qb = int(q_off[t]), int(q_off[t + 1])
+        L.bpt_point_search(trees[t], dq.ptr + 8 * qa, qb - qa, dout.ptr + 8 * qa, 0, None)
+        ra, rb = int(r_off[t]), int(r_off[t + 1])
+        L.bpt_range_count(trees[t], dlo.ptr + 8 * ra, dhi.ptr + 8 * ra, rb - ra, dfirst.ptr + 8 * ra,
+                          dcount.ptr + 8 * ra, None)
+    bptree.sync()
+    per_tree_s = time.perf_counter() - t0
+    for h in trees:
+        L.bpt_tree_free(h)
+    batch.run(graph=True)
+    bptree.sync()
+    extra = {"autotune": {"best": best, "configs_timed": len(report)},
+             "per_tree_launches_s": per_tree_s, "speedup_vs_per_tree_launches": per_tree_s / wall,
+             "points": nq, "ranges": nr, "batch_ms": kms, "points_only_ms": pms}
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        keys, vals = dk.to_host(), dv.to_host()
+        out, cnt = dout.to_host(), dcount.to_host()
+        q, lo, hi = dq.to_host(), dlo.to_host(), dhi.to_host()
+        sample = list(range(0, ntrees, 64)) + [int(np.argmax(sizes))]
+        mism, nsq = 0, 0
+        t0 = time.perf_counter()
+        for t in sample:
+            k = keys[key_off[t]:key_off[t + 1]]
+            v = vals[key_off[t]:key_off[t + 1]]
+            pq = q[q_off[t]:q_off[t + 1]]
+            pt = oracle.PageTree(k, v, 5000)
+            want = pt.point(pq, 0xFFFFFFFFFFFFFFFF, min(16, len(os.sched_getaffinity(0))))
+            pt.close()
+            _, wc = oracle.range_closed(k, lo[r_off[t]:r_off[t + 1]], hi[r_off[t]:r_off[t + 1]])
+            mism += int((want != out[q_off[t]:q_off[t + 1]]).sum()) + int((wc != cnt[r_off[t]:r_off[t + 1]]).sum())
+            nsq += len(pq)
+        dt = time.perf_counter() - t0
+        extra["cpu_baseline"] = {"value": nsq / dt, "unit": "lookups/s", "cores": min(16, len(os.sched_getaffinity(0))),
+                                 "kind": "port", "sample": f"{len(sample)} trees (every 64th + the largest), "
+                                 f"page tree PAGESIZE 5000 point search + closed-form range counts, {dt:.1f} s",
+                                 "mismatches_vs_gpu": mism}
+    bytes_pts = 144 * nq
+    line = common_line("batched point lookups/s @4096 uint64 trees (one persistent hipGraph launch)", nq / wall,
+                       "lookups/s", args, wall * 1e3, "u64",
+                       {"workload": "4096 uint64 trees, sizes log-uniform in [1K, 1M] (stratified-uniform keys, "
+                                    "value = rank); 4x size point queries per tree (80% hits) + size/16 range "
+                                    "queries per tree (widths 2^4..2^16 ranks) in the same launch",
+                        "trees": ntrees, "keys": int(key_off[-1])},
+                       {"bound": "hbm", "achieved": bytes_pts / (pms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": bytes_pts / (pms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                        "kernel": "k_forest_batch (points only)", "kernel_ms": pms, "bytes_per_lookup": 144},
+                       extra)
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,11 +286,17 @@ def main():
     ap.add_argument("--no-forms", action="store_true", help="skip grouped / Zipf side measurements")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run the 1 GiB device copy kernel (HBM counter calibration)")
+    ap.add_argument("--config", default="c2", choices=["c2", "c3", "c4"],
+                    help="c2 (default): 64M-key point search; c3: 2-D rectangles; c4: 4096-tree batch")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is "
                                    "designed (range-shard or replicate) but not built"}))
         sys.exit(2)
+    if args.config == "c3":
+        return bench_c3(args)
+    if args.config == "c4":
+        return bench_c4(args)
 
     n, nq = 1 << args.keys_log2, 1 << args.queries_log2
     dev = bptree.device_info(0)
