@@ -95,6 +95,11 @@ SIGNATURES = [
     ("bpt_morton_encode", C.c_int, [vp, vp, C.c_uint64, vp, vp]),
     ("bpt_zorder2d", C.c_int, [vp, vp, C.c_uint64, vp, vp]),
     ("bpt_group2d_keys", C.c_int, [vp, vp, C.c_uint64, vp, vp]),
+    ("bpt_learned_build", C.c_int, [vp, vp, vp, vp]),
+    ("bpt_learned_params", C.c_int, [vp, C.POINTER(C.c_float), C.c_uint32]),
+    ("bpt_mlp_loss_grad", C.c_int, [C.POINTER(C.c_float), C.c_uint32, C.POINTER(C.c_float),
+                                    C.POINTER(C.c_float), C.c_uint64, C.POINTER(C.c_float),
+                                    C.POINTER(C.c_float)]),
     ("bpt_forest_build", C.c_int, [vp, C.c_uint32, vp, vp, u64p, C.POINTER(vp), vp]),
     ("bpt_forest_free", C.c_int, [vp]),
     ("bpt_forest_info", C.c_int, [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), u64p, u64p]),
@@ -258,6 +263,35 @@ def _dev(a, dtype):
 _KDT = {4: np.uint32, 8: np.uint64}
 
 
+MLP_HIDDEN = 32
+MLP_PARAMS = 3 * MLP_HIDDEN + MLP_HIDDEN * MLP_HIDDEN + MLP_HIDDEN + 1
+
+
+class LearnedConfig(C.Structure):
+    _fields_ = [("leaf_models", C.c_uint32), ("max_window", C.c_uint32), ("train_samples", C.c_uint32),
+                ("train_steps", C.c_uint32), ("lr", C.c_float), ("seed", C.c_uint32), ("root", C.c_uint32)]
+
+
+class LearnedStats(C.Structure):
+    _fields_ = [("initial_loss", C.c_float), ("final_loss", C.c_float), ("train_ms", C.c_double),
+                ("fit_ms", C.c_double), ("breakpoints", C.c_uint32), ("models_used", C.c_uint32),
+                ("fallback_models", C.c_uint32), ("max_window_seen", C.c_uint32),
+                ("fallback_keys", C.c_uint64), ("mean_window", C.c_double)]
+
+
+def mlp_loss_grad(params, x, t):
+    """Loss and gradient of the 1-32-32-1 MLP from the training kernels."""
+    fp = C.POINTER(C.c_float)
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    t = np.ascontiguousarray(t, dtype=np.float32)
+    g = np.empty(MLP_PARAMS, dtype=np.float32)
+    loss = C.c_float(0)
+    check(lib().bpt_mlp_loss_grad(p.ctypes.data_as(fp), MLP_HIDDEN, x.ctypes.data_as(fp), t.ctypes.data_as(fp),
+                                  len(x), g.ctypes.data_as(fp), C.byref(loss)))
+    return float(loss.value), g
+
+
 class Tree:
     """A bulk-built B+tree resident on the device (genBPlusTree, BPTree.cu:127-306)."""
 
@@ -330,6 +364,19 @@ class Tree:
         if own:
             dq.free()
         return r
+
+    # ---- learned position model --------------------------------------------
+    def learned_build(self, leaf_models=65536, max_window=256, train_samples=1 << 20, train_steps=2000,
+                      lr=3e-3, seed=1, root=0, stream=None):
+        cfg = LearnedConfig(leaf_models, max_window, train_samples, train_steps, lr, seed, root)
+        st = LearnedStats()
+        check(lib().bpt_learned_build(self.h, C.byref(cfg), C.byref(st), stream))
+        return {k: getattr(st, k) for k, _ in LearnedStats._fields_}
+
+    def learned_params(self):
+        p = np.empty(MLP_PARAMS, dtype=np.float32)
+        check(lib().bpt_learned_params(self.h, p.ctypes.data_as(C.POINTER(C.c_float)), MLP_PARAMS))
+        return p
 
     # ---- range search ------------------------------------------------------
     def range_count_dev(self, dlo, dhi, dfirst, dcount, group_mode=GROUP_NONE, stream=None):

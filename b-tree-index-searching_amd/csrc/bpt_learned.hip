@@ -1,17 +1,673 @@
-// Learned position model (placeholder until the model is attached).
+// Learned position model attached to a tree (no reference counterpart).
+//
+// Root: an FP32 MLP 1-H-H-1 (ReLU, H = 32) over the feature
+// x = log2(key + 1) / key_bits, trained on the GPU by the kernels below
+// (forward + backward per sample, per-block gradient reduction in LDS, Adam;
+// one training step = three launches captured in a hipGraph and replayed) to
+// predict the rank fraction of a key. Its output picks one of M linear leaf
+// models (j = floor(y * M)). Each leaf model interpolates position between
+// the first and last keys routed to it; its error bounds [elo, ehi] are taken
+// over every key routed to it with the exact query-time arithmetic, so a key
+// present in the tree always lies in its model's window. A query computes
+// the window, takes a branch-free lower bound inside it, and compares.
+// Models whose window exceeds max_window answer through the plain descent.
+//
+// Root evaluation: a ReLU network of one scalar input is piecewise linear in
+// x. After training, the host lists the breakpoints of the first layer
+// (w1 x + b1 = 0) and, on each of those pieces, of the second layer, and
+// tabulates y = alpha * x + beta per piece; queries find their piece by a
+// branch-free search over the breakpoints in LDS and do one FMA. The
+// layer-by-layer forward is kept as a root mode for comparison. Routing is
+// whatever function the chosen mode computes; leaf models are fitted with
+// that same function, which is what keeps results exact.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <vector>
+
 #include "bpt_internal.hpp"
 
 namespace bpt {
 
-struct LearnedModel {};
+constexpr int kH = 32;
+constexpr int kNP = 3 * kH + kH * kH + kH + 1;  // 1153 parameters
+constexpr int kMaxBreaks = kH + kH * (kH + 1);  // 1088
+constexpr int kTile = 64;                       // samples per block tile (one wave)
 
-void learned_free(LearnedModel *m) { delete m; }
+enum : uint32_t { kModelEmpty = 1, kModelFallback = 2 };
+
+struct LeafModel {  // 48 bytes
+    uint64_t k0, k1;
+    double p0, slope;
+    int32_t elo, ehi;
+    uint32_t flags, cnt;
+};
+
+struct LearnedView {
+    const LeafModel *models;
+    const float *params;  // device, layer-by-layer mode
+    const float *bx;      // breakpoints (nbreak)
+    const float *alpha, *beta;  // nbreak + 1 pieces
+    uint32_t nmodels, nbreak, root, pad_;
+};
+
+struct LearnedModel {
+    LearnedView view{};
+    LeafModel *d_models = nullptr;
+    float *d_params = nullptr, *d_pwl = nullptr;
+    std::vector<float> params;
+    bpt_learned_stats stats{};
+    uint32_t max_window = 256;
+};
+
+void learned_free(LearnedModel *m) {
+    if (!m) return;
+    (void)hipFree(m->d_models);
+    (void)hipFree(m->d_params);
+    (void)hipFree(m->d_pwl);
+    delete m;
+}
+
+template <class K>
+__device__ __forceinline__ float key_feature(K key) {
+    return log2f((float)key + 1.0f) * (1.0f / (8.0f * sizeof(K)));
+}
+
+// ------------------------------------------------------------------ training
+// Forward + backward of one 64-sample tile per iteration; per-block sums of
+// the gradient (and of the loss) go to partial[block][kNP + 1].
+__global__ __launch_bounds__(kTile) void k_mlp_grad(const float *__restrict__ params, const float *__restrict__ xs,
+                                                    const float *__restrict__ ts, uint32_t m, float inv_m,
+                                                    float *__restrict__ partial) {
+    __shared__ float P[kNP];
+    __shared__ float s_h1[kTile][kH + 1], s_d2[kTile][kH + 1], s_d1[kTile][kH + 1], s_h2[kTile][kH + 1];
+    __shared__ float s_x[kTile], s_dy[kTile], s_loss[kTile];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < kNP; i += kTile) P[i] = params[i];
+    __syncthreads();
+    const float *w1 = P, *b1 = P + kH, *W2 = P + 2 * kH, *b2 = W2 + kH * kH, *w3 = b2 + kH, *b3 = w3 + kH;
+    constexpr int PER = kH * kH / kTile;  // W2 entries per thread
+    float gW2[PER];  // W2 entries tid*PER .. tid*PER+PER-1
+    for (int r = 0; r < PER; ++r) gW2[r] = 0.f;
+    float gv[4] = {0.f, 0.f, 0.f, 0.f};  // tid < 32: w1, b1, b2, w3 of unit tid
+    float gb3 = 0.f, gloss = 0.f;
+    for (uint64_t tile = blockIdx.x; tile * kTile < m; tile += gridDim.x) {
+        const uint64_t s = tile * kTile + tid;
+        const bool valid = s < m;
+        const float x = valid ? xs[s] : 0.f, t = valid ? ts[s] : 0.f;
+        float a1[kH], h1[kH], a2[kH];
+#pragma unroll
+        for (int k = 0; k < kH; ++k) {
+            a1[k] = fmaf(w1[k], x, b1[k]);
+            h1[k] = a1[k] > 0.f ? a1[k] : 0.f;
+        }
+        float y = b3[0];
+#pragma unroll 4
+        for (int j = 0; j < kH; ++j) {
+            float a = b2[j];
+#pragma unroll
+            for (int k = 0; k < kH; ++k) a = fmaf(W2[j * kH + k], h1[k], a);
+            a2[j] = a;
+            const float h = a > 0.f ? a : 0.f;
+            s_h2[tid][j] = h;
+            y = fmaf(w3[j], h, y);
+        }
+        const float e = y - t;
+        const float dy = valid ? 2.f * e * inv_m : 0.f;
+        s_loss[tid] = valid ? e * e * inv_m : 0.f;
+        float d2[kH];
+#pragma unroll
+        for (int j = 0; j < kH; ++j) {
+            d2[j] = a2[j] > 0.f ? dy * w3[j] : 0.f;
+            s_d2[tid][j] = d2[j];
+            s_h1[tid][j] = h1[j];
+        }
+#pragma unroll 4
+        for (int k = 0; k < kH; ++k) {
+            float d = 0.f;
+#pragma unroll
+            for (int j = 0; j < kH; ++j) d = fmaf(d2[j], W2[j * kH + k], d);
+            s_d1[tid][k] = a1[k] > 0.f ? d : 0.f;
+        }
+        s_x[tid] = x;
+        s_dy[tid] = dy;
+        __syncthreads();
+        for (int r = 0; r < PER; ++r) {
+            const int e2 = tid * PER + r, j = e2 / kH, k = e2 % kH;
+            float acc = 0.f;
+            for (int q = 0; q < kTile; ++q) acc = fmaf(s_d2[q][j], s_h1[q][k], acc);
+            gW2[r] += acc;
+        }
+        if (tid < kH) {
+            float a = 0.f, b = 0.f, c = 0.f, d = 0.f;
+            for (int q = 0; q < kTile; ++q) {
+                a = fmaf(s_d1[q][tid], s_x[q], a);
+                b += s_d1[q][tid];
+                c += s_d2[q][tid];
+                d = fmaf(s_dy[q], s_h2[q][tid], d);
+            }
+            gv[0] += a; gv[1] += b; gv[2] += c; gv[3] += d;
+        } else if (tid == kH) {
+            float a = 0.f, l = 0.f;
+            for (int q = 0; q < kTile; ++q) { a += s_dy[q]; l += s_loss[q]; }
+            gb3 += a;
+            gloss += l;
+        }
+        __syncthreads();
+    }
+    float *out = partial + (uint64_t)blockIdx.x * (kNP + 1);
+    for (int r = 0; r < PER; ++r) out[2 * kH + tid * PER + r] = gW2[r];
+    if (tid < kH) {
+        out[tid] = gv[0];                              // w1
+        out[kH + tid] = gv[1];                         // b1
+        out[2 * kH + kH * kH + tid] = gv[2];           // b2
+        out[3 * kH + kH * kH + tid] = gv[3];           // w3
+    } else if (tid == kH) {
+        out[kNP - 1] = gb3;
+        out[kNP] = gloss;
+    }
+}
+
+__global__ void k_reduce_partials(const float *__restrict__ partial, uint32_t blocks, float *__restrict__ grad) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p > kNP) return;
+    float s = 0.f;
+    for (uint32_t b = 0; b < blocks; ++b) s += partial[(uint64_t)b * (kNP + 1) + p];
+    grad[p] = s;
+}
+
+// Adam (Kingma & Ba 2015); one block so the step counter updates once.
+__global__ __launch_bounds__(1024) void k_adam(float *__restrict__ params, const float *__restrict__ grad,
+                                               float *__restrict__ mom, float *__restrict__ vel,
+                                               uint32_t *__restrict__ step, float lr) {
+    const uint32_t t = *step + 1;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const float c1 = 1.f - powf(b1, (float)t), c2 = 1.f - powf(b2, (float)t);
+    for (int p = threadIdx.x; p < kNP; p += blockDim.x) {
+        const float g = grad[p];
+        const float mm = b1 * mom[p] + (1.f - b1) * g;
+        const float vv = b2 * vel[p] + (1.f - b2) * g * g;
+        mom[p] = mm;
+        vel[p] = vv;
+        params[p] -= lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *step = t;
+}
+
+// ------------------------------------------------------------------ routing
+__device__ __forceinline__ float mlp_forward(const float *P, float x) {
+    const float *w1 = P, *b1 = P + kH, *W2 = P + 2 * kH, *b2 = W2 + kH * kH, *w3 = b2 + kH, *b3 = w3 + kH;
+    float h1[kH];
+#pragma unroll
+    for (int k = 0; k < kH; ++k) {
+        const float a = fmaf(w1[k], x, b1[k]);
+        h1[k] = a > 0.f ? a : 0.f;
+    }
+    float y = b3[0];
+    for (int j = 0; j < kH; ++j) {
+        float a = b2[j];
+#pragma unroll
+        for (int k = 0; k < kH; ++k) a = fmaf(W2[j * kH + k], h1[k], a);
+        y = fmaf(w3[j], a > 0.f ? a : 0.f, y);
+    }
+    return y;
+}
+
+// LDS image: [params kNP][bx nbreak][alpha nbreak+1][beta nbreak+1]
+template <class K>
+__device__ __forceinline__ uint32_t route(const LearnedView &v, const float *sh, K key) {
+    const float x = key_feature<K>(key);
+    float y;
+    if (v.root == 0) {
+        const float *bx = sh + kNP;
+        const uint32_t s = v.nbreak ? lower_bound_flat<float>(bx, v.nbreak, x) : 0u;
+        y = fmaf(sh[kNP + v.nbreak + s], x, sh[kNP + 2 * v.nbreak + 1 + s]);
+    } else if (v.root == 1) {
+        y = mlp_forward(sh, x);
+    } else if (v.root == 2) {
+        y = x;
+    } else {
+        y = (float)key * (1.0f / (sizeof(K) == 4 ? 4294967296.0f : 18446744073709551616.0f));
+    }
+    const float f = floorf(y * (float)v.nmodels);
+    return f <= 0.f ? 0u : (f >= (float)(v.nmodels - 1) ? v.nmodels - 1 : (uint32_t)f);
+}
+
+__device__ __forceinline__ int64_t model_pred(const LeafModel &m, uint64_t key) {
+    return (int64_t)floor(m.p0 + m.slope * (double)(key - m.k0));
+}
+
+__device__ void stage_learned(const LearnedView &v, float *sh) {
+    const uint32_t total = kNP + (v.root == 0 ? 3 * v.nbreak + 2 : 0);
+    for (uint32_t i = threadIdx.x; i < kNP; i += blockDim.x) sh[i] = v.params[i];
+    if (v.root == 0) {
+        for (uint32_t i = threadIdx.x; i < v.nbreak; i += blockDim.x) sh[kNP + i] = v.bx[i];
+        for (uint32_t i = threadIdx.x; i <= v.nbreak; i += blockDim.x) {
+            sh[kNP + v.nbreak + i] = v.alpha[i];
+            sh[kNP + 2 * v.nbreak + 1 + i] = v.beta[i];
+        }
+    }
+    (void)total;
+    __syncthreads();
+}
+
+constexpr size_t kLearnedLds = sizeof(float) * (kNP + 3 * kMaxBreaks + 2);
+
+template <class K, int LF>
+__global__ __launch_bounds__(256) void k_route_keys(TreeView t, LearnedView v, uint32_t *__restrict__ routes,
+                                                    uint32_t *__restrict__ pmin, uint32_t *__restrict__ pmax,
+                                                    uint32_t *__restrict__ cnt) {
+    extern __shared__ float sh_l[];
+    stage_learned(v, sh_l);
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < t.n; i += step) {
+        const uint32_t j = route<K>(v, sh_l, slot_key<K, LF>(t, i));
+        routes[i] = j;
+        atomicMin(&pmin[j], (uint32_t)i);
+        atomicMax(&pmax[j], (uint32_t)i);
+        atomicAdd(&cnt[j], 1u);
+    }
+}
+
+template <class K, int LF>
+__global__ void k_fit_models(TreeView t, uint32_t nmodels, const uint32_t *__restrict__ pmin,
+                             const uint32_t *__restrict__ pmax, const uint32_t *__restrict__ cnt,
+                             LeafModel *__restrict__ models) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nmodels) return;
+    LeafModel m{};
+    m.cnt = cnt[j];
+    if (m.cnt == 0) {
+        m.flags = kModelEmpty;
+    } else {
+        const uint64_t a = pmin[j], b = pmax[j];
+        m.k0 = (uint64_t)slot_key<K, LF>(t, a);
+        m.k1 = (uint64_t)slot_key<K, LF>(t, b);
+        m.p0 = (double)a;
+        m.slope = m.k1 > m.k0 ? (double)(b - a) / (double)(m.k1 - m.k0) : 0.0;
+        m.elo = 0x7fffffff;
+        m.ehi = -0x7fffffff;
+    }
+    models[j] = m;
+}
+
+template <class K, int LF>
+__global__ void k_model_errors(TreeView t, const uint32_t *__restrict__ routes, LeafModel *__restrict__ models) {
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < t.n; i += step) {
+        LeafModel &m = models[routes[i]];
+        const int64_t e = (int64_t)i - model_pred(m, (uint64_t)slot_key<K, LF>(t, i));
+        const int32_t ec = e > 0x3fffffff ? 0x3fffffff : (e < -0x3fffffff ? -0x3fffffff : (int32_t)e);
+        atomicMin(&m.elo, ec);
+        atomicMax(&m.ehi, ec);
+    }
+}
+
+__global__ void k_finalize_models(LeafModel *__restrict__ models, uint32_t nmodels, uint32_t max_window,
+                                  unsigned long long *__restrict__ acc /* fallback models, fallback keys, sum window*cnt, max window, used */) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nmodels) return;
+    LeafModel &m = models[j];
+    if (m.flags & kModelEmpty) return;
+    const uint64_t w = (uint64_t)((int64_t)m.ehi - (int64_t)m.elo + 1);
+    if (w > max_window) {
+        m.flags |= kModelFallback;
+        atomicAdd(&acc[0], 1ull);
+        atomicAdd(&acc[1], (unsigned long long)m.cnt);
+    }
+    atomicAdd(&acc[2], (unsigned long long)(w * m.cnt));
+    atomicMax(&acc[3], (unsigned long long)w);
+    atomicAdd(&acc[4], 1ull);
+}
+
+// ------------------------------------------------------------------ query
+template <class K, class V, int F, int LF>
+__global__ __launch_bounds__(256) void k_point_learned(TreeView t, LearnedView v, const K *__restrict__ q,
+                                                       V *__restrict__ out, uint64_t nq, V sentinel,
+                                                       uint32_t lds_tree_bytes) {
+    extern __shared__ uint4 lds_raw[];
+    uint8_t *lds = reinterpret_cast<uint8_t *>(lds_raw);
+    float *sh = reinterpret_cast<float *>(lds + lds_tree_bytes);
+    stage_top_levels(t, lds);
+    stage_learned(v, sh);
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += step) {
+        const K key = __builtin_nontemporal_load(&q[i]);
+        const LeafModel m = v.models[route<K>(v, sh, key)];
+        V res = sentinel;
+        if (m.flags & kModelFallback) {
+            res = leaf_lookup<K, V, LF>(t, descend<K, F>(t, lds, key), key, sentinel);
+        } else if (!(m.flags & kModelEmpty) && (uint64_t)key >= m.k0 && (uint64_t)key <= m.k1) {
+            const int64_t pred = model_pred(m, (uint64_t)key);
+            int64_t lo = pred + m.elo, hi = pred + m.ehi;
+            lo = lo < 0 ? 0 : lo;
+            hi = hi > (int64_t)t.n - 1 ? (int64_t)t.n - 1 : hi;
+            if (lo <= hi) {
+                // branch-free lower bound over slots [lo, hi]
+                uint64_t base = (uint64_t)lo, len = (uint64_t)(hi - lo + 1);
+                while (len > 1) {
+                    const uint64_t half = len >> 1;
+                    base = (slot_key<K, LF>(t, base + half) < key) ? base + half : base;
+                    len -= half;
+                }
+                const uint64_t p = base + (slot_key<K, LF>(t, base) < key);
+                if (p <= (uint64_t)hi && slot_key<K, LF>(t, p) == key) res = slot_val<K, V, LF>(t, p);
+            }
+        }
+        __builtin_nontemporal_store(res, &out[i]);
+    }
+}
+
+// ------------------------------------------------------------------ host side
+static void init_params(std::vector<float> &p, uint32_t seed) {
+    std::mt19937_64 g(seed);
+    std::normal_distribution<double> n1(0.0, 1.0);
+    p.assign(kNP, 0.f);
+    float *w1 = p.data(), *b1 = w1 + kH, *W2 = b1 + kH, *b2 = W2 + kH * kH, *w3 = b2 + kH, *b3 = w3 + kH;
+    // first-layer hinges spread over [0, 1]: w1 = 1, b1 = -k/H (+ jitter)
+    for (int k = 0; k < kH; ++k) {
+        w1[k] = 1.f;
+        b1[k] = -(float)k / kH + 0.01f * (float)n1(g);
+    }
+    const double s2 = std::sqrt(2.0 / kH);
+    for (int i = 0; i < kH * kH; ++i) W2[i] = (float)(s2 * n1(g));
+    for (int j = 0; j < kH; ++j) {
+        b2[j] = 0.01f;
+        w3[j] = (float)(std::sqrt(1.0 / kH) * n1(g));
+    }
+    b3[0] = 0.f;
+}
+
+// Piecewise-linear form of the network on [0, 1]: sorted breakpoints and
+// (alpha, beta) per piece, computed in double from the FP32 parameters.
+static void compile_pwl(const std::vector<float> &pf, std::vector<float> &bx, std::vector<float> &al,
+                        std::vector<float> &be) {
+    std::vector<double> p(pf.begin(), pf.end());
+    const double *w1 = p.data(), *b1 = w1 + kH, *W2 = b1 + kH, *b2 = W2 + kH * kH, *w3 = b2 + kH, *b3 = w3 + kH;
+    std::vector<double> bp;
+    for (int k = 0; k < kH; ++k)
+        if (w1[k] != 0.0) {
+            const double x = -b1[k] / w1[k];
+            if (x > 0.0 && x < 1.0) bp.push_back(x);
+        }
+    std::sort(bp.begin(), bp.end());
+    std::vector<double> cuts = {0.0};
+    cuts.insert(cuts.end(), bp.begin(), bp.end());
+    cuts.push_back(1.0);
+    std::vector<double> all = bp;
+    for (size_t s = 0; s + 1 < cuts.size(); ++s) {
+        const double lo = cuts[s], hi = cuts[s + 1], mid = 0.5 * (lo + hi);
+        for (int j = 0; j < kH; ++j) {
+            double c = 0, d = b2[j];  // a2_j(x) = c x + d on this piece
+            for (int k = 0; k < kH; ++k)
+                if (w1[k] * mid + b1[k] > 0) { c += W2[j * kH + k] * w1[k]; d += W2[j * kH + k] * b1[k]; }
+            if (c != 0.0) {
+                const double x = -d / c;
+                if (x > lo && x < hi) all.push_back(x);
+            }
+        }
+    }
+    std::sort(all.begin(), all.end());
+    all.erase(std::unique(all.begin(), all.end()), all.end());
+    std::vector<double> edges = {0.0};
+    edges.insert(edges.end(), all.begin(), all.end());
+    edges.push_back(1.0);
+    bx.clear();
+    al.clear();
+    be.clear();
+    for (double x : all) bx.push_back((float)x);
+    for (size_t s = 0; s + 1 < edges.size(); ++s) {
+        const double mid = 0.5 * (edges[s] + edges[s + 1]);
+        double a = 0, b = b3[0];
+        for (int j = 0; j < kH; ++j) {
+            double c = 0, d = b2[j];
+            for (int k = 0; k < kH; ++k)
+                if (w1[k] * mid + b1[k] > 0) { c += W2[j * kH + k] * w1[k]; d += W2[j * kH + k] * b1[k]; }
+            if (c * mid + d > 0) { a += w3[j] * c; b += w3[j] * d; }
+        }
+        al.push_back((float)a);
+        be.push_back((float)b);
+    }
+}
+
+// One loss/gradient evaluation on device buffers.
+static bpt_status mlp_grad_dev(const float *d_params, const float *d_x, const float *d_t, uint32_t m,
+                               float *d_partial, uint32_t blocks, float *d_grad, hipStream_t s) {
+    k_mlp_grad<<<blocks, kTile, 0, s>>>(d_params, d_x, d_t, m, 1.0f / (float)m, d_partial);
+    BPT_HIP(hipGetLastError());
+    k_reduce_partials<<<(kNP + 1 + 255) / 256, 256, 0, s>>>(d_partial, blocks, d_grad);
+    BPT_HIP(hipGetLastError());
+    return BPT_OK;
+}
+
+template <class K, int LF>
+__global__ void k_training_samples(TreeView t, uint32_t m, float *__restrict__ xs, float *__restrict__ ts) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const uint64_t pos = (uint64_t)(((double)i + 0.5) * (double)t.n / (double)m);
+    const uint64_t p = pos < t.n ? pos : t.n - 1;
+    xs[i] = key_feature<K>(slot_key<K, LF>(t, p));
+    ts[i] = (float)((double)p / (double)t.n);
+}
+
+template <class K, class V, int F, int LF>
+static bpt_status learned_build_impl(bpt_tree *t, const bpt_learned_config &cfg, LearnedModel *lm, hipStream_t s) {
+    const uint32_t M = cfg.leaf_models, m = std::min<uint64_t>(cfg.train_samples, t->n);
+    const uint32_t blocks = std::min<uint32_t>(256, std::max<uint32_t>(1, (m + kTile - 1) / kTile));
+    float *d_x = nullptr, *d_t = nullptr, *d_partial = nullptr, *d_grad = nullptr, *d_mom = nullptr, *d_vel = nullptr;
+    uint32_t *d_step = nullptr, *d_routes = nullptr, *d_pmin = nullptr, *d_pmax = nullptr, *d_cnt = nullptr;
+    unsigned long long *d_acc = nullptr;
+    bpt_status st = BPT_OK;
+    auto H = [&](hipError_t e, const char *w) { if (st == BPT_OK && e != hipSuccess) st = hip_fail(e, w, __FILE__, __LINE__); };
+    H(hipMalloc(&d_x, m * 4ull), "hipMalloc");
+    H(hipMalloc(&d_t, m * 4ull), "hipMalloc");
+    H(hipMalloc(&d_partial, (uint64_t)blocks * (kNP + 1) * 4), "hipMalloc");
+    H(hipMalloc(&d_grad, (kNP + 1) * 4), "hipMalloc");
+    H(hipMalloc(&d_mom, kNP * 4), "hipMalloc");
+    H(hipMalloc(&d_vel, kNP * 4), "hipMalloc");
+    H(hipMalloc(&d_step, 4), "hipMalloc");
+    H(hipMalloc(&lm->d_params, kNP * 4), "hipMalloc");
+    init_params(lm->params, cfg.seed);
+    const bool train = cfg.root <= 1;
+    auto t0 = std::chrono::steady_clock::now();
+    if (st == BPT_OK && train) {
+        H(hipMemcpyAsync(lm->d_params, lm->params.data(), kNP * 4, hipMemcpyHostToDevice, s), "params");
+        H(hipMemsetAsync(d_mom, 0, kNP * 4, s), "memset");
+        H(hipMemsetAsync(d_vel, 0, kNP * 4, s), "memset");
+        H(hipMemsetAsync(d_step, 0, 4, s), "memset");
+        if (st == BPT_OK) {
+            k_training_samples<K, LF><<<(m + 255) / 256, 256, 0, s>>>(t->view, m, d_x, d_t);
+            H(hipGetLastError(), "samples");
+        }
+        if (st == BPT_OK) st = mlp_grad_dev(lm->d_params, d_x, d_t, m, d_partial, blocks, d_grad, s);
+        float l0 = 0;
+        H(hipMemcpyAsync(&l0, d_grad + kNP, 4, hipMemcpyDeviceToHost, s), "loss");
+        H(hipStreamSynchronize(s), "sync");
+        lm->stats.initial_loss = l0;
+        // one Adam step = gradient + reduction + update, captured once and replayed
+        hipStream_t cs = nullptr;
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        H(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "stream");
+        H(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal), "capture");
+        if (st == BPT_OK) {
+            bpt_status s2 = mlp_grad_dev(lm->d_params, d_x, d_t, m, d_partial, blocks, d_grad, cs);
+            k_adam<<<1, 1024, 0, cs>>>(lm->d_params, d_grad, d_mom, d_vel, d_step, cfg.lr);
+            hipError_t e1 = hipGetLastError();
+            hipError_t e2 = hipStreamEndCapture(cs, &g);
+            if (s2 != BPT_OK) st = s2;
+            H(e1, "adam");
+            H(e2, "end capture");
+        }
+        H(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0), "instantiate");
+        H(hipStreamSynchronize(s), "sync");
+        for (uint32_t k = 0; k < cfg.train_steps && st == BPT_OK; ++k) H(hipGraphLaunch(ge, cs), "graph launch");
+        H(hipStreamSynchronize(cs), "sync");
+        if (ge) (void)hipGraphExecDestroy(ge);
+        if (g) (void)hipGraphDestroy(g);
+        if (cs) (void)hipStreamDestroy(cs);
+        if (st == BPT_OK) st = mlp_grad_dev(lm->d_params, d_x, d_t, m, d_partial, blocks, d_grad, s);
+        float l1 = 0;
+        H(hipMemcpyAsync(&l1, d_grad + kNP, 4, hipMemcpyDeviceToHost, s), "loss");
+        H(hipMemcpyAsync(lm->params.data(), lm->d_params, kNP * 4, hipMemcpyDeviceToHost, s), "params");
+        H(hipStreamSynchronize(s), "sync");
+        lm->stats.final_loss = l1;
+    } else {
+        H(hipMemcpy(lm->d_params, lm->params.data(), kNP * 4, hipMemcpyHostToDevice), "params");
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    lm->stats.train_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+
+    // root evaluation tables
+    std::vector<float> bx, al, be;
+    if (cfg.root == 0) compile_pwl(lm->params, bx, al, be);
+    lm->stats.breakpoints = (uint32_t)bx.size();
+    const uint32_t nb = (uint32_t)bx.size();
+    H(hipMalloc(&lm->d_pwl, (3 * (size_t)nb + 2) * 4 + 16), "hipMalloc");
+    if (st == BPT_OK && cfg.root == 0) {
+        std::vector<float> img(bx);
+        img.insert(img.end(), al.begin(), al.end());
+        img.insert(img.end(), be.begin(), be.end());
+        H(hipMemcpy(lm->d_pwl, img.data(), img.size() * 4, hipMemcpyHostToDevice), "pwl");
+    }
+    LearnedView &v = lm->view;
+    v.params = lm->d_params;
+    v.bx = lm->d_pwl;
+    v.alpha = lm->d_pwl + nb;
+    v.beta = lm->d_pwl + 2 * nb + 1;
+    v.nmodels = M;
+    v.nbreak = nb;
+    v.root = cfg.root;
+
+    // leaf models
+    H(hipMalloc(&lm->d_models, (size_t)M * sizeof(LeafModel)), "hipMalloc");
+    H(hipMalloc(&d_routes, t->n * 4ull), "hipMalloc");
+    H(hipMalloc(&d_pmin, M * 4ull), "hipMalloc");
+    H(hipMalloc(&d_pmax, M * 4ull), "hipMalloc");
+    H(hipMalloc(&d_cnt, M * 4ull), "hipMalloc");
+    H(hipMalloc(&d_acc, 5 * 8), "hipMalloc");
+    H(hipMemsetAsync(d_pmin, 0xFF, M * 4ull, s), "memset");
+    H(hipMemsetAsync(d_pmax, 0, M * 4ull, s), "memset");
+    H(hipMemsetAsync(d_cnt, 0, M * 4ull, s), "memset");
+    H(hipMemsetAsync(d_acc, 0, 5 * 8, s), "memset");
+    v.models = lm->d_models;
+    if (st == BPT_OK) {
+        const uint32_t g = (uint32_t)std::min<uint64_t>((t->n + 255) / 256, (uint64_t)device_cu_count() * 16);
+        k_route_keys<K, LF><<<g, 256, kLearnedLds, s>>>(t->view, v, d_routes, d_pmin, d_pmax, d_cnt);
+        k_fit_models<K, LF><<<(M + 255) / 256, 256, 0, s>>>(t->view, M, d_pmin, d_pmax, d_cnt, lm->d_models);
+        k_model_errors<K, LF><<<g, 256, 0, s>>>(t->view, d_routes, lm->d_models);
+        k_finalize_models<<<(M + 255) / 256, 256, 0, s>>>(lm->d_models, M, cfg.max_window, d_acc);
+        H(hipGetLastError(), "leaf models");
+    }
+    unsigned long long acc[5] = {0, 0, 0, 0, 0};
+    H(hipMemcpyAsync(acc, d_acc, sizeof acc, hipMemcpyDeviceToHost, s), "stats");
+    H(hipStreamSynchronize(s), "sync");
+    auto t2 = std::chrono::steady_clock::now();
+    lm->stats.fit_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
+    lm->stats.fallback_models = (uint32_t)acc[0];
+    lm->stats.fallback_keys = acc[1];
+    lm->stats.mean_window = t->n ? (double)acc[2] / (double)t->n : 0.0;
+    lm->stats.max_window_seen = (uint32_t)acc[3];
+    lm->stats.models_used = (uint32_t)acc[4];
+    lm->max_window = cfg.max_window;
+    for (void *p : {(void *)d_x, (void *)d_t, (void *)d_partial, (void *)d_grad, (void *)d_mom, (void *)d_vel,
+                    (void *)d_step, (void *)d_routes, (void *)d_pmin, (void *)d_pmax, (void *)d_cnt, (void *)d_acc})
+        (void)hipFree(p);
+    return st;
+}
 
 uint64_t point_learned_ws(const bpt_tree *, uint64_t) { return 0; }
 
-bpt_status point_learned(const bpt_tree *t, const void *, uint64_t, void *, void *, uint64_t, hipStream_t) {
-    BPT_CHECK(t && t->learned, BPT_ERR_UNSUPPORTED, "no learned model attached to this tree");
-    return BPT_ERR_UNSUPPORTED;
+bpt_status point_learned(const bpt_tree *t, const void *d_q, uint64_t nq, void *d_out, void *, uint64_t,
+                         hipStream_t s) {
+    BPT_CHECK(t->learned, BPT_ERR_UNSUPPORTED, "no learned model attached to this tree (bpt_learned_build)");
+    if (nq == 0) return BPT_OK;
+    if (t->n == 0) return launch_plain(t, d_q, nq, d_out, s);
+    return dispatch_tree(t, [&](auto kt, auto vt, auto fc, auto lc) -> bpt_status {
+        using K = typename decltype(kt)::type;
+        using V = typename decltype(vt)::type;
+        constexpr int F = decltype(fc)::value, LF = decltype(lc)::value;
+        const uint32_t tree_lds = (t->view.lds_bytes + 15) / 16 * 16;
+        k_point_learned<K, V, F, LF><<<search_grid(t, nq), t->block, tree_lds + kLearnedLds, s>>>(
+            t->view, t->learned->view, (const K *)d_q, (V *)d_out, nq, ~V(0), tree_lds);
+        BPT_HIP(hipGetLastError());
+        return BPT_OK;
+    });
 }
 
 }  // namespace bpt
+
+using namespace bpt;
+
+extern "C" {
+
+bpt_status bpt_learned_build(bpt_tree *t, const bpt_learned_config *cfg_in, bpt_learned_stats *stats,
+                             bpt_stream stream) {
+    BPT_CHECK(t, BPT_ERR_INVALID, "tree is NULL");
+    bpt_learned_config cfg{65536, 256, 1u << 20, 2000, 3e-3f, 1, 0};
+    if (cfg_in) cfg = *cfg_in;
+    if (!cfg.leaf_models) cfg.leaf_models = 65536;
+    if (!cfg.max_window) cfg.max_window = 256;
+    if (!cfg.train_samples) cfg.train_samples = 1u << 20;
+    if (cfg.lr <= 0.f) cfg.lr = 3e-3f;
+    BPT_CHECK(cfg.root <= 3, BPT_ERR_INVALID, "root mode %u", cfg.root);
+    BPT_CHECK(t->n >= 1 && t->n < (1ull << 31), BPT_ERR_CAPACITY, "learned index needs 1 <= n < 2^31 keys");
+    BPT_CHECK(cfg.leaf_models <= (1u << 24), BPT_ERR_CAPACITY, "too many leaf models");
+    if (t->learned) {
+        learned_free(t->learned);
+        t->learned = nullptr;
+    }
+    auto *lm = new LearnedModel;
+    hipStream_t s = as_stream(stream);
+    bpt_status st = dispatch_tree(t, [&](auto kt, auto vt, auto fc, auto lc) -> bpt_status {
+        using K = typename decltype(kt)::type;
+        using V = typename decltype(vt)::type;
+        return learned_build_impl<K, V, decltype(fc)::value, decltype(lc)::value>(t, cfg, lm, s);
+    });
+    if (st != BPT_OK) {
+        learned_free(lm);
+        return st;
+    }
+    t->learned = lm;
+    if (stats) *stats = lm->stats;
+    return BPT_OK;
+}
+
+bpt_status bpt_learned_params(const bpt_tree *t, float *h_params, uint32_t count) {
+    BPT_CHECK(t && t->learned && h_params, BPT_ERR_INVALID, "no learned model");
+    BPT_CHECK(count >= (uint32_t)kNP, BPT_ERR_INVALID, "need %d floats", kNP);
+    memcpy(h_params, t->learned->params.data(), kNP * sizeof(float));
+    return BPT_OK;
+}
+
+bpt_status bpt_mlp_loss_grad(const float *h_params, uint32_t hidden, const float *h_x, const float *h_t, uint64_t m,
+                             float *h_grad, float *h_loss) {
+    BPT_CHECK(hidden == (uint32_t)kH, BPT_ERR_UNSUPPORTED, "hidden width %u (built for %d)", hidden, kH);
+    BPT_CHECK(m >= 1 && m < (1ull << 31) && h_params && h_x && h_t && h_grad, BPT_ERR_INVALID, "bad argument");
+    const uint32_t blocks = std::min<uint32_t>(256, (uint32_t)((m + kTile - 1) / kTile));
+    float *d_p = nullptr, *d_x = nullptr, *d_t = nullptr, *d_part = nullptr, *d_g = nullptr;
+    bpt_status st = BPT_OK;
+    auto H = [&](hipError_t e, const char *w) { if (st == BPT_OK && e != hipSuccess) st = hip_fail(e, w, __FILE__, __LINE__); };
+    H(hipMalloc(&d_p, kNP * 4), "hipMalloc");
+    H(hipMalloc(&d_x, m * 4), "hipMalloc");
+    H(hipMalloc(&d_t, m * 4), "hipMalloc");
+    H(hipMalloc(&d_part, (uint64_t)blocks * (kNP + 1) * 4), "hipMalloc");
+    H(hipMalloc(&d_g, (kNP + 1) * 4), "hipMalloc");
+    H(hipMemcpy(d_p, h_params, kNP * 4, hipMemcpyHostToDevice), "copy");
+    H(hipMemcpy(d_x, h_x, m * 4, hipMemcpyHostToDevice), "copy");
+    H(hipMemcpy(d_t, h_t, m * 4, hipMemcpyHostToDevice), "copy");
+    if (st == BPT_OK) st = mlp_grad_dev(d_p, d_x, d_t, (uint32_t)m, d_part, blocks, d_g, nullptr);
+    std::vector<float> g(kNP + 1);
+    H(hipMemcpy(g.data(), d_g, (kNP + 1) * 4, hipMemcpyDeviceToHost), "copy");
+    if (st == BPT_OK) {
+        memcpy(h_grad, g.data(), kNP * 4);
+        if (h_loss) *h_loss = g[kNP];
+    }
+    for (void *p : {(void *)d_p, (void *)d_x, (void *)d_t, (void *)d_part, (void *)d_g}) (void)hipFree(p);
+    return st;
+}
+
+}  // extern "C"

@@ -189,3 +189,32 @@ def forest_offsets(sizes, per_key=1, ranges_div=16):
     rc = np.maximum(np.asarray(sizes, dtype=np.uint64) // np.uint64(ranges_div), np.uint64(1))
     r_off = np.concatenate([[0], np.cumsum(rc)]).astype(np.uint64)
     return key_off, r_off
+
+
+# ---------------------------------------------------------------- learned-index workload
+def lognormal_keys(n, mu=0.0, sigma=2.0, bits=50, seed=0, workload=5, sort=None):
+    """n unique uint64 keys floor(s / s_max * 2^bits), s ~ LogNormal(mu, sigma),
+    sorted; duplicates are dropped and replaced by fresh draws (top-up).
+    `sort` may be a callable sorting a uint64 array (e.g. on the device)."""
+    rng = rng_for(seed, workload, 0)
+    s = np.exp(rng.normal(mu, sigma, size=n))
+    smax = s.max()
+    scale = float(1 << bits) / smax
+    keys = np.floor(s * scale).astype(np.uint64)
+    del s
+    while True:
+        keys = sort(keys) if sort is not None else np.sort(keys)
+        keep = np.ones(len(keys), dtype=bool)
+        keep[1:] = keys[1:] != keys[:-1]
+        keys = keys[keep]
+        if len(keys) >= n:
+            break
+        extra = np.exp(rng.normal(mu, sigma, size=n - len(keys) + 16))
+        extra = np.floor(np.minimum(extra, smax) * scale).astype(np.uint64)
+        keys = np.concatenate([keys, extra])
+    if len(keys) > n:
+        drop = rng.choice(len(keys), size=len(keys) - n, replace=False)
+        keep = np.ones(len(keys), dtype=bool)
+        keep[drop] = False
+        keys = keys[keep]
+    return keys

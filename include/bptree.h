@@ -224,6 +224,47 @@ bpt_status bpt_rect_ids(bpt_tree *t, const uint32_t *d_rects, uint64_t nr, int g
                         const uint64_t *d_offsets, uint32_t *d_ids, bpt_stream s);
 
 /* ------------------------------------------------------------------ */
+/* Learned position model (no reference counterpart)                    */
+/* ------------------------------------------------------------------ */
+/* Root: FP32 MLP 1-32-32-1 (ReLU) over x = log2(key + 1) / key_bits,
+ * trained on the device by the library's forward/backward kernels (Adam)
+ * to predict the rank fraction; it selects one of `leaf_models` linear
+ * models with stored error bounds; the window lower bound answers exactly,
+ * and models whose window exceeds `max_window` use the plain descent.
+ * root: 0 = the MLP evaluated through its piecewise-linear form,
+ *       1 = the MLP evaluated layer by layer,
+ *       2 = no network, routing linear in the log-key feature,
+ *       3 = no network, routing linear in the key. */
+typedef struct bpt_learned_config {
+    uint32_t leaf_models;   /* 0: 65536 */
+    uint32_t max_window;    /* 0: 256 */
+    uint32_t train_samples; /* keys sampled at uniform ranks (0: 2^20) */
+    uint32_t train_steps;   /* Adam steps */
+    float lr;               /* 0: 3e-3 */
+    uint32_t seed;
+    uint32_t root;
+} bpt_learned_config;
+typedef struct bpt_learned_stats {
+    float initial_loss, final_loss; /* mean squared rank-fraction error */
+    double train_ms, fit_ms;
+    uint32_t breakpoints;           /* pieces - 1 of the compiled root */
+    uint32_t models_used, fallback_models, max_window_seen;
+    uint64_t fallback_keys;
+    double mean_window;             /* window slots, averaged over keys */
+} bpt_learned_stats;
+/* Trains the root, fits the leaf models and attaches them to t (replacing
+ * a previous model); then bpt_point_search(..., BPT_POINT_LEARNED, ...)
+ * uses them. Synchronizes s. */
+bpt_status bpt_learned_build(bpt_tree *t, const bpt_learned_config *cfg, bpt_learned_stats *stats,
+                             bpt_stream s);
+/* The trained root parameters: w1[32] b1[32] W2[32*32] b2[32] w3[32] b3. */
+bpt_status bpt_learned_params(const bpt_tree *t, float *h_params, uint32_t count);
+/* Mean squared error and its gradient for m samples on the device with the
+ * training kernels (host arrays in/out; hidden must be 32). */
+bpt_status bpt_mlp_loss_grad(const float *h_params, uint32_t hidden, const float *h_x, const float *h_t,
+                             uint64_t m, float *h_grad, float *h_loss);
+
+/* ------------------------------------------------------------------ */
 /* Multi-tree batches (no reference counterpart)                        */
 /* ------------------------------------------------------------------ */
 /* A forest: ntrees independent trees of 8-byte keys and values in one

@@ -1,0 +1,78 @@
+"""GPU: learned position model. Gradients of the training kernels against
+the oracle's FP64 backward; exact answers (== plain descent == oracle) for
+every root mode and with forced fallback; what the trained network gains
+over routing without it."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+SENT = 0xFFFFFFFFFFFFFFFF
+
+
+def test_mlp_gradient_matches_oracle(bpt, oracle):
+    rng = np.random.default_rng(3)
+    H = 32
+    npar = 3 * H + H * H + H + 1
+    for trial in range(3):
+        p = rng.normal(0, 0.4, size=npar).astype(np.float32)
+        x = rng.uniform(0, 1, size=5000).astype(np.float32)
+        t = np.sort(rng.uniform(0, 1, size=5000)).astype(np.float32)
+        loss, g = bpt.mlp_loss_grad(p, x, t)
+        rl, rg = oracle.mlp_loss_grad(p.astype(np.float64), H, x.astype(np.float64), t.astype(np.float64))
+        # FP32 sums over 5000 samples against an FP64 reference
+        assert abs(loss - rl) <= 1e-4 * abs(rl) + 1e-7
+        assert np.max(np.abs(g - rg)) <= 2e-4 * np.max(np.abs(rg)) + 1e-6, trial
+
+
+@pytest.fixture(scope="module")
+def lognormal_tree(bpt, wl):
+    keys = wl.lognormal_keys(1 << 21, seed=4)
+    vals = np.arange(len(keys), dtype=np.uint64)
+    rng = np.random.default_rng(5)
+    q = np.concatenate([keys[rng.integers(0, len(keys), 300_000)],
+                        rng.integers(0, 1 << 50, 60_000, dtype=np.uint64),
+                        np.array([0, 1, keys[0], keys[-1], keys[-1] + 1, SENT], dtype=np.uint64)])
+    t = bpt.Tree(keys, vals, 8, 8)
+    return t, keys, vals, q
+
+
+@pytest.mark.parametrize("root", [0, 1, 2, 3])
+def test_learned_exact_every_root(bpt, oracle, lognormal_tree, root):
+    t, keys, vals, q = lognormal_tree
+    st = t.learned_build(leaf_models=16384, train_steps=300, root=root)
+    want = oracle.point_closed(keys, vals, q, SENT)
+    assert np.array_equal(t.point_search(q, bpt.LEARNED), want), st
+    assert np.array_equal(t.point_search(q, bpt.PLAIN), want)
+
+
+def test_learned_forced_fallback_still_exact(bpt, oracle, lognormal_tree):
+    t, keys, vals, q = lognormal_tree
+    st = t.learned_build(leaf_models=4096, max_window=4, train_steps=200)
+    assert st["fallback_models"] > 0 and st["fallback_keys"] > 0
+    assert np.array_equal(t.point_search(q, bpt.LEARNED), oracle.point_closed(keys, vals, q, SENT))
+
+
+def test_trained_network_gains_over_no_network(bpt, lognormal_tree):
+    t, keys, vals, q = lognormal_tree
+    trained = t.learned_build(leaf_models=16384, train_steps=600, root=0)
+    assert trained["final_loss"] < 0.1 * trained["initial_loss"]
+    linear_key = t.learned_build(leaf_models=16384, root=3)
+    # without the network, lognormal keys pile into few leaf models
+    assert trained["mean_window"] < 0.25 * linear_key["mean_window"]
+    assert trained["fallback_keys"] < linear_key["fallback_keys"]
+    assert trained["models_used"] > 4 * linear_key["models_used"]
+
+
+def test_learned_u32_tree(bpt, oracle, wl):
+    keys, vals = wl.point_keys(1 << 20, workload=9)
+    q = wl.cpu_queries(keys, 1 << 18, workload=9)
+    t = bpt.Tree(keys, vals)
+    t.learned_build(leaf_models=8192, train_steps=200)
+    assert np.array_equal(t.point_search(q, bpt.LEARNED), oracle.point_closed(keys, vals, q, 0xFFFFFFFF).astype(np.uint32))
+
+
+def test_learned_requires_model(bpt):
+    keys = np.arange(100, dtype=np.uint64)
+    t = bpt.Tree(keys, keys, 8, 8)
+    with pytest.raises(bpt.BptError):
+        t.point_search(keys, bpt.LEARNED)
