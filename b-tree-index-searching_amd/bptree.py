@@ -100,6 +100,13 @@ SIGNATURES = [
     ("bpt_mlp_loss_grad", C.c_int, [C.POINTER(C.c_float), C.c_uint32, C.POINTER(C.c_float),
                                     C.POINTER(C.c_float), C.c_uint64, C.POINTER(C.c_float),
                                     C.POINTER(C.c_float)]),
+    ("bpt_server_create", C.c_int, [vp, C.c_uint32, C.c_uint64, C.c_int, C.POINTER(vp)]),
+    ("bpt_server_submit", C.c_int, [vp, vp, u64p]),
+    ("bpt_server_wait", C.c_int, [vp, C.c_uint64, C.POINTER(C.c_double)]),
+    ("bpt_server_get_stats", C.c_int, [vp, vp]),
+    ("bpt_server_reset_stats", C.c_int, [vp]),
+    ("bpt_server_destroy", C.c_int, [vp]),
+    ("bpt_server_replay", C.c_int, [vp, vp, C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     ("bpt_forest_build", C.c_int, [vp, C.c_uint32, vp, vp, u64p, C.POINTER(vp), vp]),
     ("bpt_forest_free", C.c_int, [vp]),
     ("bpt_forest_info", C.c_int, [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), u64p, u64p]),
@@ -571,3 +578,75 @@ def forest_autotune(dk, dv, key_off, dq, q_off, dout, dlo=None, dhi=None, r_off=
                                     C.byref(best), rep, len(rep)))
     import json as _json
     return ({k: getattr(best, k) for k, _ in BatchConfig._fields_}, _json.loads(rep.value.decode()))
+
+
+REQ_POINT, REQ_RANGE = 0, 1
+
+
+class Request(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("pad_", C.c_int32), ("n", C.c_uint64), ("q", vp), ("out", vp),
+                ("lo", vp), ("hi", vp), ("first", vp), ("count", vp)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("completed", C.c_uint64), ("lookups", C.c_uint64), ("max_inflight", C.c_uint32),
+                ("pad_", C.c_uint32), ("peak_bytes", C.c_uint64), ("budget_bytes", C.c_uint64),
+                ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("max_ms", C.c_double), ("mean_ms", C.c_double),
+                ("elapsed_s", C.c_double)]
+
+
+class Server:
+    """Continuous-batching server over a resident tree (bpt_server_*).
+    Host arrays passed to submit() must stay alive until wait()."""
+
+    def __init__(self, tree, slots=4, mem_budget=1 << 30, mode=PLAIN):
+        h = vp()
+        check(lib().bpt_server_create(tree.h, slots, mem_budget, mode, C.byref(h)))
+        self.h = h.value
+        self.tree = tree
+        self._live = {}
+
+    @staticmethod
+    def point_request(q, out):
+        return Request(REQ_POINT, 0, len(q), q.ctypes.data, out.ctypes.data, None, None, None, None)
+
+    @staticmethod
+    def range_request(lo, hi, first, count):
+        return Request(REQ_RANGE, 0, len(lo), None, None, lo.ctypes.data, hi.ctypes.data,
+                       first.ctypes.data if first is not None else None, count.ctypes.data)
+
+    def submit(self, req, keep=()):
+        t = C.c_uint64(0)
+        check(lib().bpt_server_submit(self.h, C.byref(req), C.byref(t)))
+        self._live[t.value] = keep
+        return t.value
+
+    def wait(self, ticket):
+        ms = C.c_double(0)
+        st = lib().bpt_server_wait(self.h, ticket, C.byref(ms))
+        self._live.pop(ticket, None)
+        check(st)
+        return ms.value
+
+    def replay(self, reqs, arrival_s):
+        arr = (Request * len(reqs))(*reqs)
+        a = np.ascontiguousarray(arrival_s, dtype=np.float64)
+        lat = np.zeros(len(reqs), dtype=np.float64)
+        dp = C.POINTER(C.c_double)
+        check(lib().bpt_server_replay(self.h, arr, len(reqs), a.ctypes.data_as(dp), lat.ctypes.data_as(dp)))
+        return lat
+
+    def stats(self):
+        s = ServerStats()
+        check(lib().bpt_server_get_stats(self.h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in ServerStats._fields_ if k != "pad_"}
+
+    def reset_stats(self):
+        check(lib().bpt_server_reset_stats(self.h))
+
+    def close(self):
+        if getattr(self, "h", None) and not _exiting:
+            lib().bpt_server_destroy(self.h)
+        self.h = None
+
+    __del__ = close

@@ -1,0 +1,404 @@
+// Serving loop: a resident tree answers a stream of point / range requests
+// with continuous batching (no reference counterpart; the reference runs one
+// synchronous batch per process).
+//
+//  * `slots` HIP streams (default priority); each in-flight request owns one.
+//  * Device memory for requests comes from one pool of `mem_budget` bytes
+//    (first-fit, 256-byte granules). A request is admitted when a slot is
+//    free AND its bytes fit in the pool; otherwise it waits in FIFO order, so
+//    a small budget makes requests queue, never fail (a request larger than
+//    the whole budget is rejected at submit).
+//  * Admitted work overlaps on the device: copy-in, search kernels and
+//    copy-out of different requests run on different streams while the
+//    dispatcher thread keeps admitting as earlier requests complete.
+//  * Pinned staging buffers (hipHostMalloc) per slot carry the host copies.
+//  * Latency = completion time - submit time; stats give p50/p99.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <random>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "bpt_internal.hpp"
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+
+struct Request {
+    uint64_t ticket = 0;
+    bpt_request req{};
+    uint64_t bytes = 0, dev_off = 0;
+    clk::time_point t_submit, t_done;
+    bool done = false;
+    bpt_status status = BPT_OK;
+};
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t ev = nullptr;
+    uint8_t *pin = nullptr;  // pinned staging
+    uint64_t pin_bytes = 0;
+    Request *cur = nullptr;
+};
+
+// First-fit allocator over [0, size) in 256-byte granules.
+struct Pool {
+    uint64_t size = 0;
+    std::map<uint64_t, uint64_t> free_;  // offset -> length
+    void init(uint64_t s) {
+        size = s;
+        free_.clear();
+        free_[0] = s;
+    }
+    bool alloc(uint64_t n, uint64_t *off) {
+        n = (n + 255) / 256 * 256;
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= n) {
+                *off = it->first;
+                const uint64_t rest = it->second - n, o = it->first;
+                free_.erase(it);
+                if (rest) free_[o + n] = rest;
+                return true;
+            }
+        }
+        return false;
+    }
+    void release(uint64_t off, uint64_t n) {
+        n = (n + 255) / 256 * 256;
+        auto it = free_.emplace(off, n).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+};
+
+}  // namespace
+
+struct bpt_server {
+    bpt_tree *tree = nullptr;
+    int mode = BPT_POINT_PLAIN;
+    uint64_t budget = 0, used = 0, peak = 0;
+    uint8_t *d_pool = nullptr;
+    Pool pool;
+    std::vector<Slot> slots;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Request *> queue;
+    std::unordered_map<uint64_t, Request *> reqs;
+    uint64_t next_ticket = 1;
+    std::thread worker;
+    bool stop = false;
+    uint32_t inflight = 0, max_inflight = 0;
+    std::vector<double> lat_ms;
+    uint64_t lookups = 0, completed = 0;
+    clk::time_point t_first, t_last;
+    bool any = false;
+};
+
+namespace {
+
+uint64_t request_bytes(const bpt_tree *t, int mode, const bpt_request &r) {
+    const uint64_t kb = t->key_bytes, vb = t->val_bytes, a = 256;
+    auto al = [&](uint64_t b) { return (b + a - 1) / a * a; };
+    if (r.kind == BPT_REQ_POINT)
+        return al(r.n * kb) + al(r.n * vb) + al(bpt_point_search_ws_size(t, r.n, mode));
+    return 2 * al(r.n * kb) + 2 * al(r.n * 8);
+}
+
+uint64_t staging_bytes(const bpt_tree *t, const bpt_request &r) {
+    const uint64_t kb = t->key_bytes, vb = t->val_bytes;
+    return r.kind == BPT_REQ_POINT ? r.n * (kb + vb) : r.n * (2 * kb + 16);
+}
+
+bpt_status launch(bpt_server *S, Slot &sl, Request *q) {
+    const bpt_tree *t = S->tree;
+    const bpt_request &r = q->req;
+    const uint64_t kb = t->key_bytes, vb = t->val_bytes;
+    const uint64_t need = staging_bytes(t, r);
+    if (sl.pin_bytes < need) {
+        if (sl.pin) (void)hipHostFree(sl.pin);
+        sl.pin = nullptr;
+        sl.pin_bytes = 0;
+        BPT_HIP(hipHostMalloc(&sl.pin, std::max<uint64_t>(need, 1 << 20), hipHostMallocDefault));
+        sl.pin_bytes = std::max<uint64_t>(need, 1 << 20);
+    }
+    uint8_t *dev = S->d_pool + q->dev_off;
+    auto al = [](uint64_t b) { return (b + 255) / 256 * 256; };
+    hipStream_t s = sl.stream;
+    if (r.kind == BPT_REQ_POINT) {
+        memcpy(sl.pin, r.q, r.n * kb);
+        uint8_t *dq = dev, *dout = dev + al(r.n * kb), *ws = dout + al(r.n * vb);
+        BPT_HIP(hipMemcpyAsync(dq, sl.pin, r.n * kb, hipMemcpyHostToDevice, s));
+        const uint64_t wsb = bpt_point_search_ws_size(t, r.n, S->mode);
+        BPT_TRY(bpt_point_search_ws(t, dq, r.n, dout, S->mode, wsb ? ws : nullptr, wsb, s));
+        BPT_HIP(hipMemcpyAsync(sl.pin + r.n * kb, dout, r.n * vb, hipMemcpyDeviceToHost, s));
+    } else {
+        memcpy(sl.pin, r.lo, r.n * kb);
+        memcpy(sl.pin + r.n * kb, r.hi, r.n * kb);
+        uint8_t *dlo = dev, *dhi = dev + al(r.n * kb), *df = dhi + al(r.n * kb), *dc = df + al(r.n * 8);
+        BPT_HIP(hipMemcpyAsync(dlo, sl.pin, r.n * kb, hipMemcpyHostToDevice, s));
+        BPT_HIP(hipMemcpyAsync(dhi, sl.pin + r.n * kb, r.n * kb, hipMemcpyHostToDevice, s));
+        BPT_TRY(bpt_range_count(t, dlo, dhi, r.n, reinterpret_cast<uint64_t *>(df), reinterpret_cast<uint64_t *>(dc),
+                                reinterpret_cast<bpt_stream>(s)));
+        BPT_HIP(hipMemcpyAsync(sl.pin + 2 * r.n * kb, df, r.n * 8, hipMemcpyDeviceToHost, s));
+        BPT_HIP(hipMemcpyAsync(sl.pin + 2 * r.n * kb + r.n * 8, dc, r.n * 8, hipMemcpyDeviceToHost, s));
+    }
+    BPT_HIP(hipEventRecord(sl.ev, s));
+    return BPT_OK;
+}
+
+void finish(bpt_server *S, Slot &sl) {
+    Request *q = sl.cur;
+    const bpt_request &r = q->req;
+    const uint64_t kb = S->tree->key_bytes, vb = S->tree->val_bytes;
+    if (q->status == BPT_OK) {
+        if (r.kind == BPT_REQ_POINT) {
+            memcpy(r.out, sl.pin + r.n * kb, r.n * vb);
+        } else {
+            if (r.first) memcpy(r.first, sl.pin + 2 * r.n * kb, r.n * 8);
+            memcpy(r.count, sl.pin + 2 * r.n * kb + r.n * 8, r.n * 8);
+        }
+    }
+    q->t_done = clk::now();
+    q->done = true;
+    S->pool.release(q->dev_off, q->bytes);
+    S->used -= (q->bytes + 255) / 256 * 256;
+    S->inflight--;
+    S->lat_ms.push_back(std::chrono::duration<double, std::milli>(q->t_done - q->t_submit).count());
+    S->lookups += r.n;
+    S->completed++;
+    if (!S->any || q->t_done > S->t_last) S->t_last = q->t_done;
+    sl.cur = nullptr;
+}
+
+void dispatcher(bpt_server *S) {
+    (void)hipSetDevice(S->tree->device);
+    std::unique_lock<std::mutex> lk(S->mu);
+    for (;;) {
+        bool progress = false;
+        for (Slot &sl : S->slots) {
+            if (sl.cur && hipEventQuery(sl.ev) == hipSuccess) {
+                finish(S, sl);
+                progress = true;
+            }
+        }
+        if (progress) S->cv_done.notify_all();
+        // admission: FIFO while a slot is free and the head fits in memory
+        while (!S->queue.empty()) {
+            Slot *free_slot = nullptr;
+            for (Slot &sl : S->slots)
+                if (!sl.cur) { free_slot = &sl; break; }
+            if (!free_slot) break;
+            Request *q = S->queue.front();
+            uint64_t off = 0;
+            if (!S->pool.alloc(q->bytes, &off)) break;
+            S->queue.pop_front();
+            q->dev_off = off;
+            S->used += (q->bytes + 255) / 256 * 256;
+            S->peak = std::max(S->peak, S->used);
+            free_slot->cur = q;
+            S->inflight++;
+            S->max_inflight = std::max(S->max_inflight, S->inflight);
+            q->status = launch(S, *free_slot, q);
+            if (q->status != BPT_OK) {
+                // record the failure; the event still marks the slot complete
+                (void)hipEventRecord(free_slot->ev, free_slot->stream);
+            }
+            progress = true;
+        }
+        bool busy = false;
+        for (Slot &sl : S->slots) busy |= sl.cur != nullptr;
+        if (S->stop && !busy && S->queue.empty()) break;
+        if (!progress) {
+            if (busy) {
+                lk.unlock();
+                std::this_thread::sleep_for(std::chrono::microseconds(20));
+                lk.lock();
+            } else {
+                S->cv_work.wait_for(lk, std::chrono::milliseconds(5));
+            }
+        }
+    }
+}
+
+}  // namespace
+
+using namespace bpt;
+
+extern "C" {
+
+bpt_status bpt_server_create(bpt_tree *t, uint32_t slots, uint64_t mem_budget, int mode, bpt_server **out) {
+    BPT_CHECK(t && out, BPT_ERR_INVALID, "NULL argument");
+    BPT_CHECK(slots >= 1 && slots <= 64, BPT_ERR_INVALID, "slots %u not in [1, 64]", slots);
+    BPT_CHECK(mem_budget >= 4096, BPT_ERR_INVALID, "memory budget too small");
+    BPT_CHECK(mode == BPT_POINT_PLAIN || mode == BPT_POINT_GROUPED || mode == BPT_POINT_LEARNED, BPT_ERR_INVALID,
+              "mode %d", mode);
+    BPT_CHECK(mode != BPT_POINT_LEARNED || t->learned, BPT_ERR_UNSUPPORTED, "tree has no learned model");
+    auto *S = new bpt_server;
+    S->tree = t;
+    S->mode = mode;
+    S->budget = (mem_budget + 255) / 256 * 256;
+    hipError_t e = hipMalloc(&S->d_pool, S->budget);
+    if (e != hipSuccess) {
+        delete S;
+        return hip_fail(e, "hipMalloc(server pool)", __FILE__, __LINE__);
+    }
+    S->pool.init(S->budget);
+    S->slots.resize(slots);
+    for (Slot &sl : S->slots) {
+        e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming);
+        if (e != hipSuccess) {
+            bpt_server_destroy(S);
+            return hip_fail(e, "server stream", __FILE__, __LINE__);
+        }
+    }
+    S->worker = std::thread(dispatcher, S);
+    *out = S;
+    return BPT_OK;
+}
+
+bpt_status bpt_server_submit(bpt_server *S, const bpt_request *r, uint64_t *ticket) {
+    BPT_CHECK(S && r && ticket, BPT_ERR_INVALID, "NULL argument");
+    BPT_CHECK(r->kind == BPT_REQ_POINT || r->kind == BPT_REQ_RANGE, BPT_ERR_INVALID, "request kind %d", r->kind);
+    BPT_CHECK(r->n >= 1, BPT_ERR_INVALID, "empty request");
+    BPT_CHECK(r->kind == BPT_REQ_POINT ? (r->q && r->out) : (r->lo && r->hi && r->count), BPT_ERR_INVALID,
+              "request buffers are NULL");
+    auto *q = new Request;
+    q->req = *r;
+    q->bytes = request_bytes(S->tree, S->mode, *r);
+    if (q->bytes > S->budget) {
+        delete q;
+        set_error("request needs %llu bytes, more than the whole budget", (unsigned long long)request_bytes(S->tree, S->mode, *r));
+        return BPT_ERR_CAPACITY;
+    }
+    q->t_submit = clk::now();
+    {
+        std::lock_guard<std::mutex> g(S->mu);
+        q->ticket = S->next_ticket++;
+        if (!S->any) {
+            S->t_first = q->t_submit;
+            S->any = true;
+        }
+        S->reqs[q->ticket] = q;
+        S->queue.push_back(q);
+    }
+    S->cv_work.notify_one();
+    *ticket = q->ticket;
+    return BPT_OK;
+}
+
+bpt_status bpt_server_wait(bpt_server *S, uint64_t ticket, double *latency_ms) {
+    BPT_CHECK(S, BPT_ERR_INVALID, "server is NULL");
+    std::unique_lock<std::mutex> lk(S->mu);
+    auto it = S->reqs.find(ticket);
+    BPT_CHECK(it != S->reqs.end(), BPT_ERR_NOT_FOUND, "unknown ticket %llu", (unsigned long long)ticket);
+    Request *q = it->second;
+    S->cv_done.wait(lk, [&] { return q->done; });
+    if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(q->t_done - q->t_submit).count();
+    const bpt_status st = q->status;
+    S->reqs.erase(it);
+    delete q;
+    return st;
+}
+
+bpt_status bpt_server_get_stats(bpt_server *S, bpt_server_stats *st) {
+    BPT_CHECK(S && st, BPT_ERR_INVALID, "NULL argument");
+    std::lock_guard<std::mutex> g(S->mu);
+    memset(st, 0, sizeof *st);
+    st->completed = S->completed;
+    st->lookups = S->lookups;
+    st->max_inflight = S->max_inflight;
+    st->peak_bytes = S->peak;
+    st->budget_bytes = S->budget;
+    if (!S->lat_ms.empty()) {
+        std::vector<double> v = S->lat_ms;
+        std::sort(v.begin(), v.end());
+        auto pct = [&](double p) { return v[std::min<size_t>(v.size() - 1, (size_t)(p * (v.size() - 1) + 0.5))]; };
+        st->p50_ms = pct(0.50);
+        st->p99_ms = pct(0.99);
+        st->max_ms = v.back();
+        double sum = 0;
+        for (double x : v) sum += x;
+        st->mean_ms = sum / v.size();
+    }
+    if (S->any) st->elapsed_s = std::chrono::duration<double>(S->t_last - S->t_first).count();
+    return BPT_OK;
+}
+
+bpt_status bpt_server_reset_stats(bpt_server *S) {
+    BPT_CHECK(S, BPT_ERR_INVALID, "server is NULL");
+    std::lock_guard<std::mutex> g(S->mu);
+    S->lat_ms.clear();
+    S->lookups = S->completed = 0;
+    S->max_inflight = S->inflight;
+    S->peak = S->used;
+    S->any = false;
+    return BPT_OK;
+}
+
+bpt_status bpt_server_destroy(bpt_server *S) {
+    if (!S) return BPT_OK;
+    {
+        std::lock_guard<std::mutex> g(S->mu);
+        S->stop = true;
+    }
+    S->cv_work.notify_all();
+    if (S->worker.joinable()) S->worker.join();
+    for (auto &kv : S->reqs) delete kv.second;
+    for (Slot &sl : S->slots) {
+        if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+        if (sl.ev) (void)hipEventDestroy(sl.ev);
+        if (sl.stream) (void)hipStreamDestroy(sl.stream);
+        if (sl.pin) (void)hipHostFree(sl.pin);
+    }
+    (void)hipFree(S->d_pool);
+    delete S;
+    return BPT_OK;
+}
+
+// Open-loop replay: request i is submitted at arrival_s[i] seconds after the
+// start (busy-waiting for its time, independent of completions), then every
+// request is waited for; latency_ms[i] = completion - submit.
+bpt_status bpt_server_replay(bpt_server *S, const bpt_request *reqs, uint32_t nreq, const double *arrival_s,
+                             double *latency_ms) {
+    BPT_CHECK(S && (nreq == 0 || (reqs && arrival_s)), BPT_ERR_INVALID, "NULL argument");
+    std::vector<uint64_t> tickets(nreq, 0);
+    const clk::time_point t0 = clk::now();
+    bpt_status st = BPT_OK;
+    for (uint32_t i = 0; i < nreq; ++i) {
+        const auto due = t0 + std::chrono::duration_cast<clk::duration>(std::chrono::duration<double>(arrival_s[i]));
+        while (clk::now() < due) std::this_thread::yield();
+        bpt_status s = bpt_server_submit(S, &reqs[i], &tickets[i]);
+        if (s != BPT_OK && st == BPT_OK) st = s;
+    }
+    for (uint32_t i = 0; i < nreq; ++i) {
+        if (!tickets[i]) continue;
+        double ms = 0;
+        bpt_status s = bpt_server_wait(S, tickets[i], &ms);
+        if (latency_ms) latency_ms[i] = ms;
+        if (s != BPT_OK && st == BPT_OK) st = s;
+    }
+    return st;
+}
+
+}  // extern "C"

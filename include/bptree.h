@@ -265,6 +265,44 @@ bpt_status bpt_mlp_loss_grad(const float *h_params, uint32_t hidden, const float
                              uint64_t m, float *h_grad, float *h_loss);
 
 /* ------------------------------------------------------------------ */
+/* Serving loop (no reference counterpart)                              */
+/* ------------------------------------------------------------------ */
+/* Continuous batching over a resident tree: `slots` streams, requests
+ * admitted FIFO when a slot is free and their device bytes fit in a pool of
+ * `mem_budget` bytes (they wait otherwise; one larger than the whole budget
+ * is rejected with BPT_ERR_CAPACITY). Host buffers stay the caller's until
+ * the request completes. Point requests use `mode` (bpt_point_mode). */
+typedef struct bpt_server bpt_server;
+enum { BPT_REQ_POINT = 0, BPT_REQ_RANGE = 1 };
+typedef struct bpt_request {
+    int32_t kind;   /* BPT_REQ_POINT or BPT_REQ_RANGE */
+    int32_t pad_;
+    uint64_t n;     /* queries in the request */
+    const void *q;  /* point: host keys */
+    void *out;      /* point: host values (sentinel on a miss) */
+    const void *lo, *hi;     /* range: host bounds (inclusive) */
+    uint64_t *first, *count; /* range: host lower bound (may be NULL) and count */
+} bpt_request;
+typedef struct bpt_server_stats {
+    uint64_t completed, lookups;
+    uint32_t max_inflight, pad_;
+    uint64_t peak_bytes, budget_bytes;
+    double p50_ms, p99_ms, max_ms, mean_ms; /* submit -> completion */
+    double elapsed_s;                       /* first submit -> last completion */
+} bpt_server_stats;
+bpt_status bpt_server_create(bpt_tree *t, uint32_t slots, uint64_t mem_budget, int mode, bpt_server **out);
+bpt_status bpt_server_submit(bpt_server *srv, const bpt_request *r, uint64_t *ticket);
+/* Blocks until the request is done; returns its status. */
+bpt_status bpt_server_wait(bpt_server *srv, uint64_t ticket, double *latency_ms);
+bpt_status bpt_server_get_stats(bpt_server *srv, bpt_server_stats *st);
+bpt_status bpt_server_reset_stats(bpt_server *srv);
+bpt_status bpt_server_destroy(bpt_server *srv);
+/* Open-loop replay: request i is submitted at arrival_s[i] seconds after
+ * the call starts, regardless of completions; then all are waited for. */
+bpt_status bpt_server_replay(bpt_server *srv, const bpt_request *reqs, uint32_t nreq, const double *arrival_s,
+                             double *latency_ms);
+
+/* ------------------------------------------------------------------ */
 /* Multi-tree batches (no reference counterpart)                        */
 /* ------------------------------------------------------------------ */
 /* A forest: ntrees independent trees of 8-byte keys and values in one
