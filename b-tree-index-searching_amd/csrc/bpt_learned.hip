@@ -216,15 +216,15 @@ __device__ __forceinline__ float mlp_forward(const float *P, float x) {
     return y;
 }
 
-// LDS image: [params kNP][bx nbreak][alpha nbreak+1][beta nbreak+1]
+// LDS image: root 0: [bx nbreak][alpha nbreak+1][beta nbreak+1];
+// root 1: [params kNP]; roots 2 and 3 need none.
 template <class K>
 __device__ __forceinline__ uint32_t route(const LearnedView &v, const float *sh, K key) {
     const float x = key_feature<K>(key);
     float y;
     if (v.root == 0) {
-        const float *bx = sh + kNP;
-        const uint32_t s = v.nbreak ? lower_bound_flat<float>(bx, v.nbreak, x) : 0u;
-        y = fmaf(sh[kNP + v.nbreak + s], x, sh[kNP + 2 * v.nbreak + 1 + s]);
+        const uint32_t s = v.nbreak ? lower_bound_flat<float>(sh, v.nbreak, x) : 0u;
+        y = fmaf(sh[v.nbreak + s], x, sh[2 * v.nbreak + 1 + s]);
     } else if (v.root == 1) {
         y = mlp_forward(sh, x);
     } else if (v.root == 2) {
@@ -241,20 +241,24 @@ __device__ __forceinline__ int64_t model_pred(const LeafModel &m, uint64_t key) 
 }
 
 __device__ void stage_learned(const LearnedView &v, float *sh) {
-    const uint32_t total = kNP + (v.root == 0 ? 3 * v.nbreak + 2 : 0);
-    for (uint32_t i = threadIdx.x; i < kNP; i += blockDim.x) sh[i] = v.params[i];
     if (v.root == 0) {
-        for (uint32_t i = threadIdx.x; i < v.nbreak; i += blockDim.x) sh[kNP + i] = v.bx[i];
+        for (uint32_t i = threadIdx.x; i < v.nbreak; i += blockDim.x) sh[i] = v.bx[i];
         for (uint32_t i = threadIdx.x; i <= v.nbreak; i += blockDim.x) {
-            sh[kNP + v.nbreak + i] = v.alpha[i];
-            sh[kNP + 2 * v.nbreak + 1 + i] = v.beta[i];
+            sh[v.nbreak + i] = v.alpha[i];
+            sh[2 * v.nbreak + 1 + i] = v.beta[i];
         }
+    } else if (v.root == 1) {
+        for (uint32_t i = threadIdx.x; i < kNP; i += blockDim.x) sh[i] = v.params[i];
     }
-    (void)total;
     __syncthreads();
 }
 
-constexpr size_t kLearnedLds = sizeof(float) * (kNP + 3 * kMaxBreaks + 2);
+static size_t learned_lds(const LearnedView &v) {
+    if (v.root == 0) return sizeof(float) * (3 * v.nbreak + 2);
+    if (v.root == 1) return sizeof(float) * kNP;
+    return 16;
+}
+
 
 template <class K, int LF>
 __global__ __launch_bounds__(256) void k_route_keys(TreeView t, LearnedView v, uint32_t *__restrict__ routes,
@@ -358,6 +362,147 @@ __global__ __launch_bounds__(256) void k_point_learned(TreeView t, LearnedView v
             }
         }
         __builtin_nontemporal_store(res, &out[i]);
+    }
+}
+
+// Cooperative form (equal key/value widths). Phase A, one lane per query:
+// route, model and window [lo, hi] in slots -> leaf range [la, lb]; queries of
+// fallback models take the plain descent here (one lane each). Phase B,
+// eight lanes per query, the group's eight queries in flight together: the
+// level-1 separators (leaf maxima, one flat array) of leaves la..la+31 are
+// ranked with four per lane, which picks the leaf (wider windows continue
+// 32 leaves at a time); the leaves are then matched as in the plain
+// cooperative descent. The fallback descent reads every level from global
+// memory, so the kernel's LDS holds only the root's tables.
+template <class K, class V, int F, int LF>
+__global__ __launch_bounds__(256) void k_point_learned_coop(TreeView t, LearnedView v, const K *__restrict__ q,
+                                                            V *__restrict__ out, uint64_t nq, V sentinel,
+                                                            uint32_t lds_tree_bytes, uint32_t *__restrict__ fb_count,
+                                                            uint32_t *__restrict__ fb_list, uint32_t fb_cap) {
+    static_assert(sizeof(K) == sizeof(V), "cooperative leaf split needs equal key/value widths");
+    constexpr int LB = LF * (int)sizeof(K) / 4;
+    constexpr int KPC = LB / (int)sizeof(K);
+    constexpr uint32_t SPL = 4;  // separators per lane, 32 leaves per group step
+    extern __shared__ uint4 lds_raw[];
+    uint8_t *lds = reinterpret_cast<uint8_t *>(lds_raw);
+    float *sh = reinterpret_cast<float *>(lds + lds_tree_bytes);
+    __shared__ uint32_t fb_n;  // this block's fallback queries (its own segment of fb_list)
+    if (threadIdx.x == 0) fb_n = 0;
+    stage_top_levels(t, lds);
+    stage_learned(v, sh);
+    uint32_t *my_list = fb_list + (uint64_t)blockIdx.x * fb_cap;
+    const uint32_t lane = threadIdx.x & 63, g = lane >> 3, sub = lane & 7;
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    const K *seps = reinterpret_cast<const K *>(t.inner + t.off[1]);  // leaf maxima
+    const uint32_t nleaves = (uint32_t)t.count[0];
+    for (uint64_t base = wave * 64; base < nq; base += nwaves * 64) {
+        const uint64_t my = base + lane;
+        const K myq = my < nq ? __builtin_nontemporal_load(&q[my]) : K(0);
+        // phase A: la > lb marks a miss (or a query handed to the fallback pass)
+        const LeafModel m = v.models[route<K>(v, sh, myq)];
+        uint32_t la = 1, lb = 0;
+        const bool fb = my < nq && (m.flags & kModelFallback);
+        const uint64_t fbm = __ballot(fb);
+        if (fbm) {  // one LDS atomic per wave appends this wave's fallback queries
+            uint32_t basei = 0;
+            if (lane == (uint32_t)(__ffsll((long long)fbm) - 1)) basei = atomicAdd(&fb_n, (uint32_t)__popcll(fbm));
+            basei = __shfl(basei, __ffsll((long long)fbm) - 1, 64);
+            if (fb) my_list[basei + __popcll(fbm & ((1ull << lane) - 1))] = (uint32_t)my;
+        }
+        if (fb) {
+            // answered by the fallback pass
+        } else if (!(m.flags & kModelEmpty) && (uint64_t)myq >= m.k0 && (uint64_t)myq <= m.k1) {
+            const int64_t pred = model_pred(m, (uint64_t)myq);
+            int64_t lo = pred + m.elo, hi = pred + m.ehi;
+            lo = lo < 0 ? 0 : lo;
+            hi = hi > (int64_t)t.n - 1 ? (int64_t)t.n - 1 : hi;
+            if (lo <= hi) {
+                la = (uint32_t)(lo / LF);
+                lb = (uint32_t)(hi / LF);
+            }
+        }
+        K qk[8];
+        uint32_t a[8], b[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            qk[k] = __shfl(myq, (int)(g * 8 + k), 64);
+            a[k] = (uint32_t)__shfl((int)la, (int)(g * 8 + k), 64);
+            b[k] = (uint32_t)__shfl((int)lb, (int)(g * 8 + k), 64);
+        }
+        // phase B.1: rank among leaf maxima (all eight queries' loads in flight)
+        K sv[8][SPL];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+#pragma unroll
+            for (uint32_t j = 0; j < SPL; ++j) {
+                const uint32_t l = a[k] + sub * SPL + j;
+                sv[k][j] = (l <= b[k] && l < nleaves) ? seps[l] : key_max<K>();
+            }
+        }
+        uint32_t leaf[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            uint32_t c = 0;
+#pragma unroll
+            for (uint32_t j = 0; j < SPL; ++j) c += (uint32_t)(sv[k][j] < qk[k]) & (uint32_t)(a[k] + sub * SPL + j <= b[k]);
+            uint32_t r = group8_sum(c);
+            uint32_t aa = a[k];
+            while (r == 8 * SPL && aa + 8 * SPL <= b[k]) {  // windows wider than 64 leaves
+                aa += 8 * SPL;
+                c = 0;
+                for (uint32_t j = 0; j < SPL; ++j) {
+                    const uint32_t l = aa + sub * SPL + j;
+                    c += (l <= b[k] && l < nleaves) ? (uint32_t)(seps[l] < qk[k]) : 0u;
+                }
+                r = group8_sum(c);
+            }
+            const uint32_t lf = aa + r;
+            leaf[k] = a[k] > b[k] ? 0u : (lf > b[k] ? b[k] : lf);
+        }
+        // phase B.2: leaf lines
+        Chunk<LB> c[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) load_chunk_nt<LB>(t.leaves + (uint64_t)leaf[k] * t.leaf_stride + sub * LB, c[k]);
+        V res = sentinel;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const uint32_t part = sub < 4 ? chunk_count_lt<K, LB>(c[k], qk[k]) : 0u;
+            const uint32_t r = group8_sum(part);
+            const uint32_t rl = r < LF ? r : LF - 1;
+            const uint32_t ksrc = g * 8 + rl / KPC, comp = rl % KPC;
+            const K key = __shfl(chunk_elem<K, LB>(c[k], comp), (int)ksrc, 64);
+            const V val = __shfl((V)chunk_elem<K, LB>(c[k], comp), (int)(ksrc + 4), 64);
+            const bool hit = a[k] <= b[k] && r < LF && (uint64_t)leaf[k] * LF + r < t.n && key == qk[k];
+            if (sub == (uint32_t)k && hit) res = val;
+        }
+        if (my < nq && !fb) __builtin_nontemporal_store(res, &out[my]);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) fb_count[blockIdx.x] = fb_n;
+}
+
+// Fallback pass: block b runs the plain cooperative descent over the queries
+// block b of the learned kernel queued (same grid, one segment per block).
+template <class K, class V, int F, int LF>
+__global__ __launch_bounds__(256) void k_point_coop_list(TreeView t, const K *__restrict__ q, V *__restrict__ out,
+                                                         const uint32_t *__restrict__ count,
+                                                         const uint32_t *__restrict__ fb_list, uint32_t fb_cap,
+                                                         V sentinel) {
+    extern __shared__ uint4 lds_raw[];
+    uint8_t *lds = reinterpret_cast<uint8_t *>(lds_raw);
+    const uint32_t n = count[blockIdx.x];
+    if (n == 0) return;  // the whole block
+    stage_top_levels(t, lds);
+    const uint32_t *list = fb_list + (uint64_t)blockIdx.x * fb_cap;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (uint64_t base = (uint64_t)w * 64; base < n; base += (uint64_t)nw * 64) {
+        const uint64_t j = base + lane;
+        const uint32_t idx = j < n ? list[j] : 0u;
+        const K key = j < n ? q[idx] : K(0);
+        const V r = coop_lookup64<K, V, F, LF, true, kOpValue>(t, lds, key, sentinel);
+        if (j < n) out[idx] = r;
     }
 }
 
@@ -556,7 +701,7 @@ static bpt_status learned_build_impl(bpt_tree *t, const bpt_learned_config &cfg,
     v.models = lm->d_models;
     if (st == BPT_OK) {
         const uint32_t g = (uint32_t)std::min<uint64_t>((t->n + 255) / 256, (uint64_t)device_cu_count() * 16);
-        k_route_keys<K, LF><<<g, 256, kLearnedLds, s>>>(t->view, v, d_routes, d_pmin, d_pmax, d_cnt);
+        k_route_keys<K, LF><<<g, 256, learned_lds(v), s>>>(t->view, v, d_routes, d_pmin, d_pmax, d_cnt);
         k_fit_models<K, LF><<<(M + 255) / 256, 256, 0, s>>>(t->view, M, d_pmin, d_pmax, d_cnt, lm->d_models);
         k_model_errors<K, LF><<<g, 256, 0, s>>>(t->view, d_routes, lm->d_models);
         k_finalize_models<<<(M + 255) / 256, 256, 0, s>>>(lm->d_models, M, cfg.max_window, d_acc);
@@ -579,9 +724,22 @@ static bpt_status learned_build_impl(bpt_tree *t, const bpt_learned_config &cfg,
     return st;
 }
 
-uint64_t point_learned_ws(const bpt_tree *, uint64_t) { return 0; }
+// Fallback queues of the cooperative kernel: one count and one segment of
+// `cap` indices per block of its persistent grid.
+static void fb_plan(const bpt_tree *t, uint64_t nq, uint32_t *blocks, uint32_t *cap) {
+    *blocks = search_grid(t, nq);
+    const uint64_t per_iter = (uint64_t)(*blocks) * t->block;
+    *cap = (uint32_t)(((nq + per_iter - 1) / per_iter) * t->block);
+}
 
-bpt_status point_learned(const bpt_tree *t, const void *d_q, uint64_t nq, void *d_out, void *, uint64_t,
+uint64_t point_learned_ws(const bpt_tree *t, uint64_t nq) {
+    if (!t || t->key_bytes != t->val_bytes || nq == 0) return 0;
+    uint32_t blocks, cap;
+    fb_plan(t, nq, &blocks, &cap);
+    return ((uint64_t)blocks * 4 + 255) / 256 * 256 + (uint64_t)blocks * cap * 4;
+}
+
+bpt_status point_learned(const bpt_tree *t, const void *d_q, uint64_t nq, void *d_out, void *d_ws, uint64_t ws_bytes,
                          hipStream_t s) {
     BPT_CHECK(t->learned, BPT_ERR_UNSUPPORTED, "no learned model attached to this tree (bpt_learned_build)");
     if (nq == 0) return BPT_OK;
@@ -590,9 +748,33 @@ bpt_status point_learned(const bpt_tree *t, const void *d_q, uint64_t nq, void *
         using K = typename decltype(kt)::type;
         using V = typename decltype(vt)::type;
         constexpr int F = decltype(fc)::value, LF = decltype(lc)::value;
-        const uint32_t tree_lds = (t->view.lds_bytes + 15) / 16 * 16;
-        k_point_learned<K, V, F, LF><<<search_grid(t, nq), t->block, tree_lds + kLearnedLds, s>>>(
-            t->view, t->learned->view, (const K *)d_q, (V *)d_out, nq, ~V(0), tree_lds);
+        TreeView gv = t->view;  // fallback descents read every level from global memory
+        gv.lds_levels = 0;
+        gv.lds_bytes = 0;
+        const size_t lds = learned_lds(t->learned->view);
+        if constexpr (sizeof(K) == sizeof(V)) {
+            if (t->coop && t->view.height >= 1 && t->view.count[0] < (1ull << 32) && nq < (1ull << 32)) {
+                BPT_CHECK(d_ws && ws_bytes >= point_learned_ws(t, nq), BPT_ERR_INVALID,
+                          "learned search needs a workspace of %llu bytes",
+                          (unsigned long long)point_learned_ws(t, nq));
+                uint32_t blocks, cap;
+                fb_plan(t, nq, &blocks, &cap);
+                uint32_t *fb_count = static_cast<uint32_t *>(d_ws);
+                uint32_t *fb_list = reinterpret_cast<uint32_t *>(static_cast<uint8_t *>(d_ws) +
+                                                                 ((uint64_t)blocks * 4 + 255) / 256 * 256);
+                k_point_learned_coop<K, V, F, LF><<<blocks, t->block, lds, s>>>(
+                    gv, t->learned->view, (const K *)d_q, (V *)d_out, nq, ~V(0), 0, fb_count, fb_list, cap);
+                if (t->learned->stats.fallback_models) {
+                    const size_t tl = t->view.lds_bytes ? t->view.lds_bytes : 16;
+                    k_point_coop_list<K, V, F, LF><<<blocks, t->block, tl, s>>>(
+                        t->view, (const K *)d_q, (V *)d_out, fb_count, fb_list, cap, ~V(0));
+                }
+                BPT_HIP(hipGetLastError());
+                return BPT_OK;
+            }
+        }
+        k_point_learned<K, V, F, LF><<<search_grid(t, nq), t->block, lds, s>>>(
+            gv, t->learned->view, (const K *)d_q, (V *)d_out, nq, ~V(0), 0);
         BPT_HIP(hipGetLastError());
         return BPT_OK;
     });

@@ -275,6 +275,117 @@ def bench_c4(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_c5(args):
+    """256M unique uint64 lognormal keys: learned position model vs plain
+    descent (throughput), then an open-loop Poisson stream of mixed batches
+    through the serving loop at 0.5x and 0.8x of its saturation rate."""
+    n = 1 << 28
+
+    def dev_sort(a):
+        return bptree.sort_pairs(a)[0]
+
+    t0 = time.perf_counter()
+    keys = workloads.lognormal_keys(n, sort=dev_sort)
+    vals = np.arange(n, dtype=np.uint64)
+    log(f"c5: {n} lognormal keys in {time.perf_counter() - t0:.1f} s")
+    tree = bptree.Tree(keys, vals, 8, 8)
+    lst = tree.learned_build(leaf_models=65536, max_window=256, train_samples=1 << 20, train_steps=2000, root=0)
+    log(f"learned: {lst}")
+    rng = workloads.rng_for(0, 5, 1)
+    nq = 1 << 27
+    q = np.where(rng.random(nq) < 0.9, keys[rng.integers(0, n, nq)],
+                 rng.integers(0, 1 << 50, nq, dtype=np.uint64)).astype(np.uint64)
+    dq = bptree.DeviceArray.from_host(q)
+    dout = bptree.DeviceArray(nq, np.uint64)
+    forms = {}
+    outs = {}
+    for mode, name in ((bptree.PLAIN, "plain"), (bptree.LEARNED, "learned")):
+        wall, ev = time_form(tree, dq, dout, mode, args.steps, args.warmup)
+        forms[name] = {"lookups_per_s": nq / wall, "kernel_ms": float(np.mean(ev))}
+        outs[name] = dout.to_host()
+        log(f"c5 {name}: {wall * 1e3:.2f} ms, {nq / wall / 1e9:.2f} G lookups/s")
+    forms["learned"]["speedup_vs_plain"] = forms["plain"]["kernel_ms"] / forms["learned"]["kernel_ms"]
+    forms["learned"]["mismatches_vs_plain"] = int((outs["plain"] != outs["learned"]).sum())
+    # layer-by-layer root evaluation, for comparison with the piecewise-linear form
+    tree.learned_build(leaf_models=65536, max_window=256, train_samples=1 << 20, train_steps=2000, root=1)
+    wall, ev = time_form(tree, dq, dout, bptree.LEARNED, max(2, args.steps // 2), 1)
+    forms["learned_layerwise_root"] = {"lookups_per_s": nq / wall, "kernel_ms": float(np.mean(ev)),
+                                       "mismatches_vs_plain": int((dout.to_host() != outs["plain"]).sum())}
+    tree.learned_build(leaf_models=65536, max_window=256, train_samples=1 << 20, train_steps=2000, root=0)
+    dq.free()
+    dout.free()
+
+    # serving: mixed batches, sizes log-uniform in [1, 64K], 90% point / 10% range of 2^4..2^12 keys
+    nreq = 3000
+    rr = workloads.rng_for(0, 5, 2)
+    reqs, keep = [], []
+    for i in range(nreq):
+        m = int(np.floor(np.exp2(rr.uniform(0, 16))))
+        if rr.random() < 0.9:
+            qq = np.where(rr.random(m) < 0.9, keys[rr.integers(0, n, m)],
+                          rr.integers(0, 1 << 50, m, dtype=np.uint64)).astype(np.uint64)
+            out = np.zeros(m, dtype=np.uint64)
+            reqs.append(bptree.Server.point_request(qq, out))
+            keep.append((qq, out))
+        else:
+            st_ = rr.integers(0, n, m)
+            w = np.floor(np.exp2(rr.uniform(4, 12, m))).astype(np.int64)
+            lo, hi = keys[st_], keys[np.minimum(st_ + w, n - 1)]
+            first, cnt = np.zeros(m, dtype=np.uint64), np.zeros(m, dtype=np.uint64)
+            reqs.append(bptree.Server.range_request(lo, hi, first, cnt))
+            keep.append((lo, hi, first, cnt))
+    total_lookups = sum(int(r.n) for r in reqs)
+    serving = {}
+    for mode, name in ((bptree.LEARNED, "learned"), (bptree.PLAIN, "plain")):
+        srv = bptree.Server(tree, slots=8, mem_budget=8 << 30, mode=mode)
+        srv.replay(reqs[:200], np.zeros(200))  # warm-up
+        srv.reset_stats()
+        t0 = time.perf_counter()
+        srv.replay(reqs, np.zeros(nreq))
+        sat = nreq / (time.perf_counter() - t0)
+        res = {"saturation_requests_per_s": sat, "saturation": srv.stats()}
+        for frac in (0.5, 0.8):
+            srv.reset_stats()
+            arr = np.cumsum(rr.exponential(1.0 / (frac * sat), size=nreq))
+            srv.replay(reqs, arr)
+            stt = srv.stats()
+            res[f"load_{frac}"] = {"rate_requests_per_s": frac * sat, "p50_ms": stt["p50_ms"], "p99_ms": stt["p99_ms"],
+                                   "lookups_per_s": stt["lookups"] / stt["elapsed_s"] if stt["elapsed_s"] else None,
+                                   "max_inflight": stt["max_inflight"]}
+        srv.close()
+        serving[name] = res
+        log(f"serving {name}: " + json.dumps(res))
+    extra = {"forms": forms, "learned_model": lst, "serving": serving, "serving_requests": nreq,
+             "serving_lookups": total_lookups}
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        threads = min(16, len(os.sched_getaffinity(0)))
+        pt = oracle.PageTree(keys, vals, 5000)
+        sample = q[: 1 << 24]
+        t0 = time.perf_counter()
+        want = pt.point(sample, 0xFFFFFFFFFFFFFFFF, threads)
+        dt = time.perf_counter() - t0
+        pt.close()
+        extra["cpu_baseline"] = {"value": len(sample) / dt, "unit": "lookups/s", "cores": threads, "kind": "port",
+                                 "sample": f"first {len(sample)} queries, page tree PAGESIZE 5000, {dt:.1f} s",
+                                 "mismatches_vs_gpu": int((want != outs["learned"][: len(sample)]).sum())}
+    lk = forms["learned"]
+    line = common_line("learned-index point lookups/s @256M lognormal uint64 keys; p50/p99 ms serving",
+                       lk["lookups_per_s"], "lookups/s", args, lk["kernel_ms"], "u64",
+                       {"workload": "256M unique uint64 keys, lognormal(0, 2) scaled to [0, 2^50], value = rank; "
+                                    "MLP 1-32-32-1 root over 65536 linear leaves, window cap 256; 128M point "
+                                    "queries (90% hits); serving: 3000 mixed batches (sizes log-uniform 1..64K, "
+                                    "90% point / 10% range of 2^4..2^12 keys), open-loop Poisson",
+                        "keys": n, "queries": nq},
+                       {"bound": "hbm", "achieved": 144 * nq / (lk["kernel_ms"] * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": 144 * nq / (lk["kernel_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                        "traffic": None, "kernel": "k_point_learned", "kernel_ms": lk["kernel_ms"],
+                        "bytes_per_lookup": 144},
+                       extra)
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -286,8 +397,9 @@ def main():
     ap.add_argument("--no-forms", action="store_true", help="skip grouped / Zipf side measurements")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run the 1 GiB device copy kernel (HBM counter calibration)")
-    ap.add_argument("--config", default="c2", choices=["c2", "c3", "c4"],
-                    help="c2 (default): 64M-key point search; c3: 2-D rectangles; c4: 4096-tree batch")
+    ap.add_argument("--config", default="c2", choices=["c2", "c3", "c4", "c5"],
+                    help="c2 (default): 64M-key point search; c3: 2-D rectangles; c4: 4096-tree batch; "
+                         "c5: learned index + serving")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is "
@@ -297,6 +409,8 @@ def main():
         return bench_c3(args)
     if args.config == "c4":
         return bench_c4(args)
+    if args.config == "c5":
+        return bench_c5(args)
 
     n, nq = 1 << args.keys_log2, 1 << args.queries_log2
     dev = bptree.device_info(0)

@@ -42,6 +42,9 @@ def test_learned_exact_every_root(bpt, oracle, lognormal_tree, root):
     st = t.learned_build(leaf_models=16384, train_steps=300, root=root)
     want = oracle.point_closed(keys, vals, q, SENT)
     assert np.array_equal(t.point_search(q, bpt.LEARNED), want), st
+    t.set_option("coop", 0)   # one lane per query, binary search in the window
+    assert np.array_equal(t.point_search(q, bpt.LEARNED), want), st
+    t.set_option("coop", 1)
     assert np.array_equal(t.point_search(q, bpt.PLAIN), want)
 
 
@@ -49,7 +52,15 @@ def test_learned_forced_fallback_still_exact(bpt, oracle, lognormal_tree):
     t, keys, vals, q = lognormal_tree
     st = t.learned_build(leaf_models=4096, max_window=4, train_steps=200)
     assert st["fallback_models"] > 0 and st["fallback_keys"] > 0
-    assert np.array_equal(t.point_search(q, bpt.LEARNED), oracle.point_closed(keys, vals, q, SENT))
+    want = oracle.point_closed(keys, vals, q, SENT)
+    assert np.array_equal(t.point_search(q, bpt.LEARNED), want)
+    t.set_option("coop", 0)
+    assert np.array_equal(t.point_search(q, bpt.LEARNED), want)
+    t.set_option("coop", 1)
+    # windows wider than 32 leaves take several 32-leaf ranking steps
+    st = t.learned_build(leaf_models=256, max_window=1 << 20, train_steps=200)
+    assert st["max_window_seen"] > 8 * 64
+    assert np.array_equal(t.point_search(q, bpt.LEARNED), want)
 
 
 def test_trained_network_gains_over_no_network(bpt, lognormal_tree):
