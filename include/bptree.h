@@ -119,9 +119,9 @@ bpt_status bpt_tree_set_option(bpt_tree *t, const char *key, int64_t value);
 /* Point search                                                        */
 /* ------------------------------------------------------------------ */
 typedef enum bpt_point_mode {
-    BPT_POINT_PLAIN = 0,   /* SearchingBTreeGPUOnlyOneState: thread-per-query descent */
+    BPT_POINT_PLAIN = 0,   /* SearchingBTreeGPUOnlyOneState: root-to-leaf descent */
     BPT_POINT_GROUPED = 1, /* SearchingBTreeGPUCombineStep: on-device radix grouping first */
-    BPT_POINT_LEARNED = 2  /* learned position model + window search (bpt_learned_attach) */
+    BPT_POINT_LEARNED = 2  /* learned position model + window search (bpt_learned_build) */
 } bpt_point_mode;
 
 /* out[i] = value of the first key equal to q[i], else the sentinel of the
@@ -180,7 +180,8 @@ bpt_status bpt_range_count_grouped(const bpt_tree *t, const void *d_lo, const vo
  * (key >> shift) & ((1 << bits) - 1), bits <= 16. d_perm[j] = original index
  * of the j-th element in grouped order; d_bucket_start (1 << bits) + 1
  * entries, may be NULL. Replaces the MSD passes of genGrouping
- * (grouping.cu:48-478,1677-1851) with one on-device pass; synchronizes s. */
+ * (grouping.cu:48-478,1677-1851) with an on-device stable radix partition;
+ * synchronizes s. */
 bpt_status bpt_radix_group(const void *d_keys, uint32_t key_bytes, uint64_t n, uint32_t shift,
                            uint32_t bits, uint32_t *d_perm, uint64_t *d_bucket_start,
                            bpt_stream s);
