@@ -293,6 +293,15 @@ bpt_status bpt_tree_set_option(bpt_tree *t, const char *key, int64_t value) {
         t->leaf_nt = value ? 1 : 0;
         return BPT_OK;
     }
+    if (strcmp(key, "addr32") == 0) {
+        t->addr32 = value ? 1 : 0;
+        return BPT_OK;
+    }
+    if (strcmp(key, "lanes") == 0) {
+        BPT_CHECK(value == 4 || value == 8, BPT_ERR_INVALID, "lanes per query must be 4 or 8");
+        t->coop_lanes = (uint32_t)value;
+        return BPT_OK;
+    }
     set_error("unknown tree option '%s'", key);
     return BPT_ERR_NOT_FOUND;
 }

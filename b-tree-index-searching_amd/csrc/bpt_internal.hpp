@@ -79,8 +79,10 @@ struct bpt_tree {
     uint32_t block = 256, blocks_per_cu = 32;
     uint32_t group_bits = 12;
     uint32_t lds_budget = 32768;
-    uint32_t coop = 1;               // cooperative 8-lane descent when widths match
+    uint32_t coop = 1;               // cooperative descent when key and value widths match
     uint32_t leaf_nt = 1;            // non-temporal leaf loads
+    uint32_t coop_lanes = 8;         // lanes per query of the cooperative descent (4 or 8)
+    uint32_t addr32 = 1;             // 32-bit node / leaf offsets when the tree allows them
     bpt::TreeView view{};
     bpt::Workspace ws;             // grouped / learned scratch owned by the tree
     bpt::LearnedModel *learned = nullptr;
@@ -294,16 +296,45 @@ __device__ __forceinline__ void load_chunk_nt(const uint8_t *p, Chunk<BYTES> &c)
     }
 }
 
+// Same loads through a buffer resource: `base` is uniform and `off` a 32-bit
+// per-lane byte offset (no 64-bit address arithmetic per lane).
+template <int BYTES, bool NT>
+__device__ __forceinline__ void load_chunk_buf(__amdgpu_buffer_rsrc_t r, uint32_t off, Chunk<BYTES> &c) {
+    constexpr int aux = NT ? 2 : 0;  // cache policy: non-temporal
+    if constexpr (BYTES == 8) {
+        const u32x2_t v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, aux);
+        c.w[0] = v.x; c.w[1] = v.y;
+    } else {
+#pragma unroll
+        for (int i = 0; i < BYTES / 16; ++i) {
+            const u32x4_t v = __builtin_amdgcn_raw_buffer_load_b128(r, off + 16u * i, 0, aux);
+            c.w[4 * i] = v.x; c.w[4 * i + 1] = v.y; c.w[4 * i + 2] = v.z; c.w[4 * i + 3] = v.w;
+        }
+    }
+}
+
+// Raw buffer resource over [base, base + 4 GiB): stride 0, 32-bit data format
+// word of the gfx9 descriptor.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t raw_rsrc(const void *base) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)0xFFFFFFFF, 0x00020000);
+}
+
+// Element i mod N of a chunk of N keys, as a chain of compare-selects
+// (kept as selects: an indexed register array would be lowered to scratch).
 template <class K, int BYTES>
 __device__ __forceinline__ K chunk_elem(const Chunk<BYTES> &c, uint32_t i) {
     constexpr int N = BYTES / (int)sizeof(K);
-    K r = 0;
+    static_assert((N & (N - 1)) == 0, "power-of-two chunk");
+    const uint32_t j = i & (uint32_t)(N - 1);
+    K r;
+    if constexpr (sizeof(K) == 4) r = c.w[0];
+    else r = (uint64_t)c.w[0] | ((uint64_t)c.w[1] << 32);
 #pragma unroll
-    for (int j = 0; j < N; ++j) {
+    for (int e = 1; e < N; ++e) {
         K v;
-        if constexpr (sizeof(K) == 4) v = c.w[j];
-        else v = (uint64_t)c.w[2 * j] | ((uint64_t)c.w[2 * j + 1] << 32);
-        r = (i == (uint32_t)j) ? v : r;
+        if constexpr (sizeof(K) == 4) v = c.w[e];
+        else v = (uint64_t)c.w[2 * e] | ((uint64_t)c.w[2 * e + 1] << 32);
+        r = j == (uint32_t)e ? v : r;
     }
     return r;
 }
@@ -325,10 +356,22 @@ __device__ __forceinline__ uint32_t chunk_count_lt(const Chunk<BYTES> &c, K q) {
 // Sum over the 8 lanes of a group with DPP lane moves (no LDS traffic):
 // quad_perm [1,0,3,2], quad_perm [2,3,0,1], then row_half_mirror, which
 // pairs lane i with lane 7-i of the same 8-lane half row (the other quad).
+// Every lane has a valid source under these patterns, so bound_ctrl (which
+// only matters for invalid sources) is set and no old value is needed.
 __device__ __forceinline__ uint32_t group8_sum(uint32_t x) {
-    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xF, 0xF, false);
-    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x4E, 0xF, 0xF, false);
-    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x141, 0xF, 0xF, false);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xF, 0xF, true);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x4E, 0xF, 0xF, true);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x141, 0xF, 0xF, true);
+    return x;
+}
+
+// Sum over the G lanes of an aligned group (G = 4: one quad; G = 8).
+template <int G>
+__device__ __forceinline__ uint32_t group_sum(uint32_t x) {
+    static_assert(G == 4 || G == 8, "group of 4 or 8 lanes");
+    if constexpr (G == 8) return group8_sum(x);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xF, 0xF, true);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x4E, 0xF, 0xF, true);
     return x;
 }
 
@@ -352,26 +395,32 @@ __device__ __forceinline__ uint32_t lower_bound_flat(const K *__restrict__ a, ui
 // LDS level -- the separators of one level, concatenated in node order, are
 // sorted (pads only trail the last node), and the first separator >= q there
 // is exactly the child the root-to-that-level descent reaches.
-// Phase B (eight lanes per query): group g (lanes 8g..8g+7) walks queries
-// 8g..8g+7 through the global levels and the leaf, all eight in flight at
-// each level; lane s of the group reads bytes [s*NB, (s+1)*NB) of a node, so
-// one load instruction fetches eight whole nodes. Leaf: lanes 0-3 hold the
-// keys, lanes 4-7 the values (sizeof(K) == sizeof(V)).
+// Phase B (G lanes per query, G = 8 or 4): group g (lanes Gg..Gg+G-1) walks
+// queries Gg..Gg+G-1 through the global levels and the leaf, all G in flight
+// at each level; lane s of the group reads bytes [s*NB, (s+1)*NB) of a node,
+// so one load instruction fetches 64/G node pieces of whole lines. Leaf: the
+// first G/2 lanes hold the keys, the others the values (sizeof(K) ==
+// sizeof(V)). Every step is branch-free so the G queries interleave.
+// OFF32: node and leaf byte offsets fit 32 bits (levels and leaves < 4 GiB),
+// so addresses are a uniform base plus a 32-bit lane offset.
 // OP kOpValue returns the value of the first key equal to the lane's query
 // (else `sentinel`); kOpLowerBound returns the global lower-bound position.
 constexpr int kOpValue = 0;
 constexpr int kOpLowerBound = 1;
 
-template <class K, class V, int F, int LF, bool NT, int OP>
+template <class K, class V, int F, int LF, bool NT, int OP, int G = 8, bool OFF32 = false>
 __device__ __forceinline__ typename std::conditional<OP == kOpValue, V, uint64_t>::type
 coop_lookup64(const TreeView &t, const uint8_t *lds, K myq, V sentinel) {
     static_assert(sizeof(K) == sizeof(V), "cooperative leaf split needs equal key/value widths");
-    constexpr int NB = F * (int)sizeof(K) / 8;   // node bytes per lane
-    constexpr int LB = LF * (int)sizeof(K) / 4;  // leaf bytes per lane (keys in lanes 0-3)
-    constexpr int KPC = LB / (int)sizeof(K);     // keys per leaf chunk
+    static_assert(G == 4 || G == 8, "4 or 8 lanes per query");
+    constexpr int NB = F * (int)sizeof(K) / G;       // node bytes per lane
+    constexpr int LB = 2 * LF * (int)sizeof(K) / G;  // leaf bytes per lane (keys in lanes 0..G/2-1)
+    constexpr int KPC = LB / (int)sizeof(K);         // keys per leaf chunk
     constexpr uint32_t NODE = F * sizeof(K);
+    constexpr uint32_t STRIDE = 2 * LF * sizeof(K);  // leaf line: LF keys then LF values
+    static_assert(NB >= 8 && LB >= 8, "chunks of at least 8 bytes");
     using R = typename std::conditional<OP == kOpValue, V, uint64_t>::type;
-    const uint32_t lane = threadIdx.x & 63, g = lane >> 3, sub = lane & 7;
+    const uint32_t lane = threadIdx.x & 63, g = lane / G, sub = lane % G;
     const int flat_lvl = t.height - t.lds_levels + 1;
     const bool has_flat = t.lds_levels > 0;
     uint32_t mynode = 0;
@@ -382,47 +431,65 @@ coop_lookup64(const TreeView &t, const uint8_t *lds, K myq, V sentinel) {
         mynode = c < flat_n - 1 ? c : flat_n - 1;
     }
     const int top = has_flat ? flat_lvl - 1 : t.height;  // first level walked cooperatively
-    K qk[8];
-    uint32_t node[8];
+    K qk[G];
+    uint32_t node[G];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        qk[k] = __shfl(myq, (int)(g * 8 + k), 64);
-        node[k] = (uint32_t)__shfl((int)mynode, (int)(g * 8 + k), 64);
+    for (int k = 0; k < G; ++k) {
+        qk[k] = __shfl(myq, (int)(g * G + k), 64);
+        node[k] = (uint32_t)__shfl((int)mynode, (int)(g * G + k), 64);
     }
     for (int lvl = top; lvl >= 1; --lvl) {
-        const uint8_t *lv = t.inner + t.off[lvl] + sub * NB;
+        const uint8_t *lv = t.inner + t.off[lvl];
         const uint32_t last = (uint32_t)t.count[lvl - 1] - 1;
-        Chunk<NB> c[8];
+        Chunk<NB> c[G];
+        if constexpr (OFF32) {
+            const __amdgpu_buffer_rsrc_t rs = raw_rsrc(lv);
 #pragma unroll
-        for (int k = 0; k < 8; ++k) load_chunk<NB>(lv + (uint64_t)node[k] * NODE, c[k]);
+            for (int k = 0; k < G; ++k) load_chunk_buf<NB, false>(rs, node[k] * NODE + sub * NB, c[k]);
+        } else {
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const uint32_t ch = node[k] * F + group8_sum(chunk_count_lt<K, NB>(c[k], qk[k]));
+            for (int k = 0; k < G; ++k) load_chunk<NB>(lv + (uint64_t)node[k] * NODE + sub * NB, c[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            const uint32_t ch = node[k] * F + group_sum<G>(chunk_count_lt<K, NB>(c[k], qk[k]));
             node[k] = ch < last ? ch : last;
         }
     }
-    Chunk<LB> c[8];
+    Chunk<LB> c[G];
+    if constexpr (OFF32) {
+        const __amdgpu_buffer_rsrc_t rs = raw_rsrc(t.leaves);
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const uint8_t *lp = t.leaves + (uint64_t)node[k] * t.leaf_stride + sub * LB;
-        if constexpr (NT) load_chunk_nt<LB>(lp, c[k]);
-        else load_chunk<LB>(lp, c[k]);
+        for (int k = 0; k < G; ++k) load_chunk_buf<LB, NT>(rs, node[k] * STRIDE + sub * LB, c[k]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            const uint8_t *lp = t.leaves + (uint64_t)node[k] * STRIDE + sub * LB;
+            if constexpr (NT) load_chunk_nt<LB>(lp, c[k]);
+            else load_chunk<LB>(lp, c[k]);
+        }
     }
+    const uint32_t keymask = sub < (uint32_t)(G / 2) ? ~0u : 0u;  // value lanes count nothing
+    // slot leaf*LF + r holds a key iff leaf < last_leaf or r < last_cnt
+    const uint32_t last_leaf = (uint32_t)(t.count[0] - 1);
+    const uint32_t last_cnt = (uint32_t)(t.n - (uint64_t)last_leaf * LF);
     R res = OP == kOpValue ? (R)sentinel : (R)0;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const uint32_t part = sub < 4 ? chunk_count_lt<K, LB>(c[k], qk[k]) : 0u;
-        const uint32_t r = group8_sum(part);  // lower bound inside the leaf, 0..LF
+    for (int k = 0; k < G; ++k) {
+        const uint32_t r = group_sum<G>(chunk_count_lt<K, LB>(c[k], qk[k]) & keymask);  // 0..LF
         if constexpr (OP == kOpValue) {
-            const uint32_t rl = r < LF ? r : LF - 1;
-            const uint32_t src = g * 8 + rl / KPC, comp = rl % KPC;
-            const K key = __shfl(chunk_elem<K, LB>(c[k], comp), (int)src, 64);
-            const V val = __shfl((V)chunk_elem<K, LB>(c[k], comp), (int)(src + 4), 64);
-            const bool hit = r < LF && (uint64_t)node[k] * LF + r < t.n && key == qk[k];
-            if (sub == (uint32_t)k) res = hit ? val : sentinel;
+            // slot r sits at element r % KPC of key lane r / KPC and of value lane r / KPC + G/2
+            const K e = chunk_elem<K, LB>(c[k], r);
+            const uint32_t src = g * G + (r / KPC);
+            const K key = __shfl(e, (int)src, 64);
+            const V val = __shfl((V)e, (int)(src + G / 2), 64);
+            const bool hit = r < LF && key == qk[k] && (node[k] < last_leaf || r < last_cnt);
+            const V mine = hit ? val : sentinel;
+            res = sub == (uint32_t)k ? mine : res;
         } else {
             const uint64_t p = (uint64_t)node[k] * LF + r;
-            if (sub == (uint32_t)k) res = p < t.n ? p : t.n;
+            const uint64_t mine = p < t.n ? p : t.n;
+            res = sub == (uint32_t)k ? mine : res;
         }
     }
     return res;

@@ -29,8 +29,8 @@ __global__ __launch_bounds__(256) void k_point_plain(TreeView t, const K *__rest
 }
 
 // Cooperative single-tree kernel: every wave walks 64 consecutive queries
-// at a time with coop_lookup64 (bpt_internal.hpp).
-template <class K, class V, int F, int LF, bool NT>
+// at a time with coop_lookup64 (bpt_internal.hpp), G lanes per query.
+template <class K, class V, int F, int LF, bool NT, int G, bool OFF32>
 __global__ __launch_bounds__(256) void k_point_coop(TreeView t, const K *__restrict__ q,
                                                     V *__restrict__ out, uint64_t nq, V sentinel) {
     extern __shared__ uint4 lds_raw[];
@@ -42,7 +42,7 @@ __global__ __launch_bounds__(256) void k_point_coop(TreeView t, const K *__restr
     for (uint64_t base = wave * 64; base < nq; base += nwaves * 64) {
         const uint64_t my = base + lane;
         const K myq = my < nq ? __builtin_nontemporal_load(&q[my]) : K(0);
-        const V res = coop_lookup64<K, V, F, LF, NT, kOpValue>(t, lds, myq, sentinel);
+        const V res = coop_lookup64<K, V, F, LF, NT, kOpValue, G, OFF32>(t, lds, myq, sentinel);
         if (my < nq) __builtin_nontemporal_store(res, &out[my]);
     }
 }
@@ -235,8 +235,22 @@ bpt_status launch_plain(const bpt_tree *t, const void *d_q, uint64_t nq, void *d
             if (t->coop && t->view.count[0] < (1ull << 32)) {
                 const uint32_t g = search_grid(t, nq);
                 const size_t l = lds_of(t);
-                if (t->leaf_nt) k_point_coop<K, V, F, LF, true><<<g, t->block, l, s>>>(t->view, (const K *)d_q, (V *)d_out, nq, sentinel);
-                else k_point_coop<K, V, F, LF, false><<<g, t->block, l, s>>>(t->view, (const K *)d_q, (V *)d_out, nq, sentinel);
+                const bool off32 = t->addr32 && t->leaf_bytes <= (1ull << 32) && t->inner_bytes <= (1ull << 32);
+                auto go = [&](auto nt, auto lanes, auto o32) {
+                    k_point_coop<K, V, F, LF, decltype(nt)::value, decltype(lanes)::value, decltype(o32)::value>
+                        <<<g, t->block, l, s>>>(t->view, (const K *)d_q, (V *)d_out, nq, sentinel);
+                };
+                auto with_lanes = [&](auto nt) {
+                    if (t->coop_lanes == 4) {
+                        if (off32) go(nt, IC<4>{}, std::true_type{});
+                        else go(nt, IC<4>{}, std::false_type{});
+                    } else {
+                        if (off32) go(nt, IC<8>{}, std::true_type{});
+                        else go(nt, IC<8>{}, std::false_type{});
+                    }
+                };
+                if (t->leaf_nt) with_lanes(std::true_type{});
+                else with_lanes(std::false_type{});
                 BPT_HIP(hipGetLastError());
                 return BPT_OK;
             }

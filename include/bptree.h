@@ -110,9 +110,12 @@ bpt_status bpt_tree_get_layout(const bpt_tree *t, const void **d_leaves, const v
  * blocks per CU of the persistent grid (0 keeps the default). */
 bpt_status bpt_tree_set_launch(bpt_tree *t, uint32_t block, uint32_t blocks_per_cu);
 
-/* Search-kernel options: "coop" (1 = eight lanes share each node, the
+/* Search-kernel options: "coop" (1 = several lanes share each node, the
  * default when key and value widths match; 0 = one lane per query),
- * "leaf_nt" (1 = non-temporal leaf loads, the default). */
+ * "lanes" (lanes per query of the cooperative descent: 8, the default, or 4),
+ * "leaf_nt" (1 = non-temporal leaf loads, the default),
+ * "addr32" (1 = 32-bit byte offsets into levels and leaves below 4 GiB, the
+ * default; 0 = 64-bit addresses throughout). */
 bpt_status bpt_tree_set_option(bpt_tree *t, const char *key, int64_t value);
 
 /* ------------------------------------------------------------------ */

@@ -42,6 +42,16 @@ def test_point_plain_and_grouped(bpt, oracle, cfg, n):
     t = bpt.Tree(keys, vals, kb, vb, fanout=F, leaf_cap=LF)
     want = _expect(oracle, keys, vals, q, vb)
     assert np.array_equal(t.point_search(q, bpt.PLAIN), want)
+    t.set_option("lanes", 4)  # four lanes per query
+    assert np.array_equal(t.point_search(q, bpt.PLAIN), want)
+    t.set_option("addr32", 0)  # 64-bit addressing
+    assert np.array_equal(t.point_search(q, bpt.PLAIN), want)
+    t.set_option("lanes", 8)
+    assert np.array_equal(t.point_search(q, bpt.PLAIN), want)
+    t.set_option("leaf_nt", 0)
+    assert np.array_equal(t.point_search(q, bpt.PLAIN), want)
+    t.set_option("addr32", 1)
+    t.set_option("leaf_nt", 1)
     t.set_option("coop", 0)   # one lane per query
     assert np.array_equal(t.point_search(q, bpt.PLAIN), want)
     t.set_option("coop", 1)
