@@ -1,0 +1,626 @@
+// Grouped point search: queries are partitioned into leaf-range buckets, so
+// that each bucket's slice of an upper level fits in LDS and a lookup reads
+// one line (the leaf) from global memory instead of one per level.
+//
+// The reference groups queries on their key bits (genGrouping,
+// grouping.cu:1677-1851) and then searches each group from the root
+// (kernelSearchingKeyGroupTwoStep, BPTreeExactSearch.cu:206-294). Here a
+// bucket is a range of leaves chosen from the tree itself, so buckets are
+// balanced in leaves whatever the key distribution:
+//   bucket b = leaves [b*lpb, (b+1)*lpb); its separator = its largest key;
+//   slice    = the bucket's entries of level s (s = 1: the leaf maxima),
+//              at most 32 KiB, staged in LDS by the search kernel.
+// Passes over the queries (tiles of T queries):
+//   k_bkt_hist       bucket counts of each tile, one row per tile
+//   column scan      first slot of every (tile, bucket) run in bucket-major
+//                    order: per-group column sums (k_bkt_colsum), a scan
+//                    across groups and buckets (k_bkt_colscan), per-row
+//                    offsets (k_bkt_colprefix); every row stays contiguous
+//   k_bkt_scatter    tile counting-sorted in LDS, written out as runs
+//   k_bkt_search     chunks of the bucket-ordered queries, grid-strided so
+//                    that the blocks in flight share a window of buckets
+//                    (their leaves stay in cache); lower bound in the LDS
+//                    slice, then levels s-1..1 and the leaf from global memory
+//   k_bkt_unpermute  each tile gathers its runs back through LDS into the
+//                    caller's order
+// Every step is deterministic in its results (the order inside a run is not,
+// but each query carries its position in the tile).
+#include <algorithm>
+
+#include "bpt_internal.hpp"
+
+namespace bpt {
+
+constexpr int kBBlock = 256;
+constexpr uint32_t kSliceBytes = 32768;    // LDS slice of the search kernel
+constexpr uint32_t kMaxBuckets = 1024;     // runs per partition tile
+constexpr uint64_t kBucketChunk = 1ull << 28;  // queries per partition (u32 slots)
+
+constexpr uint32_t kColGroup = 256;          // tiles per column-sum group
+
+struct BucketDev {
+    uint32_t nb;       // buckets
+    uint32_t level;    // slice level s (1 = leaf maxima)
+    uint64_t lpb;      // leaves per bucket
+    uint32_t epb;      // slice entries per bucket (entries of level s)
+    uint32_t tile;     // queries per partition tile (T)
+    uint32_t ntiles;
+    uint32_t ngroups;  // column-sum groups of kColGroup tiles
+    uint32_t chunk;    // queries per search work chunk
+    uint32_t xcd;      // search chunks split per XCD
+};
+
+static inline uint64_t align256(uint64_t b) { return (b + 255) / 256 * 256; }
+
+// Partition tile: T queries, kBBlock threads with T / kBBlock each (all of a
+// thread's loads are issued before its first bucket search).
+template <class K> constexpr uint32_t tile_of() { return sizeof(K) == 4 ? 4096u : 2048u; }
+template <class K> constexpr int per_thread() { return (int)(tile_of<K>() / kBBlock); }
+
+// Key-prefix index over a sorted array a[0, n) with a[0] >= base: entry p of
+// its table (p < 2^bits) is the lower bound of base + (p << shift) in a, entry
+// 2^bits is n. The lower bound of a key with prefix p then lies in
+// [tab[p], tab[p + 1]], so a search touches one table entry and a window of
+// about n / 2^bits elements instead of all log2(n) levels (fewer random LDS
+// reads, which is where the bank conflicts are).
+template <class K>
+struct PrefixIndex {
+    K base;
+    uint32_t shift;
+};
+
+template <class K>
+__device__ __forceinline__ int bit_width(K v) {
+    if constexpr (sizeof(K) == 4) return v ? 32 - __clz((int)v) : 0;
+    else return v ? 64 - __clzll((long long)v) : 0;
+}
+
+template <class K>
+__device__ __forceinline__ PrefixIndex<K> prefix_index(K lo, K hi, uint32_t bits) {
+    const int bw = bit_width<K>(hi - lo);
+    return {lo, bw > (int)bits ? (uint32_t)(bw - (int)bits) : 0u};
+}
+
+// Start key of prefix p (saturating at the largest key).
+template <class K>
+__device__ __forceinline__ K prefix_start(const PrefixIndex<K> &pi, uint32_t p) {
+    const K off = (K)p << pi.shift;
+    const K s = pi.base + off;
+    return s < pi.base ? ~K(0) : s;
+}
+
+template <class K>
+__device__ __forceinline__ uint32_t prefix_of(const PrefixIndex<K> &pi, K q, uint32_t bits) {
+    const K dlt = q > pi.base ? q - pi.base : K(0);
+    const K p = dlt >> pi.shift;
+    const K top = ((K)1 << bits) - 1;
+    return (uint32_t)(p < top ? p : top);
+}
+
+// Lower bound of q in a[lo, hi) (hi when every element there is < q).
+template <class K>
+__device__ __forceinline__ uint32_t lower_bound_window(const K *a, uint32_t lo, uint32_t hi, K q) {
+    return hi > lo ? lo + lower_bound_flat<K>(a + lo, hi - lo, q) : lo;
+}
+
+constexpr uint32_t kBucketTabBits = 10;  // prefix table of the bucket separators
+constexpr uint32_t kSliceTabBits = 10;   // prefix table of a search slice
+
+template <class K>
+__device__ __forceinline__ uint32_t bucket_of(const K *bsep, uint32_t nb, const uint16_t *btab,
+                                              const PrefixIndex<K> &pi, K q) {
+    const uint32_t p = prefix_of<K>(pi, q, kBucketTabBits);
+    const uint32_t b = lower_bound_window<K>(bsep, btab[p], btab[p + 1], q);
+    return b < nb ? b : nb - 1;
+}
+
+// Blocks are dispatched to the 8 XCDs round-robin (block i on XCD i % 8).
+// Logical index of block i such that every XCD owns a contiguous range of
+// [0, n) (n = number of blocks): neighbouring tiles then share an L2.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_contiguous(uint32_t i, uint32_t n) {
+    const uint32_t x = i % kXcds, k = i / kXcds, per = n / kXcds, rem = n % kXcds;
+    return x * per + (x < rem ? x : rem) + k;
+}
+
+// In-place exclusive scan of a[0, n) in LDS, all threads of the block;
+// returns the total. wsum: blockDim.x / 64 entries of LDS.
+__device__ uint32_t block_exclusive_scan(uint32_t *a, uint32_t n, uint32_t *wsum) {
+    const uint32_t per = (n + blockDim.x - 1) / blockDim.x;
+    const uint32_t b0 = threadIdx.x * per;
+    uint32_t local = 0;
+    for (uint32_t k = 0; k < per; ++k)
+        if (b0 + k < n) local += a[b0 + k];
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    uint32_t x = local;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= (uint32_t)o) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    uint32_t run = x - local, total = 0;
+    for (uint32_t k = 0; k < nw; ++k) {
+        if (k < w) run += wsum[k];
+        total += wsum[k];
+    }
+    for (uint32_t k = 0; k < per; ++k)
+        if (b0 + k < n) {
+            const uint32_t v = a[b0 + k];
+            a[b0 + k] = run;
+            run += v;
+        }
+    __syncthreads();
+    return total;
+}
+
+// Bucket separators: the largest key of every bucket (a leaf maximum).
+template <class K>
+__global__ void k_bkt_seps(TreeView t, BucketDev d, K *__restrict__ bsep) {
+    const K *leafmax = reinterpret_cast<const K *>(t.inner + t.off[1]);
+    const uint64_t L = t.count[0];
+    for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < d.nb; b += gridDim.x * blockDim.x)
+        bsep[b] = leafmax[std::min<uint64_t>((uint64_t)(b + 1) * d.lpb, L) - 1];
+}
+
+// Prefix table of the bucket separators: hdr = {base, shift}, btab[2^10 + 1].
+template <class K>
+__global__ __launch_bounds__(kBBlock) void k_bkt_btab(TreeView t, BucketDev d, const K *__restrict__ bsep,
+                                                      uint64_t *__restrict__ hdr, uint16_t *__restrict__ btab) {
+    const K kmin = *reinterpret_cast<const K *>(t.leaves);  // smallest key (slot 0)
+    const PrefixIndex<K> pi = prefix_index<K>(kmin, bsep[d.nb - 1], kBucketTabBits);
+    const uint32_t nent = 1u << kBucketTabBits;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p <= nent; p += gridDim.x * blockDim.x)
+        btab[p] = (uint16_t)(p == nent ? d.nb : lower_bound_flat<K>(bsep, d.nb, prefix_start<K>(pi, p)));
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        hdr[0] = (uint64_t)pi.base;
+        hdr[1] = pi.shift;
+    }
+}
+
+template <class K>
+__global__ __launch_bounds__(kBBlock) void k_bkt_hist(const K *__restrict__ q, uint64_t nq, BucketDev d,
+                                                      const K *__restrict__ bsep_g, const uint64_t *__restrict__ hdr,
+                                                      const uint16_t *__restrict__ btab_g,
+                                                      uint32_t *__restrict__ hist) {
+    extern __shared__ uint4 lds_raw[];
+    K *bsep = reinterpret_cast<K *>(lds_raw);
+    uint32_t *cnt = reinterpret_cast<uint32_t *>(bsep + d.nb);
+    uint16_t *btab = reinterpret_cast<uint16_t *>(cnt + d.nb);
+    for (uint32_t b = threadIdx.x; b < d.nb; b += blockDim.x) {
+        bsep[b] = bsep_g[b];
+        cnt[b] = 0;
+    }
+    for (uint32_t p = threadIdx.x; p <= (1u << kBucketTabBits); p += blockDim.x) btab[p] = btab_g[p];
+    const PrefixIndex<K> pi{(K)hdr[0], (uint32_t)hdr[1]};
+    const uint32_t tile = xcd_contiguous(blockIdx.x, gridDim.x);
+    constexpr int E = per_thread<K>();
+    const uint64_t lo = (uint64_t)tile * d.tile;
+    const uint32_t cq = (uint32_t)(std::min<uint64_t>(lo + d.tile, nq) - lo);
+    K key[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const uint32_t i = e * kBBlock + threadIdx.x;
+        key[e] = i < cq ? __builtin_nontemporal_load(&q[lo + i]) : K(0);
+    }
+    __syncthreads();
+    uint32_t bk[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) bk[e] = bucket_of<K>(bsep, d.nb, btab, pi, key[e]);
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+        if (e * kBBlock + threadIdx.x < cq) atomicAdd(&cnt[bk[e]], 1u);
+    __syncthreads();
+    uint32_t *row = hist + (uint64_t)tile * d.nb;
+    for (uint32_t b = threadIdx.x; b < d.nb; b += blockDim.x) row[b] = cnt[b];
+}
+
+// Column sums of groups of kColGroup rows: S[g][b].
+__global__ __launch_bounds__(kBBlock) void k_bkt_colsum(const uint32_t *__restrict__ hist, BucketDev d,
+                                                        uint32_t *__restrict__ S) {
+    const uint32_t g = blockIdx.x;
+    const uint32_t t0 = g * kColGroup, t1 = std::min(t0 + kColGroup, d.ntiles);
+    for (uint32_t b = threadIdx.x; b < d.nb; b += blockDim.x) {
+        uint32_t acc = 0;
+        for (uint32_t t = t0; t < t1; ++t) acc += hist[(uint64_t)t * d.nb + b];
+        S[(uint64_t)g * d.nb + b] = acc;
+    }
+}
+
+// One block of kScanThreads: S[g][b] becomes the first slot of group g's
+// part of bucket b (bucket base + earlier groups); bstart[b] = bucket base,
+// bstart[nb] = total.
+constexpr int kScanThreads = 1024;
+__global__ __launch_bounds__(kScanThreads) void k_bkt_colscan(uint32_t *__restrict__ S, BucketDev d,
+                                                              uint32_t *__restrict__ bstart) {
+    __shared__ uint32_t tot[kMaxBuckets];
+    __shared__ uint32_t wsum[kScanThreads / 64];
+    for (uint32_t b = threadIdx.x; b < d.nb; b += blockDim.x) {
+        uint32_t run = 0;
+        for (uint32_t g = 0; g < d.ngroups; ++g) {
+            uint32_t *e = S + (uint64_t)g * d.nb + b;
+            const uint32_t v = *e;
+            *e = run;
+            run += v;
+        }
+        tot[b] = run;
+    }
+    __syncthreads();
+    const uint32_t total = block_exclusive_scan(tot, d.nb, wsum);
+    for (uint32_t b = threadIdx.x; b < d.nb; b += blockDim.x) {
+        const uint32_t base = tot[b];
+        bstart[b] = base;
+        for (uint32_t g = 0; g < d.ngroups; ++g) S[(uint64_t)g * d.nb + b] += base;
+    }
+    if (threadIdx.x == 0) bstart[d.nb] = total;
+}
+
+// off[t][b] = first slot of tile t's run of bucket b.
+__global__ __launch_bounds__(kBBlock) void k_bkt_colprefix(const uint32_t *__restrict__ hist,
+                                                           const uint32_t *__restrict__ S, BucketDev d,
+                                                           uint32_t *__restrict__ off) {
+    const uint32_t g = blockIdx.x;
+    const uint32_t t0 = g * kColGroup, t1 = std::min(t0 + kColGroup, d.ntiles);
+    for (uint32_t b = threadIdx.x; b < d.nb; b += blockDim.x) {
+        uint32_t run = S[(uint64_t)g * d.nb + b];
+        for (uint32_t t = t0; t < t1; ++t) {
+            const uint64_t e = (uint64_t)t * d.nb + b;
+            off[e] = run;
+            run += hist[e];
+        }
+    }
+}
+
+// LDS (widest first, so every array is aligned):
+//   sk[T] | bsep[nb] | gstart[nb] | lstart[nb] | lcur[nb] | sl[T] | sb[T] | btab[2^10 + 1]
+template <class K>
+__global__ __launch_bounds__(kBBlock) void k_bkt_scatter(const K *__restrict__ q, uint64_t nq, BucketDev d,
+                                                         const K *__restrict__ bsep_g,
+                                                         const uint64_t *__restrict__ hdr,
+                                                         const uint16_t *__restrict__ btab_g,
+                                                         const uint32_t *__restrict__ hist,
+                                                         const uint32_t *__restrict__ off,
+                                                         K *__restrict__ qs, uint16_t *__restrict__ lidx) {
+    extern __shared__ uint4 lds_raw[];
+    __shared__ uint32_t wsum[kBBlock / 64];
+    K *sk = reinterpret_cast<K *>(lds_raw);
+    K *bsep = sk + d.tile;
+    uint32_t *gstart = reinterpret_cast<uint32_t *>(bsep + d.nb);
+    uint32_t *lstart = gstart + d.nb;
+    uint32_t *lcur = lstart + d.nb;
+    uint16_t *sl = reinterpret_cast<uint16_t *>(lcur + d.nb);
+    uint16_t *sb = sl + d.tile;
+    uint16_t *btab = sb + d.tile;
+    const uint32_t tile = xcd_contiguous(blockIdx.x, gridDim.x);
+    for (uint32_t b = threadIdx.x; b < d.nb; b += blockDim.x) {
+        bsep[b] = bsep_g[b];
+        const uint64_t e = (uint64_t)tile * d.nb + b;
+        gstart[b] = off[e];
+        lstart[b] = hist[e];  // this tile's count of bucket b
+    }
+    for (uint32_t p = threadIdx.x; p <= (1u << kBucketTabBits); p += blockDim.x) btab[p] = btab_g[p];
+    const PrefixIndex<K> pi{(K)hdr[0], (uint32_t)hdr[1]};
+    __syncthreads();
+    block_exclusive_scan(lstart, d.nb, wsum);
+    for (uint32_t b = threadIdx.x; b < d.nb; b += blockDim.x) lcur[b] = lstart[b];
+    __syncthreads();
+    constexpr int E = per_thread<K>();
+    const uint64_t lo = (uint64_t)tile * d.tile;
+    const uint32_t cnt = (uint32_t)(std::min<uint64_t>(lo + d.tile, nq) - lo);
+    K key[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const uint32_t i = e * kBBlock + threadIdx.x;
+        key[e] = i < cnt ? __builtin_nontemporal_load(&q[lo + i]) : K(0);
+    }
+    uint32_t bk[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) bk[e] = bucket_of<K>(bsep, d.nb, btab, pi, key[e]);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const uint32_t i = e * kBBlock + threadIdx.x;
+        if (i < cnt) {
+            const uint32_t slot = atomicAdd(&lcur[bk[e]], 1u);
+            sk[slot] = key[e];
+            sl[slot] = (uint16_t)i;
+            sb[slot] = (uint16_t)bk[e];
+        }
+    }
+    __syncthreads();
+    // tile-sorted element j goes to its bucket run: consecutive j, consecutive slots
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const uint32_t j = e * kBBlock + threadIdx.x;
+        if (j < cnt) {
+            const uint32_t b = sb[j];
+            const uint32_t g = gstart[b] + (j - lstart[b]);
+            qs[g] = sk[j];
+            lidx[g] = sl[j];
+        }
+    }
+}
+
+// LDS: bstart[nb + 1] | slice[epb] | stab[2^10 + 1]
+// Work: the bucket-ordered queries in chunks of d.chunk. With d.xcd the
+// chunks are split into 8 contiguous parts, one per XCD, and the blocks of an
+// XCD stride through their part together, so an XCD works on about one bucket
+// at a time and that bucket's leaves stay in its L2.
+constexpr int kSearchBlock = 512;  // 8 waves share one staged slice; two blocks per CU
+
+template <class K, class V, int F, int LF, bool COOP, int G, bool OFF32>
+__global__ __launch_bounds__(kSearchBlock) void k_bkt_search(TreeView t, BucketDev d, const K *__restrict__ qs,
+                                                        const uint32_t *__restrict__ bstart_g, uint64_t nq,
+                                                        V *__restrict__ res, V sentinel) {
+    extern __shared__ uint4 lds_raw[];
+    uint32_t *bstart = reinterpret_cast<uint32_t *>(lds_raw);
+    K *slice = reinterpret_cast<K *>(lds_raw + (d.nb + 1 + 3) / 4);
+    uint16_t *stab = reinterpret_cast<uint16_t *>(slice + d.epb);
+    for (uint32_t b = threadIdx.x; b <= d.nb; b += blockDim.x) bstart[b] = bstart_g[b];
+    __syncthreads();
+    const K *level = reinterpret_cast<const K *>(t.inner + t.off[d.level]);
+    const uint64_t level_count = t.count[d.level - 1];  // entries of level s = nodes of level s-1
+    const int top = (int)d.level - 1;
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const uint64_t nchunks = (nq + d.chunk - 1) / d.chunk;
+    uint64_t c0 = 0, c1 = nchunks, cstep = gridDim.x, cfirst = blockIdx.x;
+    if (d.xcd && gridDim.x % kXcds == 0) {
+        const uint32_t x = blockIdx.x % kXcds;
+        const uint64_t per = nchunks / kXcds, rem = nchunks % kXcds;
+        c0 = x * per + (x < rem ? x : rem);
+        c1 = c0 + per + (x < rem ? 1 : 0);
+        cstep = gridDim.x / kXcds;
+        cfirst = c0 + blockIdx.x / kXcds;
+    }
+    uint32_t cur = ~0u;  // bucket whose slice is staged
+    uint32_t eb = 0;
+    uint64_t first = 0;
+    PrefixIndex<K> pi{K(0), 0};
+    for (uint64_t chunk = cfirst; chunk < c1; chunk += cstep) {
+        uint64_t pos = chunk * d.chunk;
+        const uint64_t end = std::min<uint64_t>(pos + d.chunk, nq);
+        while (pos < end) {  // the same in every thread of the block
+            uint32_t lo = 0, hi = d.nb;  // last bucket b with bstart[b] <= pos
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (bstart[mid] <= pos) lo = mid; else hi = mid;
+            }
+            const uint32_t b = lo;
+            const uint64_t seg_end = std::min<uint64_t>(end, bstart[b + 1]);
+            if (b != cur) {
+                __syncthreads();  // every wave is done with the previous slice
+                first = (uint64_t)b * d.epb;
+                eb = (uint32_t)std::min<uint64_t>(d.epb, level_count - first);
+                // 16-byte copies when the slice starts on a 16-byte boundary (all but
+                // odd-sized single buckets), element copies for the rest
+                const uint32_t per16 = 16 / sizeof(K);
+                const uint32_t nvec = (first % per16 == 0) ? eb / per16 : 0;
+                const uint4 *src4 = reinterpret_cast<const uint4 *>(level + first);
+                for (uint32_t e = threadIdx.x; e < nvec; e += blockDim.x) reinterpret_cast<uint4 *>(slice)[e] = src4[e];
+                for (uint32_t e = nvec * per16 + threadIdx.x; e < eb; e += blockDim.x) slice[e] = level[first + e];
+                __syncthreads();
+                pi = prefix_index<K>(slice[0], slice[eb - 1], kSliceTabBits);
+                for (uint32_t p = threadIdx.x; p <= (1u << kSliceTabBits); p += blockDim.x)
+                    stab[p] = (uint16_t)(p == (1u << kSliceTabBits)
+                                             ? eb : lower_bound_flat<K>(slice, eb, prefix_start<K>(pi, p)));
+                __syncthreads();
+                cur = b;
+            }
+            for (uint64_t base = pos + (uint64_t)w * 64; base < seg_end; base += (uint64_t)nw * 64) {
+                const uint64_t my = base + lane;
+                const bool ok = my < seg_end;
+                const K key = ok ? __builtin_nontemporal_load(&qs[my]) : K(0);
+                const uint32_t p = prefix_of<K>(pi, key, kSliceTabBits);
+                const uint32_t at = lower_bound_window<K>(slice, stab[p], stab[p + 1], key);
+                const uint32_t node = (uint32_t)first + (at < eb ? at : eb - 1);
+                V r;
+                if constexpr (COOP) r = coop_finish64<K, V, F, LF, true, kOpValue, G, OFF32>(t, key, node, top, sentinel);
+                else r = thread_finish<K, V, F, LF>(t, key, node, top, sentinel);
+                if (ok) __builtin_nontemporal_store(r, &res[my]);
+            }
+            pos = seg_end;
+        }
+    }
+}
+
+// LDS: gstart[nb] | lstart[nb + 1] | outt[T]
+template <class K, class V>
+__global__ __launch_bounds__(kBBlock) void k_bkt_unpermute(const uint32_t *__restrict__ hist,
+                                                           const uint32_t *__restrict__ off, BucketDev d,
+                                                           const uint16_t *__restrict__ lidx,
+                                                           const V *__restrict__ res, uint64_t nq,
+                                                           V *__restrict__ out) {
+    extern __shared__ uint4 lds_raw[];
+    __shared__ uint32_t wsum[kBBlock / 64];
+    uint32_t *gstart = reinterpret_cast<uint32_t *>(lds_raw);
+    uint32_t *lstart = gstart + d.nb;
+    V *outt = reinterpret_cast<V *>(lds_raw + (2 * d.nb + 1 + 3) / 4);
+    const uint32_t tile = xcd_contiguous(blockIdx.x, gridDim.x);
+    for (uint32_t b = threadIdx.x; b < d.nb; b += blockDim.x) {
+        const uint64_t e = (uint64_t)tile * d.nb + b;
+        gstart[b] = off[e];
+        lstart[b] = hist[e];
+    }
+    __syncthreads();
+    const uint32_t total = block_exclusive_scan(lstart, d.nb, wsum);
+    if (threadIdx.x == 0) lstart[d.nb] = total;
+    __syncthreads();
+    constexpr int E = per_thread<K>();
+    const uint64_t lo = (uint64_t)tile * d.tile;
+    const uint32_t cnt = (uint32_t)(std::min<uint64_t>(lo + d.tile, nq) - lo);
+    uint32_t g[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const uint32_t j = e * kBBlock + threadIdx.x;
+        // bucket of tile-sorted element j: the last b with lstart[b] <= j
+        // (lstart[nb] = cnt bounds it; trip count depends on nb only)
+        uint32_t a = 0, n = d.nb;
+        while (n > 1) {
+            const uint32_t half = n >> 1;
+            a = lstart[a + half] <= j ? a + half : a;
+            n -= half;
+        }
+        g[e] = gstart[a] + (j - lstart[a]);
+    }
+    uint16_t li[E];
+    V rv[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const bool ok = e * kBBlock + threadIdx.x < cnt;
+        li[e] = ok ? lidx[g[e]] : (uint16_t)0;
+        rv[e] = ok ? __builtin_nontemporal_load(&res[g[e]]) : V(0);
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+        if (e * kBBlock + threadIdx.x < cnt) outt[li[e]] = rv[e];
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const uint32_t i = e * kBBlock + threadIdx.x;
+        if (i < cnt) __builtin_nontemporal_store(outt[i], &out[lo + i]);
+    }
+}
+
+// ------------------------------------------------------------------ host side
+struct BucketPlan {
+    BucketDev d;
+    uint32_t grid;                 // search blocks
+    uint64_t hist_entries;
+    uint64_t off_qs, off_lidx, off_res, off_hist, off_off, off_cols, off_bsep, off_bstart, off_hdr, off_btab,
+        total;
+    size_t lds_hist, lds_scatter, lds_search, lds_unpermute;
+};
+
+static BucketPlan bucket_plan(const bpt_tree *t, uint64_t nq) {
+    BucketPlan p{};
+    const TreeView &v = t->view;
+    const uint64_t L = v.count[0], F = t->fanout;
+    const uint64_t slice_cap = kSliceBytes / t->key_bytes;
+    const uint64_t nb_cap = std::min<uint64_t>(1ull << t->group_bits, kMaxBuckets);
+    uint64_t span = slice_cap;  // leaves per bucket at level s = slice_cap * F^(s-1)
+    uint32_t s = 1;
+    for (;; ++s) {
+        if (span >= L || (L + span - 1) / span <= nb_cap || s == (uint32_t)v.height) break;
+        span *= F;
+    }
+    BucketDev &d = p.d;
+    d.level = s;
+    d.lpb = std::min(span, L);
+    d.nb = (uint32_t)((L + d.lpb - 1) / d.lpb);
+    uint64_t per = 1;  // leaves per entry of level s
+    for (uint32_t k = 1; k < s; ++k) per *= F;
+    d.epb = (uint32_t)((d.lpb + per - 1) / per);
+    d.tile = t->key_bytes == 4 ? tile_of<uint32_t>() : tile_of<uint64_t>();
+    d.ntiles = (uint32_t)std::max<uint64_t>(1, (nq + d.tile - 1) / d.tile);
+    d.ngroups = (d.ntiles + kColGroup - 1) / kColGroup;
+    d.chunk = t->group_chunk;
+    const uint64_t nchunks = std::max<uint64_t>(1, (nq + d.chunk - 1) / d.chunk);
+    const uint64_t cap = (uint64_t)device_cu_count() * 2;
+    p.grid = (uint32_t)std::min(cap, nchunks);
+    // split per XCD only when every XCD gets whole strides of chunks
+    d.xcd = (t->group_xcd && p.grid % kXcds == 0 && nchunks >= 2ull * p.grid) ? 1u : 0u;
+    p.hist_entries = (uint64_t)d.nb * d.ntiles;
+    uint64_t off = 0;
+    p.off_qs = off;     off += align256(nq * t->key_bytes);
+    p.off_lidx = off;   off += align256(nq * 2);
+    p.off_res = off;    off += align256(nq * t->val_bytes);
+    p.off_hist = off;   off += align256(p.hist_entries * 4);
+    p.off_off = off;    off += align256(p.hist_entries * 4);
+    p.off_cols = off;   off += align256((uint64_t)d.ngroups * d.nb * 4);
+    p.off_bsep = off;   off += align256((uint64_t)d.nb * t->key_bytes);
+    p.off_bstart = off; off += align256(((uint64_t)d.nb + 1) * 4);
+    p.off_hdr = off;    off += align256(16);
+    p.off_btab = off;   off += align256(((1ull << kBucketTabBits) + 1) * 2);
+    p.total = off;
+    const size_t kb = t->key_bytes, vb = t->val_bytes;
+    const size_t btab_bytes = ((1u << kBucketTabBits) + 1) * 2, stab_bytes = ((1u << kSliceTabBits) + 1) * 2;
+    p.lds_hist = d.nb * (kb + 4) + btab_bytes;
+    p.lds_scatter = d.nb * (kb + 12) + (size_t)d.tile * (kb + 4) + btab_bytes;
+    p.lds_search = ((d.nb + 1 + 3) / 4) * 16 + (size_t)d.epb * kb + stab_bytes;
+    p.lds_unpermute = ((2 * d.nb + 1 + 3) / 4) * 16 + (size_t)d.tile * vb;
+    return p;
+}
+
+uint64_t point_grouped_ws(const bpt_tree *t, uint64_t nq) {
+    if (t->n == 0 || t->view.height == 0) return 0;
+    return bucket_plan(t, std::min(nq, kBucketChunk)).total;
+}
+
+template <class K, class V, int F, int LF>
+static bpt_status bucket_chunk(const bpt_tree *t, const K *q, uint64_t nq, V *out, uint8_t *ws, hipStream_t s) {
+    const BucketPlan p = bucket_plan(t, nq);
+    const BucketDev &d = p.d;
+    K *qs = reinterpret_cast<K *>(ws + p.off_qs);
+    uint16_t *lidx = reinterpret_cast<uint16_t *>(ws + p.off_lidx);
+    V *res = reinterpret_cast<V *>(ws + p.off_res);
+    uint32_t *hist = reinterpret_cast<uint32_t *>(ws + p.off_hist);
+    uint32_t *offs = reinterpret_cast<uint32_t *>(ws + p.off_off);
+    uint32_t *cols = reinterpret_cast<uint32_t *>(ws + p.off_cols);
+    K *bsep = reinterpret_cast<K *>(ws + p.off_bsep);
+    uint32_t *bstart = reinterpret_cast<uint32_t *>(ws + p.off_bstart);
+    uint64_t *hdr = reinterpret_cast<uint64_t *>(ws + p.off_hdr);
+    uint16_t *btab = reinterpret_cast<uint16_t *>(ws + p.off_btab);
+    BPT_CHECK(p.lds_scatter <= 65536 && p.lds_search <= 65536 && p.lds_unpermute <= 65536, BPT_ERR_INVALID,
+              "bucket plan exceeds LDS");
+    k_bkt_seps<K><<<(d.nb + 255) / 256, 256, 0, s>>>(t->view, d, bsep);
+    BPT_HIP(hipGetLastError());
+    k_bkt_btab<K><<<((1u << kBucketTabBits) + 256) / 256, kBBlock, 0, s>>>(t->view, d, bsep, hdr, btab);
+    BPT_HIP(hipGetLastError());
+    k_bkt_hist<K><<<d.ntiles, kBBlock, p.lds_hist, s>>>(q, nq, d, bsep, hdr, btab, hist);
+    BPT_HIP(hipGetLastError());
+    k_bkt_colsum<<<d.ngroups, kBBlock, 0, s>>>(hist, d, cols);
+    BPT_HIP(hipGetLastError());
+    k_bkt_colscan<<<1, kScanThreads, 0, s>>>(cols, d, bstart);
+    BPT_HIP(hipGetLastError());
+    k_bkt_colprefix<<<d.ngroups, kBBlock, 0, s>>>(hist, cols, d, offs);
+    BPT_HIP(hipGetLastError());
+    k_bkt_scatter<K><<<d.ntiles, kBBlock, p.lds_scatter, s>>>(q, nq, d, bsep, hdr, btab, hist, offs, qs, lidx);
+    BPT_HIP(hipGetLastError());
+    const V sentinel = ~V(0);
+    bool coop = false;
+    if constexpr (sizeof(K) == sizeof(V)) coop = t->coop && t->view.count[0] < (1ull << 32);
+    const bool off32 = t->addr32 && t->leaf_bytes <= (1ull << 32) && t->inner_bytes <= (1ull << 32);
+    auto go = [&](auto cp, auto lanes, auto o32) {
+        k_bkt_search<K, V, F, LF, decltype(cp)::value, decltype(lanes)::value, decltype(o32)::value>
+            <<<p.grid, kSearchBlock, p.lds_search, s>>>(t->view, d, qs, bstart, nq, res, sentinel);
+    };
+    if constexpr (sizeof(K) == sizeof(V)) {
+        if (coop) {
+            if (t->coop_lanes == 4) {
+                if (off32) go(std::true_type{}, IC<4>{}, std::true_type{});
+                else go(std::true_type{}, IC<4>{}, std::false_type{});
+            } else {
+                if (off32) go(std::true_type{}, IC<8>{}, std::true_type{});
+                else go(std::true_type{}, IC<8>{}, std::false_type{});
+            }
+        } else {
+            go(std::false_type{}, IC<8>{}, std::false_type{});
+        }
+    } else {
+        go(std::false_type{}, IC<8>{}, std::false_type{});
+    }
+    BPT_HIP(hipGetLastError());
+    k_bkt_unpermute<K, V><<<d.ntiles, kBBlock, p.lds_unpermute, s>>>(hist, offs, d, lidx, res, nq, out);
+    BPT_HIP(hipGetLastError());
+    return BPT_OK;
+}
+
+bpt_status point_grouped(const bpt_tree *t, const void *d_q, uint64_t nq, void *d_out, void *d_ws,
+                         uint64_t ws_bytes, hipStream_t s) {
+    if (nq == 0) return BPT_OK;
+    if (t->n == 0 || t->view.height == 0) return launch_plain(t, d_q, nq, d_out, s);
+    BPT_CHECK(d_ws && ws_bytes >= point_grouped_ws(t, nq), BPT_ERR_INVALID,
+              "grouped search needs a workspace of %llu bytes",
+              (unsigned long long)point_grouped_ws(t, nq));
+    return dispatch_tree(t, [&](auto kt, auto vt, auto fc, auto lc) -> bpt_status {
+        using K = typename decltype(kt)::type;
+        using V = typename decltype(vt)::type;
+        constexpr int F = decltype(fc)::value, LF = decltype(lc)::value;
+        for (uint64_t b = 0; b < nq; b += kBucketChunk) {
+            const uint64_t m = std::min(kBucketChunk, nq - b);
+            BPT_TRY((bucket_chunk<K, V, F, LF>(t, (const K *)d_q + b, m, (V *)d_out + b, (uint8_t *)d_ws, s)));
+        }
+        return BPT_OK;
+    });
+}
+
+}  // namespace bpt

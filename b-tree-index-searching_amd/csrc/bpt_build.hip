@@ -297,6 +297,16 @@ bpt_status bpt_tree_set_option(bpt_tree *t, const char *key, int64_t value) {
         t->addr32 = value ? 1 : 0;
         return BPT_OK;
     }
+    if (strcmp(key, "group_chunk") == 0) {
+        BPT_CHECK(value >= 256 && value <= (1 << 20) && value % 64 == 0, BPT_ERR_INVALID,
+                  "group_chunk must be a multiple of 64 in [256, 2^20]");
+        t->group_chunk = (uint32_t)value;
+        return BPT_OK;
+    }
+    if (strcmp(key, "group_xcd") == 0) {
+        t->group_xcd = value ? 1 : 0;
+        return BPT_OK;
+    }
     if (strcmp(key, "lanes") == 0) {
         BPT_CHECK(value == 4 || value == 8, BPT_ERR_INVALID, "lanes per query must be 4 or 8");
         t->coop_lanes = (uint32_t)value;

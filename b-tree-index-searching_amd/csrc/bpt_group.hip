@@ -23,256 +23,13 @@ bpt_status range_count_ordered(const bpt_tree *t, const void *d_lo, const void *
 
 static inline uint64_t align256(uint64_t b) { return (b + 255) / 256 * 256; }
 
-// ==================================================================== grouped point search
-// Queries are partitioned on their top `bits` key bits (default 12) in MSD
-// passes of at most 8 bits (8 + 4 for 12 bits): no pass scatters into more
-// than 256 runs per tile, so the partially written lines of a block stay in
-// L2 and merge there before reaching HBM. A pass over segments is
-//   k_pass_hist     digit counts per (segment, tile), laid out [segment][digit][tile]
-//   exclusive scan  of that flat array = the first slot of every run
-//   k_pass_scatter  key -> its slot; slot of every input position -> pos
-// The search walks the grouped queries (lanes of a wave share subtrees and a
-// bucket's leaves stay in L2), then one gather per pass, in reverse, brings
-// the results back to the caller's order with the same run locality.
-constexpr int kPBlock = 512;
-constexpr uint32_t kPTile = 16384;
-
-struct PassDesc {
-    uint32_t shift, bits, nseg, pad_;
-    const uint32_t *seg_begin;  // nseg + 1 entries
-    const uint32_t *tile_base;  // nseg + 1 entries: first tile of each segment
-};
-
-__device__ __forceinline__ uint32_t find_seg(const PassDesc &p, uint32_t tile) {
-    uint32_t lo = 0, hi = p.nseg;  // largest s with tile_base[s] <= tile
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (p.tile_base[mid] <= tile) lo = mid; else hi = mid;
-    }
-    return lo;
-}
-
-__device__ __forceinline__ uint64_t lanemask_lt64() {
-    const uint32_t lane = threadIdx.x & 63;
-    return lane ? (~0ull >> (64 - lane)) : 0ull;
-}
-
-template <class K>
-__global__ __launch_bounds__(kPBlock) void k_pass_hist(const K *__restrict__ keys, PassDesc p,
-                                                       uint32_t *__restrict__ hist) {
-    __shared__ uint32_t cnt[256];
-    const uint32_t tile = blockIdx.x;
-    if (tile >= p.tile_base[p.nseg]) return;  // whole block
-    const uint32_t s = find_seg(p, tile);
-    const uint32_t tiles_s = p.tile_base[s + 1] - p.tile_base[s], tl = tile - p.tile_base[s];
-    const uint64_t lo = (uint64_t)p.seg_begin[s] + (uint64_t)tl * kPTile;
-    const uint64_t hi = std::min<uint64_t>(lo + kPTile, p.seg_begin[s + 1]);
-    const uint32_t D = 1u << p.bits, mask = D - 1;
-    for (uint32_t d = threadIdx.x; d < D; d += blockDim.x) cnt[d] = 0;
-    __syncthreads();
-    for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x)
-        atomicAdd(&cnt[(uint32_t)(keys[i] >> p.shift) & mask], 1u);
-    __syncthreads();
-    uint32_t *row = hist + (uint64_t)D * p.tile_base[s] + tl;
-    for (uint32_t d = threadIdx.x; d < D; d += blockDim.x) row[(uint64_t)d * tiles_s] = cnt[d];
-}
-
-template <class K>
-__global__ __launch_bounds__(kPBlock) void k_pass_scatter(const K *__restrict__ keys_in, PassDesc p,
-                                                          const uint32_t *__restrict__ offs,
-                                                          K *__restrict__ keys_out,
-                                                          uint32_t *__restrict__ pos_out) {
-    __shared__ uint32_t cur[256];
-    const uint32_t tile = blockIdx.x;
-    if (tile >= p.tile_base[p.nseg]) return;
-    const uint32_t s = find_seg(p, tile);
-    const uint32_t tiles_s = p.tile_base[s + 1] - p.tile_base[s], tl = tile - p.tile_base[s];
-    const uint64_t lo = (uint64_t)p.seg_begin[s] + (uint64_t)tl * kPTile;
-    const uint64_t hi = std::min<uint64_t>(lo + kPTile, p.seg_begin[s + 1]);
-    const uint32_t D = 1u << p.bits, mask = D - 1;
-    const uint32_t *row = offs + (uint64_t)D * p.tile_base[s] + tl;
-    for (uint32_t d = threadIdx.x; d < D; d += blockDim.x) cur[d] = row[(uint64_t)d * tiles_s];
-    __syncthreads();
-    for (uint64_t b = lo; b < hi; b += blockDim.x) {
-        const uint64_t i = b + threadIdx.x;
-        const bool ok = i < hi;
-        const K key = ok ? keys_in[i] : K(0);
-        const uint32_t d = (uint32_t)(key >> p.shift) & mask;
-        // lanes of the wave with the same digit claim consecutive slots at once
-        uint64_t peers = __ballot(ok);
-        for (uint32_t bit = 0; bit < p.bits; ++bit) {
-            const bool x = (d >> bit) & 1u;
-            const uint64_t m = __ballot(x);
-            peers &= x ? m : ~m;
-        }
-        const uint32_t rank = (uint32_t)__popcll(peers & lanemask_lt64());
-        const int leader = __ffsll((long long)peers) - 1;
-        uint32_t base = 0;
-        if (ok && rank == 0) base = atomicAdd(&cur[d], (uint32_t)__popcll(peers));
-        base = __shfl(base, leader < 0 ? 0 : leader, 64);
-        if (ok) {
-            const uint32_t slot = base + rank;
-            keys_out[slot] = key;
-            pos_out[i] = slot;
-        }
-    }
-}
-
-// One segment [0, n) split into tiles.
-__global__ void k_pass_init_single(uint32_t *seg_begin, uint32_t *tile_base, uint64_t n) {
-    seg_begin[0] = 0;
-    seg_begin[1] = (uint32_t)n;
-    tile_base[0] = 0;
-    tile_base[1] = (uint32_t)((n + kPTile - 1) / kPTile);
-}
-
-// Segments of pass 2 = digit ranges of pass 1 (first slot of each digit in
-// the scanned pass-1 histogram), tiled.
-__global__ __launch_bounds__(256) void k_pass_init_segments(const uint32_t *__restrict__ scanned1,
-                                                            uint32_t ntiles1, uint32_t nseg, uint64_t n,
-                                                            uint32_t *__restrict__ seg_begin,
-                                                            uint32_t *__restrict__ tile_base) {
-    __shared__ uint32_t sh[256];
-    __shared__ uint32_t wsum[4];
-    const uint32_t d = threadIdx.x;
-    const uint32_t begin = d < nseg ? scanned1[(uint64_t)d * ntiles1] : (uint32_t)n;
-    sh[d] = begin;
-    __syncthreads();
-    const uint32_t end = (d + 1 < nseg) ? sh[d + 1] : (uint32_t)n;
-    const uint32_t tiles = d < nseg ? (end - begin + kPTile - 1) / kPTile : 0;
-    // block exclusive scan of tiles
-    const int lane = d & 63, w = d >> 6;
-    uint32_t x = tiles;
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_up(x, o, 64);
-        if (lane >= o) x += y;
-    }
-    if (lane == 63) wsum[w] = x;
-    __syncthreads();
-    uint32_t add = 0;
-    for (int k = 0; k < w; ++k) add += wsum[k];
-    const uint32_t excl = x - tiles + add;
-    if (d < nseg) {
-        seg_begin[d] = begin;
-        tile_base[d] = excl;
-    }
-    if (d == nseg - 1) {
-        seg_begin[nseg] = (uint32_t)n;
-        tile_base[nseg] = excl + tiles;
-    }
-}
-
+// ==================================================================== gather
 template <class V>
 __global__ __launch_bounds__(256) void k_gather(const V *__restrict__ src, const uint32_t *__restrict__ idx,
                                                 uint64_t n, V *__restrict__ dst) {
     const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step)
         __builtin_nontemporal_store(src[__builtin_nontemporal_load(&idx[i])], &dst[i]);
-}
-
-struct GroupPlan {
-    uint32_t bits1, bits2, ntiles, maxtiles2;
-    uint64_t off_key1, off_pos1, off_key2, off_pos2, off_res, off_resa, off_hist, off_seg, off_scan;
-    uint64_t hist_entries, scan_bytes, total;
-};
-
-static GroupPlan group_plan(const bpt_tree *t, uint64_t nq) {
-    GroupPlan p{};
-    const uint32_t bits = t->group_bits;
-    p.bits1 = bits > 8 ? 8 : bits;
-    p.bits2 = bits - p.bits1;
-    p.ntiles = (uint32_t)std::max<uint64_t>(1, (nq + kPTile - 1) / kPTile);
-    p.maxtiles2 = p.ntiles + (1u << p.bits1);
-    p.hist_entries = 256ull * p.maxtiles2;
-    p.scan_bytes = scan_tmp_bytes(p.hist_entries, 4);
-    uint64_t off = 0;
-    p.off_key1 = off; off += align256(nq * t->key_bytes);
-    p.off_pos1 = off; off += align256(nq * 4);
-    p.off_key2 = off; off += align256(nq * t->key_bytes);
-    p.off_pos2 = off; off += align256(nq * 4);
-    p.off_res = off;  off += align256(nq * t->val_bytes);
-    p.off_resa = off; off += align256(nq * t->val_bytes);
-    p.off_hist = off; off += align256(p.hist_entries * 4);
-    p.off_seg = off;  off += align256(4 * 4 * 260);
-    p.off_scan = off; off += align256(p.scan_bytes);
-    p.total = off;
-    return p;
-}
-
-uint64_t point_grouped_ws(const bpt_tree *t, uint64_t nq) {
-    const uint64_t chunk = std::min<uint64_t>(nq, 1ull << 31);
-    return group_plan(t, chunk).total;
-}
-
-template <class K, class V>
-static bpt_status grouped_chunk(const bpt_tree *t, const K *q, uint64_t nq, V *out, uint8_t *ws,
-                                hipStream_t s) {
-    const GroupPlan p = group_plan(t, nq);
-    K *key1 = reinterpret_cast<K *>(ws + p.off_key1);
-    K *key2 = reinterpret_cast<K *>(ws + p.off_key2);
-    uint32_t *pos1 = reinterpret_cast<uint32_t *>(ws + p.off_pos1);
-    uint32_t *pos2 = reinterpret_cast<uint32_t *>(ws + p.off_pos2);
-    V *res = reinterpret_cast<V *>(ws + p.off_res);
-    V *resa = reinterpret_cast<V *>(ws + p.off_resa);
-    uint32_t *hist = reinterpret_cast<uint32_t *>(ws + p.off_hist);
-    uint32_t *seg = reinterpret_cast<uint32_t *>(ws + p.off_seg);
-    uint32_t *seg1 = seg, *tb1 = seg + 260, *seg2 = seg + 520, *tb2 = seg + 780;
-    void *scan_tmp = ws + p.off_scan;
-    const uint32_t kbits = t->key_bytes * 8;
-
-    // pass 1: top bits1 bits, one segment
-    PassDesc d1{kbits - p.bits1, p.bits1, 1, 0, seg1, tb1};
-    k_pass_init_single<<<1, 1, 0, s>>>(seg1, tb1, nq);
-    BPT_HIP(hipGetLastError());
-    k_pass_hist<K><<<p.ntiles, kPBlock, 0, s>>>(q, d1, hist);
-    BPT_HIP(hipGetLastError());
-    BPT_TRY(exclusive_scan<uint32_t>(hist, (uint64_t)p.ntiles << p.bits1, hist, scan_tmp, p.scan_bytes, s));
-    k_pass_scatter<K><<<p.ntiles, kPBlock, 0, s>>>(q, d1, hist, key1, pos1);
-    BPT_HIP(hipGetLastError());
-    const K *grouped = key1;
-    if (p.bits2) {
-        // pass 2: next bits2 bits inside each pass-1 digit
-        const uint32_t nseg = 1u << p.bits1;
-        k_pass_init_segments<<<1, 256, 0, s>>>(hist, p.ntiles, nseg, nq, seg2, tb2);
-        BPT_HIP(hipGetLastError());
-        PassDesc d2{kbits - p.bits1 - p.bits2, p.bits2, nseg, 0, seg2, tb2};
-        k_pass_hist<K><<<p.maxtiles2, kPBlock, 0, s>>>(key1, d2, hist);
-        BPT_HIP(hipGetLastError());
-        BPT_TRY(exclusive_scan<uint32_t>(hist, (uint64_t)p.maxtiles2 << p.bits2, hist, scan_tmp, p.scan_bytes, s));
-        k_pass_scatter<K><<<p.maxtiles2, kPBlock, 0, s>>>(key1, d2, hist, key2, pos2);
-        BPT_HIP(hipGetLastError());
-        grouped = key2;
-    }
-    BPT_TRY(launch_plain(t, grouped, nq, res, s));
-    const uint32_t g = search_grid(t, nq);
-    if (p.bits2) {
-        k_gather<V><<<g, 256, 0, s>>>(res, pos2, nq, resa);  // pass-2 order -> pass-1 order
-        BPT_HIP(hipGetLastError());
-        k_gather<V><<<g, 256, 0, s>>>(resa, pos1, nq, out);  // pass-1 order -> caller order
-    } else {
-        k_gather<V><<<g, 256, 0, s>>>(res, pos1, nq, out);
-    }
-    BPT_HIP(hipGetLastError());
-    return BPT_OK;
-}
-
-bpt_status point_grouped(const bpt_tree *t, const void *d_q, uint64_t nq, void *d_out, void *d_ws,
-                         uint64_t ws_bytes, hipStream_t s) {
-    if (nq == 0) return BPT_OK;
-    if (t->n == 0) return launch_plain(t, d_q, nq, d_out, s);
-    BPT_CHECK(d_ws && ws_bytes >= point_grouped_ws(t, nq), BPT_ERR_INVALID,
-              "grouped search needs a workspace of %llu bytes",
-              (unsigned long long)point_grouped_ws(t, nq));
-    return dispatch_tree(t, [&](auto kt, auto vt, auto, auto) -> bpt_status {
-        using K = typename decltype(kt)::type;
-        using V = typename decltype(vt)::type;
-        const uint64_t chunk = 1ull << 31;
-        for (uint64_t b = 0; b < nq; b += chunk) {
-            const uint64_t m = std::min(chunk, nq - b);
-            BPT_TRY(grouped_chunk<K, V>(t, (const K *)d_q + b, m, (V *)d_out + b, (uint8_t *)d_ws, s));
-        }
-        return BPT_OK;
-    });
 }
 
 // ==================================================================== stable LSD radix sort

@@ -83,6 +83,8 @@ struct bpt_tree {
     uint32_t leaf_nt = 1;            // non-temporal leaf loads
     uint32_t coop_lanes = 8;         // lanes per query of the cooperative descent (4 or 8)
     uint32_t addr32 = 1;             // 32-bit node / leaf offsets when the tree allows them
+    uint32_t group_chunk = 16384;    // grouped search: queries per work chunk
+    uint32_t group_xcd = 1;          // grouped search: chunks split per XCD
     bpt::TreeView view{};
     bpt::Workspace ws;             // grouped / learned scratch owned by the tree
     bpt::LearnedModel *learned = nullptr;
@@ -219,6 +221,19 @@ __device__ __forceinline__ V leaf_lookup(const TreeView &t, uint64_t leaf, K q, 
     for (int j = LF - 1; j >= 0; --j)
         if (k[j] == q && base + j < t.n) res = v[j];
     return res;
+}
+
+// One lane per query from node `node` of level `top` (top = 0: node is the
+// leaf) down through the global levels to the value.
+template <class K, class V, int F, int LF>
+__device__ __forceinline__ V thread_finish(const TreeView &t, K q, uint64_t node, int top, V sentinel) {
+    for (int lvl = top; lvl >= 1; --lvl) {
+        const K *nd = reinterpret_cast<const K *>(t.inner + t.off[lvl]) + node * F;
+        const uint64_t c = node * F + node_rank<K, F>(nd, q);
+        const uint64_t last = t.count[lvl - 1] - 1;
+        node = c < last ? c : last;
+    }
+    return leaf_lookup<K, V, LF>(t, node, q, sentinel);
 }
 
 // Global lower_bound position of q inside leaf `leaf`.
@@ -408,9 +423,11 @@ __device__ __forceinline__ uint32_t lower_bound_flat(const K *__restrict__ a, ui
 constexpr int kOpValue = 0;
 constexpr int kOpLowerBound = 1;
 
+// Phase B alone: the lane's query `myq` sits in node `mynode` of level `top`
+// (top = 0: mynode is its leaf); walks levels top..1 and the leaf.
 template <class K, class V, int F, int LF, bool NT, int OP, int G = 8, bool OFF32 = false>
 __device__ __forceinline__ typename std::conditional<OP == kOpValue, V, uint64_t>::type
-coop_lookup64(const TreeView &t, const uint8_t *lds, K myq, V sentinel) {
+coop_finish64(const TreeView &t, K myq, uint32_t mynode, int top, V sentinel) {
     static_assert(sizeof(K) == sizeof(V), "cooperative leaf split needs equal key/value widths");
     static_assert(G == 4 || G == 8, "4 or 8 lanes per query");
     constexpr int NB = F * (int)sizeof(K) / G;       // node bytes per lane
@@ -421,16 +438,6 @@ coop_lookup64(const TreeView &t, const uint8_t *lds, K myq, V sentinel) {
     static_assert(NB >= 8 && LB >= 8, "chunks of at least 8 bytes");
     using R = typename std::conditional<OP == kOpValue, V, uint64_t>::type;
     const uint32_t lane = threadIdx.x & 63, g = lane / G, sub = lane % G;
-    const int flat_lvl = t.height - t.lds_levels + 1;
-    const bool has_flat = t.lds_levels > 0;
-    uint32_t mynode = 0;
-    if (has_flat) {
-        const K *flat = reinterpret_cast<const K *>(lds + t.off[flat_lvl]);
-        const uint32_t flat_n = (uint32_t)t.count[flat_lvl - 1];
-        const uint32_t c = lower_bound_flat<K>(flat, flat_n, myq);
-        mynode = c < flat_n - 1 ? c : flat_n - 1;
-    }
-    const int top = has_flat ? flat_lvl - 1 : t.height;  // first level walked cooperatively
     K qk[G];
     uint32_t node[G];
 #pragma unroll
@@ -493,6 +500,23 @@ coop_lookup64(const TreeView &t, const uint8_t *lds, K myq, V sentinel) {
         }
     }
     return res;
+}
+
+// Phase A (LDS levels, one lane per query) then phase B.
+template <class K, class V, int F, int LF, bool NT, int OP, int G = 8, bool OFF32 = false>
+__device__ __forceinline__ typename std::conditional<OP == kOpValue, V, uint64_t>::type
+coop_lookup64(const TreeView &t, const uint8_t *lds, K myq, V sentinel) {
+    const int flat_lvl = t.height - t.lds_levels + 1;
+    const bool has_flat = t.lds_levels > 0;
+    uint32_t mynode = 0;
+    if (has_flat) {
+        const K *flat = reinterpret_cast<const K *>(lds + t.off[flat_lvl]);
+        const uint32_t flat_n = (uint32_t)t.count[flat_lvl - 1];
+        const uint32_t c = lower_bound_flat<K>(flat, flat_n, myq);
+        mynode = c < flat_n - 1 ? c : flat_n - 1;
+    }
+    const int top = has_flat ? flat_lvl - 1 : t.height;  // first level walked cooperatively
+    return coop_finish64<K, V, F, LF, NT, OP, G, OFF32>(t, myq, mynode, top, sentinel);
 }
 
 // ---------------------------------------------------------------- host helpers
