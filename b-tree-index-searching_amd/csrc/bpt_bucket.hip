@@ -34,6 +34,7 @@ namespace bpt {
 constexpr int kBBlock = 256;
 constexpr uint32_t kSliceBytes = 32768;    // LDS slice of the search kernel
 constexpr uint32_t kMaxBuckets = 1024;     // runs per partition tile
+constexpr size_t kMaxLds = 160 * 1024;     // LDS of a workgroup (gfx950)
 constexpr uint64_t kBucketChunk = 1ull << 28;  // queries per partition (u32 slots)
 
 constexpr uint32_t kColGroup = 256;          // tiles per column-sum group
@@ -52,10 +53,13 @@ struct BucketDev {
 
 static inline uint64_t align256(uint64_t b) { return (b + 255) / 256 * 256; }
 
-// Partition tile: T queries, kBBlock threads with T / kBBlock each (all of a
-// thread's loads are issued before its first bucket search).
-template <class K> constexpr uint32_t tile_of() { return sizeof(K) == 4 ? 4096u : 2048u; }
-template <class K> constexpr int per_thread() { return (int)(tile_of<K>() / kBBlock); }
+// Partition tile: T queries, kPBlock threads with T / kPBlock each (all of a
+// thread's loads are issued before its first bucket search). Large tiles keep
+// the (tile, bucket) runs long, so the runs are written and read back as
+// whole lines; a tile's staging uses up to ~144 KiB of the 160 KiB of LDS.
+constexpr int kPBlock = 1024;
+template <class K> constexpr uint32_t tile_of() { return sizeof(K) == 4 ? 16384u : 8192u; }
+template <class K> constexpr int per_thread() { return (int)(tile_of<K>() / kPBlock); }
 
 // Key-prefix index over a sorted array a[0, n) with a[0] >= base: entry p of
 // its table (p < 2^bits) is the lower bound of base + (p << shift) in a, entry
@@ -179,7 +183,7 @@ __global__ __launch_bounds__(kBBlock) void k_bkt_btab(TreeView t, BucketDev d, c
 }
 
 template <class K>
-__global__ __launch_bounds__(kBBlock) void k_bkt_hist(const K *__restrict__ q, uint64_t nq, BucketDev d,
+__global__ __launch_bounds__(kPBlock) void k_bkt_hist(const K *__restrict__ q, uint64_t nq, BucketDev d,
                                                       const K *__restrict__ bsep_g, const uint64_t *__restrict__ hdr,
                                                       const uint16_t *__restrict__ btab_g,
                                                       uint32_t *__restrict__ hist) {
@@ -200,7 +204,7 @@ __global__ __launch_bounds__(kBBlock) void k_bkt_hist(const K *__restrict__ q, u
     K key[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-        const uint32_t i = e * kBBlock + threadIdx.x;
+        const uint32_t i = e * kPBlock + threadIdx.x;
         key[e] = i < cq ? __builtin_nontemporal_load(&q[lo + i]) : K(0);
     }
     __syncthreads();
@@ -209,7 +213,7 @@ __global__ __launch_bounds__(kBBlock) void k_bkt_hist(const K *__restrict__ q, u
     for (int e = 0; e < E; ++e) bk[e] = bucket_of<K>(bsep, d.nb, btab, pi, key[e]);
 #pragma unroll
     for (int e = 0; e < E; ++e)
-        if (e * kBBlock + threadIdx.x < cq) atomicAdd(&cnt[bk[e]], 1u);
+        if (e * kPBlock + threadIdx.x < cq) atomicAdd(&cnt[bk[e]], 1u);
     __syncthreads();
     uint32_t *row = hist + (uint64_t)tile * d.nb;
     for (uint32_t b = threadIdx.x; b < d.nb; b += blockDim.x) row[b] = cnt[b];
@@ -274,7 +278,7 @@ __global__ __launch_bounds__(kBBlock) void k_bkt_colprefix(const uint32_t *__res
 // LDS (widest first, so every array is aligned):
 //   sk[T] | bsep[nb] | gstart[nb] | lstart[nb] | lcur[nb] | sl[T] | sb[T] | btab[2^10 + 1]
 template <class K>
-__global__ __launch_bounds__(kBBlock) void k_bkt_scatter(const K *__restrict__ q, uint64_t nq, BucketDev d,
+__global__ __launch_bounds__(kPBlock) void k_bkt_scatter(const K *__restrict__ q, uint64_t nq, BucketDev d,
                                                          const K *__restrict__ bsep_g,
                                                          const uint64_t *__restrict__ hdr,
                                                          const uint16_t *__restrict__ btab_g,
@@ -282,7 +286,7 @@ __global__ __launch_bounds__(kBBlock) void k_bkt_scatter(const K *__restrict__ q
                                                          const uint32_t *__restrict__ off,
                                                          K *__restrict__ qs, uint16_t *__restrict__ lidx) {
     extern __shared__ uint4 lds_raw[];
-    __shared__ uint32_t wsum[kBBlock / 64];
+    __shared__ uint32_t wsum[kPBlock / 64];
     K *sk = reinterpret_cast<K *>(lds_raw);
     K *bsep = sk + d.tile;
     uint32_t *gstart = reinterpret_cast<uint32_t *>(bsep + d.nb);
@@ -310,7 +314,7 @@ __global__ __launch_bounds__(kBBlock) void k_bkt_scatter(const K *__restrict__ q
     K key[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-        const uint32_t i = e * kBBlock + threadIdx.x;
+        const uint32_t i = e * kPBlock + threadIdx.x;
         key[e] = i < cnt ? __builtin_nontemporal_load(&q[lo + i]) : K(0);
     }
     uint32_t bk[E];
@@ -318,7 +322,7 @@ __global__ __launch_bounds__(kBBlock) void k_bkt_scatter(const K *__restrict__ q
     for (int e = 0; e < E; ++e) bk[e] = bucket_of<K>(bsep, d.nb, btab, pi, key[e]);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-        const uint32_t i = e * kBBlock + threadIdx.x;
+        const uint32_t i = e * kPBlock + threadIdx.x;
         if (i < cnt) {
             const uint32_t slot = atomicAdd(&lcur[bk[e]], 1u);
             sk[slot] = key[e];
@@ -330,7 +334,7 @@ __global__ __launch_bounds__(kBBlock) void k_bkt_scatter(const K *__restrict__ q
     // tile-sorted element j goes to its bucket run: consecutive j, consecutive slots
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-        const uint32_t j = e * kBBlock + threadIdx.x;
+        const uint32_t j = e * kPBlock + threadIdx.x;
         if (j < cnt) {
             const uint32_t b = sb[j];
             const uint32_t g = gstart[b] + (j - lstart[b]);
@@ -413,7 +417,8 @@ __global__ __launch_bounds__(kSearchBlock) void k_bkt_search(TreeView t, BucketD
                 const uint32_t at = lower_bound_window<K>(slice, stab[p], stab[p + 1], key);
                 const uint32_t node = (uint32_t)first + (at < eb ? at : eb - 1);
                 V r;
-                if constexpr (COOP) r = coop_finish64<K, V, F, LF, true, kOpValue, G, OFF32>(t, key, node, top, sentinel);
+                // leaves are reused by every query of the bucket: regular (temporal) loads
+                if constexpr (COOP) r = coop_finish64<K, V, F, LF, false, kOpValue, G, OFF32>(t, key, node, top, sentinel);
                 else r = thread_finish<K, V, F, LF>(t, key, node, top, sentinel);
                 if (ok) __builtin_nontemporal_store(r, &res[my]);
             }
@@ -424,13 +429,13 @@ __global__ __launch_bounds__(kSearchBlock) void k_bkt_search(TreeView t, BucketD
 
 // LDS: gstart[nb] | lstart[nb + 1] | outt[T]
 template <class K, class V>
-__global__ __launch_bounds__(kBBlock) void k_bkt_unpermute(const uint32_t *__restrict__ hist,
+__global__ __launch_bounds__(kPBlock) void k_bkt_unpermute(const uint32_t *__restrict__ hist,
                                                            const uint32_t *__restrict__ off, BucketDev d,
                                                            const uint16_t *__restrict__ lidx,
                                                            const V *__restrict__ res, uint64_t nq,
                                                            V *__restrict__ out) {
     extern __shared__ uint4 lds_raw[];
-    __shared__ uint32_t wsum[kBBlock / 64];
+    __shared__ uint32_t wsum[kPBlock / 64];
     uint32_t *gstart = reinterpret_cast<uint32_t *>(lds_raw);
     uint32_t *lstart = gstart + d.nb;
     V *outt = reinterpret_cast<V *>(lds_raw + (2 * d.nb + 1 + 3) / 4);
@@ -450,7 +455,7 @@ __global__ __launch_bounds__(kBBlock) void k_bkt_unpermute(const uint32_t *__res
     uint32_t g[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-        const uint32_t j = e * kBBlock + threadIdx.x;
+        const uint32_t j = e * kPBlock + threadIdx.x;
         // bucket of tile-sorted element j: the last b with lstart[b] <= j
         // (lstart[nb] = cnt bounds it; trip count depends on nb only)
         uint32_t a = 0, n = d.nb;
@@ -465,17 +470,17 @@ __global__ __launch_bounds__(kBBlock) void k_bkt_unpermute(const uint32_t *__res
     V rv[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-        const bool ok = e * kBBlock + threadIdx.x < cnt;
+        const bool ok = e * kPBlock + threadIdx.x < cnt;
         li[e] = ok ? lidx[g[e]] : (uint16_t)0;
         rv[e] = ok ? __builtin_nontemporal_load(&res[g[e]]) : V(0);
     }
 #pragma unroll
     for (int e = 0; e < E; ++e)
-        if (e * kBBlock + threadIdx.x < cnt) outt[li[e]] = rv[e];
+        if (e * kPBlock + threadIdx.x < cnt) outt[li[e]] = rv[e];
     __syncthreads();
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-        const uint32_t i = e * kBBlock + threadIdx.x;
+        const uint32_t i = e * kPBlock + threadIdx.x;
         if (i < cnt) __builtin_nontemporal_store(outt[i], &out[lo + i]);
     }
 }
@@ -559,13 +564,13 @@ static bpt_status bucket_chunk(const bpt_tree *t, const K *q, uint64_t nq, V *ou
     uint32_t *bstart = reinterpret_cast<uint32_t *>(ws + p.off_bstart);
     uint64_t *hdr = reinterpret_cast<uint64_t *>(ws + p.off_hdr);
     uint16_t *btab = reinterpret_cast<uint16_t *>(ws + p.off_btab);
-    BPT_CHECK(p.lds_scatter <= 65536 && p.lds_search <= 65536 && p.lds_unpermute <= 65536, BPT_ERR_INVALID,
+    BPT_CHECK(p.lds_scatter <= kMaxLds && p.lds_search <= kMaxLds && p.lds_unpermute <= kMaxLds, BPT_ERR_INVALID,
               "bucket plan exceeds LDS");
     k_bkt_seps<K><<<(d.nb + 255) / 256, 256, 0, s>>>(t->view, d, bsep);
     BPT_HIP(hipGetLastError());
     k_bkt_btab<K><<<((1u << kBucketTabBits) + 256) / 256, kBBlock, 0, s>>>(t->view, d, bsep, hdr, btab);
     BPT_HIP(hipGetLastError());
-    k_bkt_hist<K><<<d.ntiles, kBBlock, p.lds_hist, s>>>(q, nq, d, bsep, hdr, btab, hist);
+    k_bkt_hist<K><<<d.ntiles, kPBlock, p.lds_hist, s>>>(q, nq, d, bsep, hdr, btab, hist);
     BPT_HIP(hipGetLastError());
     k_bkt_colsum<<<d.ngroups, kBBlock, 0, s>>>(hist, d, cols);
     BPT_HIP(hipGetLastError());
@@ -573,7 +578,7 @@ static bpt_status bucket_chunk(const bpt_tree *t, const K *q, uint64_t nq, V *ou
     BPT_HIP(hipGetLastError());
     k_bkt_colprefix<<<d.ngroups, kBBlock, 0, s>>>(hist, cols, d, offs);
     BPT_HIP(hipGetLastError());
-    k_bkt_scatter<K><<<d.ntiles, kBBlock, p.lds_scatter, s>>>(q, nq, d, bsep, hdr, btab, hist, offs, qs, lidx);
+    k_bkt_scatter<K><<<d.ntiles, kPBlock, p.lds_scatter, s>>>(q, nq, d, bsep, hdr, btab, hist, offs, qs, lidx);
     BPT_HIP(hipGetLastError());
     const V sentinel = ~V(0);
     bool coop = false;
@@ -585,7 +590,7 @@ static bpt_status bucket_chunk(const bpt_tree *t, const K *q, uint64_t nq, V *ou
     };
     if constexpr (sizeof(K) == sizeof(V)) {
         if (coop) {
-            if (t->coop_lanes == 4) {
+            if (t->group_lanes == 4) {
                 if (off32) go(std::true_type{}, IC<4>{}, std::true_type{});
                 else go(std::true_type{}, IC<4>{}, std::false_type{});
             } else {
@@ -599,7 +604,7 @@ static bpt_status bucket_chunk(const bpt_tree *t, const K *q, uint64_t nq, V *ou
         go(std::false_type{}, IC<8>{}, std::false_type{});
     }
     BPT_HIP(hipGetLastError());
-    k_bkt_unpermute<K, V><<<d.ntiles, kBBlock, p.lds_unpermute, s>>>(hist, offs, d, lidx, res, nq, out);
+    k_bkt_unpermute<K, V><<<d.ntiles, kPBlock, p.lds_unpermute, s>>>(hist, offs, d, lidx, res, nq, out);
     BPT_HIP(hipGetLastError());
     return BPT_OK;
 }

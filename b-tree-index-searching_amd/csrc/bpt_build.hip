@@ -303,6 +303,11 @@ bpt_status bpt_tree_set_option(bpt_tree *t, const char *key, int64_t value) {
         t->group_chunk = (uint32_t)value;
         return BPT_OK;
     }
+    if (strcmp(key, "group_lanes") == 0) {
+        BPT_CHECK(value == 4 || value == 8, BPT_ERR_INVALID, "group_lanes must be 4 or 8");
+        t->group_lanes = (uint32_t)value;
+        return BPT_OK;
+    }
     if (strcmp(key, "group_xcd") == 0) {
         t->group_xcd = value ? 1 : 0;
         return BPT_OK;

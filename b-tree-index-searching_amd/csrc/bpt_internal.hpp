@@ -85,6 +85,7 @@ struct bpt_tree {
     uint32_t addr32 = 1;             // 32-bit node / leaf offsets when the tree allows them
     uint32_t group_chunk = 16384;    // grouped search: queries per work chunk
     uint32_t group_xcd = 1;          // grouped search: chunks split per XCD
+    uint32_t group_lanes = 4;        // grouped search: lanes per query at the leaf (4 or 8)
     bpt::TreeView view{};
     bpt::Workspace ws;             // grouped / learned scratch owned by the tree
     bpt::LearnedModel *learned = nullptr;

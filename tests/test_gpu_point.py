@@ -57,9 +57,9 @@ def test_point_plain_and_grouped(bpt, oracle, cfg, n):
     t.set_option("coop", 1)
     if kb * 8 >= 12:
         assert np.array_equal(t.point_search(q, bpt.GROUPED), want)
-        t.set_option("lanes", 4)
+        t.set_option("group_lanes", 8)
         assert np.array_equal(t.point_search(q, bpt.GROUPED), want)
-        t.set_option("lanes", 8)
+        t.set_option("group_lanes", 4)
     pos = t.lower_bound(q)
     assert np.array_equal(pos, oracle.lower_bound(keys, q))
     t.close()
