@@ -37,7 +37,7 @@ constexpr uint32_t kMaxBuckets = 1024;     // runs per partition tile
 constexpr size_t kMaxLds = 160 * 1024;     // LDS of a workgroup (gfx950)
 constexpr uint64_t kBucketChunk = 1ull << 28;  // queries per partition (u32 slots)
 
-constexpr uint32_t kColGroup = 256;          // tiles per column-sum group
+constexpr uint32_t kColGroup = 128;          // tiles per column-sum group
 
 struct BucketDev {
     uint32_t nb;       // buckets
@@ -231,9 +231,8 @@ __global__ __launch_bounds__(kBBlock) void k_bkt_colsum(const uint32_t *__restri
     }
 }
 
-// One block of kScanThreads: S[g][b] becomes the first slot of group g's
-// part of bucket b (bucket base + earlier groups); bstart[b] = bucket base,
-// bstart[nb] = total.
+// One block of kScanThreads: S[g][b] becomes the count of bucket b in the
+// groups before g; bstart[b] = first slot of bucket b, bstart[nb] = total.
 constexpr int kScanThreads = 1024;
 __global__ __launch_bounds__(kScanThreads) void k_bkt_colscan(uint32_t *__restrict__ S, BucketDev d,
                                                               uint32_t *__restrict__ bstart) {
@@ -251,22 +250,19 @@ __global__ __launch_bounds__(kScanThreads) void k_bkt_colscan(uint32_t *__restri
     }
     __syncthreads();
     const uint32_t total = block_exclusive_scan(tot, d.nb, wsum);
-    for (uint32_t b = threadIdx.x; b < d.nb; b += blockDim.x) {
-        const uint32_t base = tot[b];
-        bstart[b] = base;
-        for (uint32_t g = 0; g < d.ngroups; ++g) S[(uint64_t)g * d.nb + b] += base;
-    }
+    for (uint32_t b = threadIdx.x; b < d.nb; b += blockDim.x) bstart[b] = tot[b];
     if (threadIdx.x == 0) bstart[d.nb] = total;
 }
 
 // off[t][b] = first slot of tile t's run of bucket b.
 __global__ __launch_bounds__(kBBlock) void k_bkt_colprefix(const uint32_t *__restrict__ hist,
-                                                           const uint32_t *__restrict__ S, BucketDev d,
+                                                           const uint32_t *__restrict__ S,
+                                                           const uint32_t *__restrict__ bstart, BucketDev d,
                                                            uint32_t *__restrict__ off) {
     const uint32_t g = blockIdx.x;
     const uint32_t t0 = g * kColGroup, t1 = std::min(t0 + kColGroup, d.ntiles);
     for (uint32_t b = threadIdx.x; b < d.nb; b += blockDim.x) {
-        uint32_t run = S[(uint64_t)g * d.nb + b];
+        uint32_t run = bstart[b] + S[(uint64_t)g * d.nb + b];
         for (uint32_t t = t0; t < t1; ++t) {
             const uint64_t e = (uint64_t)t * d.nb + b;
             off[e] = run;
@@ -409,16 +405,22 @@ __global__ __launch_bounds__(kSearchBlock) void k_bkt_search(TreeView t, BucketD
                 __syncthreads();
                 cur = b;
             }
-            for (uint64_t base = pos + (uint64_t)w * 64; base < seg_end; base += (uint64_t)nw * 64) {
+            // the next batch's queries are loaded while this batch is searched
+            const uint64_t stride = (uint64_t)nw * 64;
+            uint64_t base = pos + (uint64_t)w * 64;
+            K next = base + lane < seg_end ? __builtin_nontemporal_load(&qs[base + lane]) : K(0);
+            for (; base < seg_end; base += stride) {
                 const uint64_t my = base + lane;
                 const bool ok = my < seg_end;
-                const K key = ok ? __builtin_nontemporal_load(&qs[my]) : K(0);
+                const K key = next;
+                next = my + stride < seg_end ? __builtin_nontemporal_load(&qs[my + stride]) : K(0);
                 const uint32_t p = prefix_of<K>(pi, key, kSliceTabBits);
                 const uint32_t at = lower_bound_window<K>(slice, stab[p], stab[p + 1], key);
                 const uint32_t node = (uint32_t)first + (at < eb ? at : eb - 1);
                 V r;
                 // leaves are reused by every query of the bucket: regular (temporal) loads
-                if constexpr (COOP) r = coop_finish64<K, V, F, LF, false, kOpValue, G, OFF32>(t, key, node, top, sentinel);
+                if constexpr (COOP)
+                    r = coop_finish64<K, V, F, LF, false, kOpValue, G, OFF32, true>(t, key, node, top, sentinel);
                 else r = thread_finish<K, V, F, LF>(t, key, node, top, sentinel);
                 if (ok) __builtin_nontemporal_store(r, &res[my]);
             }
@@ -519,7 +521,7 @@ static BucketPlan bucket_plan(const bpt_tree *t, uint64_t nq) {
     d.ngroups = (d.ntiles + kColGroup - 1) / kColGroup;
     d.chunk = t->group_chunk;
     const uint64_t nchunks = std::max<uint64_t>(1, (nq + d.chunk - 1) / d.chunk);
-    const uint64_t cap = (uint64_t)device_cu_count() * 2;
+    const uint64_t cap = (uint64_t)device_cu_count() * t->group_blocks;
     p.grid = (uint32_t)std::min(cap, nchunks);
     // split per XCD only when every XCD gets whole strides of chunks
     d.xcd = (t->group_xcd && p.grid % kXcds == 0 && nchunks >= 2ull * p.grid) ? 1u : 0u;
@@ -576,7 +578,7 @@ static bpt_status bucket_chunk(const bpt_tree *t, const K *q, uint64_t nq, V *ou
     BPT_HIP(hipGetLastError());
     k_bkt_colscan<<<1, kScanThreads, 0, s>>>(cols, d, bstart);
     BPT_HIP(hipGetLastError());
-    k_bkt_colprefix<<<d.ngroups, kBBlock, 0, s>>>(hist, cols, d, offs);
+    k_bkt_colprefix<<<d.ngroups, kBBlock, 0, s>>>(hist, cols, bstart, d, offs);
     BPT_HIP(hipGetLastError());
     k_bkt_scatter<K><<<d.ntiles, kPBlock, p.lds_scatter, s>>>(q, nq, d, bsep, hdr, btab, hist, offs, qs, lidx);
     BPT_HIP(hipGetLastError());

@@ -308,6 +308,11 @@ bpt_status bpt_tree_set_option(bpt_tree *t, const char *key, int64_t value) {
         t->group_lanes = (uint32_t)value;
         return BPT_OK;
     }
+    if (strcmp(key, "group_blocks") == 0) {
+        BPT_CHECK(value >= 1 && value <= 8, BPT_ERR_INVALID, "group_blocks must be in [1, 8]");
+        t->group_blocks = (uint32_t)value;
+        return BPT_OK;
+    }
     if (strcmp(key, "group_xcd") == 0) {
         t->group_xcd = value ? 1 : 0;
         return BPT_OK;

@@ -86,6 +86,7 @@ struct bpt_tree {
     uint32_t group_chunk = 16384;    // grouped search: queries per work chunk
     uint32_t group_xcd = 1;          // grouped search: chunks split per XCD
     uint32_t group_lanes = 4;        // grouped search: lanes per query at the leaf (4 or 8)
+    uint32_t group_blocks = 2;       // grouped search: 512-thread blocks per CU
     bpt::TreeView view{};
     bpt::Workspace ws;             // grouped / learned scratch owned by the tree
     bpt::LearnedModel *learned = nullptr;
@@ -369,6 +370,20 @@ __device__ __forceinline__ uint32_t chunk_count_lt(const Chunk<BYTES> &c, K q) {
     return n;
 }
 
+template <class K, int BYTES>
+__device__ __forceinline__ uint32_t chunk_count_eq(const Chunk<BYTES> &c, K q) {
+    constexpr int N = BYTES / (int)sizeof(K);
+    uint32_t n = 0;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        K v;
+        if constexpr (sizeof(K) == 4) v = c.w[j];
+        else v = (uint64_t)c.w[2 * j] | ((uint64_t)c.w[2 * j + 1] << 32);
+        n += (v == q);
+    }
+    return n;
+}
+
 // Sum over the 8 lanes of a group with DPP lane moves (no LDS traffic):
 // quad_perm [1,0,3,2], quad_perm [2,3,0,1], then row_half_mirror, which
 // pairs lane i with lane 7-i of the same 8-lane half row (the other quad).
@@ -426,7 +441,12 @@ constexpr int kOpLowerBound = 1;
 
 // Phase B alone: the lane's query `myq` sits in node `mynode` of level `top`
 // (top = 0: mynode is its leaf); walks levels top..1 and the leaf.
-template <class K, class V, int F, int LF, bool NT, int OP, int G = 8, bool OFF32 = false>
+// KEYS: at the leaf the G lanes share only its LF keys; the rank comes with
+// the count of equal keys in the same sum, and the lane that owns the query
+// then reads the one value it needs (a value lookup is a hit iff an equal key
+// exists in the leaf at a real slot). Fewer instructions per query than
+// splitting the whole line, for one more dependent load.
+template <class K, class V, int F, int LF, bool NT, int OP, int G = 8, bool OFF32 = false, bool KEYS = false>
 __device__ __forceinline__ typename std::conditional<OP == kOpValue, V, uint64_t>::type
 coop_finish64(const TreeView &t, K myq, uint32_t mynode, int top, V sentinel) {
     static_assert(sizeof(K) == sizeof(V), "cooperative leaf split needs equal key/value widths");
@@ -464,6 +484,64 @@ coop_finish64(const TreeView &t, K myq, uint32_t mynode, int top, V sentinel) {
             node[k] = ch < last ? ch : last;
         }
     }
+    // slot leaf*LF + r holds a key iff leaf < last_leaf or r < last_cnt
+    const uint32_t last_leaf = (uint32_t)(t.count[0] - 1);
+    const uint32_t last_cnt = (uint32_t)(t.n - (uint64_t)last_leaf * LF);
+    constexpr int KB = LF * (int)sizeof(K) / G;  // key bytes per lane when only keys are shared
+    if constexpr (KEYS && KB >= 8) {
+        Chunk<KB> c[G];
+        if constexpr (OFF32) {
+            const __amdgpu_buffer_rsrc_t rs = raw_rsrc(t.leaves);
+#pragma unroll
+            for (int k = 0; k < G; ++k) load_chunk_buf<KB, NT>(rs, node[k] * STRIDE + sub * KB, c[k]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < G; ++k) {
+                const uint8_t *lp = t.leaves + (uint64_t)node[k] * STRIDE + sub * KB;
+                if constexpr (NT) load_chunk_nt<KB>(lp, c[k]);
+                else load_chunk<KB>(lp, c[k]);
+            }
+        }
+        if constexpr (OP == kOpValue) {
+            uint32_t myr = 0, myleaf = 0;
+            bool myhit = false;
+#pragma unroll
+            for (int k = 0; k < G; ++k) {
+                const uint32_t tot =
+                    group_sum<G>(chunk_count_lt<K, KB>(c[k], qk[k]) | (chunk_count_eq<K, KB>(c[k], qk[k]) << 16));
+                const uint32_t r = tot & 0xFFFFu;  // lower bound inside the leaf
+                const bool hit = (tot >> 16) != 0 && (node[k] < last_leaf || r < last_cnt);
+                const bool own = sub == (uint32_t)k;
+                myr = own ? r : myr;
+                myleaf = own ? node[k] : myleaf;
+                myhit = own ? hit : myhit;
+            }
+            V res = sentinel;
+            if (myhit) {
+                const uint64_t off = (uint64_t)myleaf * STRIDE + LF * sizeof(K) + (uint64_t)myr * sizeof(V);
+                if constexpr (OFF32) {
+                    const __amdgpu_buffer_rsrc_t rs = raw_rsrc(t.leaves);
+                    if constexpr (sizeof(V) == 4) res = (V)__builtin_amdgcn_raw_buffer_load_b32(rs, (uint32_t)off, 0, 0);
+                    else {
+                        const u32x2_t v2 = __builtin_amdgcn_raw_buffer_load_b64(rs, (uint32_t)off, 0, 0);
+                        res = (V)((uint64_t)v2.x | ((uint64_t)v2.y << 32));
+                    }
+                } else {
+                    res = *reinterpret_cast<const V *>(t.leaves + off);
+                }
+            }
+            return res;
+        } else {
+            uint64_t res = 0;
+#pragma unroll
+            for (int k = 0; k < G; ++k) {
+                const uint32_t r = group_sum<G>(chunk_count_lt<K, KB>(c[k], qk[k]));
+                const uint64_t p = (uint64_t)node[k] * LF + r;
+                res = sub == (uint32_t)k ? (p < t.n ? p : t.n) : res;
+            }
+            return res;
+        }
+    }
     Chunk<LB> c[G];
     if constexpr (OFF32) {
         const __amdgpu_buffer_rsrc_t rs = raw_rsrc(t.leaves);
@@ -478,9 +556,6 @@ coop_finish64(const TreeView &t, K myq, uint32_t mynode, int top, V sentinel) {
         }
     }
     const uint32_t keymask = sub < (uint32_t)(G / 2) ? ~0u : 0u;  // value lanes count nothing
-    // slot leaf*LF + r holds a key iff leaf < last_leaf or r < last_cnt
-    const uint32_t last_leaf = (uint32_t)(t.count[0] - 1);
-    const uint32_t last_cnt = (uint32_t)(t.n - (uint64_t)last_leaf * LF);
     R res = OP == kOpValue ? (R)sentinel : (R)0;
 #pragma unroll
     for (int k = 0; k < G; ++k) {

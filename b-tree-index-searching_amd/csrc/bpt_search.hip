@@ -39,9 +39,14 @@ __global__ __launch_bounds__(256) void k_point_coop(TreeView t, const K *__restr
     const uint32_t lane = threadIdx.x & 63;
     const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-    for (uint64_t base = wave * 64; base < nq; base += nwaves * 64) {
+    // the next batch's queries are loaded while this batch is searched
+    const uint64_t stride = nwaves * 64;
+    uint64_t base = wave * 64;
+    K next = base + lane < nq ? __builtin_nontemporal_load(&q[base + lane]) : K(0);
+    for (; base < nq; base += stride) {
         const uint64_t my = base + lane;
-        const K myq = my < nq ? __builtin_nontemporal_load(&q[my]) : K(0);
+        const K myq = next;
+        next = my + stride < nq ? __builtin_nontemporal_load(&q[my + stride]) : K(0);
         const V res = coop_lookup64<K, V, F, LF, NT, kOpValue, G, OFF32>(t, lds, myq, sentinel);
         if (my < nq) __builtin_nontemporal_store(res, &out[my]);
     }

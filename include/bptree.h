@@ -117,7 +117,8 @@ bpt_status bpt_tree_set_launch(bpt_tree *t, uint32_t block, uint32_t blocks_per_
  * "addr32" (1 = 32-bit byte offsets into levels and leaves below 4 GiB, the
  * default; 0 = 64-bit addresses throughout), and for the grouped mode
  * "group_chunk" (queries per search work chunk, default 16384),
- * "group_lanes" (lanes per query at the leaf: 4, the default, or 8) and
+ * "group_lanes" (lanes per query at the leaf: 4, the default, or 8),
+ * "group_blocks" (search blocks of 512 threads per CU, default 2) and
  * "group_xcd" (1 = each XCD searches its own contiguous part of the
  * bucket-ordered queries, the default). */
 bpt_status bpt_tree_set_option(bpt_tree *t, const char *key, int64_t value);

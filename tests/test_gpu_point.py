@@ -95,6 +95,39 @@ def test_grouped_bits_and_launch_shapes(bpt, oracle, bits):
         assert np.array_equal(t.point_search(q, bpt.PLAIN), want)
 
 
+@pytest.mark.parametrize("kb,n,bits", [(4, 400000, 1), (8, 1200000, 1), (8, 300000, 4), (4, 6000000, 2)])
+def test_grouped_slice_levels(bpt, oracle, kb, n, bits):
+    """Few buckets allowed: the LDS slice moves up to level 2 or 3 and the
+    search reads the levels below it from global memory."""
+    keys, vals, q = _case(n, kb, kb, seed=n + bits)
+    t = bpt.Tree(keys, vals, kb, kb)
+    t.set_group_bits(bits)
+    want = _expect(oracle, keys, vals, q, kb)
+    for lanes in (4, 8):
+        t.set_option("group_lanes", lanes)
+        assert np.array_equal(t.point_search(q, bpt.GROUPED), want), lanes
+    t.set_option("coop", 0)
+    assert np.array_equal(t.point_search(q, bpt.GROUPED), want)
+
+
+def test_grouped_skewed_queries(bpt, oracle):
+    """Queries piled into a few buckets, below the smallest and above the
+    largest key, and repeated: runs of very uneven length."""
+    rng = np.random.default_rng(5)
+    keys = np.unique(rng.integers(1000, 1 << 31, size=2_000_000, dtype=np.uint64)).astype(np.uint32)
+    vals = np.arange(len(keys), dtype=np.uint32)
+    t = bpt.Tree(keys, vals)
+    hot = keys[rng.integers(100000, 100400, size=900_000)]
+    q = np.concatenate([hot, np.full(50_000, 7, np.uint32), np.full(50_000, 0xFFFFFFF0, np.uint32),
+                        rng.integers(0, 1 << 32, size=100_000, dtype=np.uint64).astype(np.uint32)])
+    rng.shuffle(q)
+    want = _expect(oracle, keys, vals, q, 4)
+    for chunk, xcd in ((16384, 1), (256, 1), (4096, 0)):
+        t.set_option("group_chunk", chunk)
+        t.set_option("group_xcd", xcd)
+        assert np.array_equal(t.point_search(q, bpt.GROUPED), want), (chunk, xcd)
+
+
 def test_workload_c1_scaled_and_page_tree(bpt, oracle, wl):
     """1M uint32 keys, order-32 page tree of the reference restated on the
     CPU, against the flat GPU tree, on the CPU workload's query mix."""
