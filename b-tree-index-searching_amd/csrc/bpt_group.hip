@@ -23,6 +23,23 @@ bpt_status range_count_ordered(const bpt_tree *t, const void *d_lo, const void *
 
 static inline uint64_t align256(uint64_t b) { return (b + 255) / 256 * 256; }
 
+// First position of every digit value b in [0, nb] of keys sorted on the
+// digit (key >> shift) & mask; position nb is n.
+template <class KT>
+__global__ void k_digit_starts(const KT *__restrict__ ks, uint64_t n, uint32_t shift, uint32_t mask, uint32_t nb,
+                               uint64_t *__restrict__ start) {
+    for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b <= nb; b += gridDim.x * blockDim.x) {
+        uint64_t lo = 0, hi = n;
+        if (b < nb) {
+            while (lo < hi) {
+                const uint64_t mid = lo + (hi - lo) / 2;
+                if (((uint32_t)(ks[mid] >> shift) & mask) < b) lo = mid + 1; else hi = mid;
+            }
+        }
+        start[b] = b < nb ? lo : n;
+    }
+}
+
 // ==================================================================== gather
 template <class V>
 __global__ __launch_bounds__(256) void k_gather(const V *__restrict__ src, const uint32_t *__restrict__ idx,
@@ -363,41 +380,33 @@ bpt_status bpt_radix_group(const void *d_keys, uint32_t key_bytes, uint64_t n, u
     BPT_HIP(hipMalloc(&ws, ws_bytes));
     bpt_status st;
     uint64_t *bucket_keys_u64 = nullptr;
+    uint32_t *sorted_u32 = nullptr;
     uint32_t *perm = nullptr;
     if (key_bytes == 4) {
         uint32_t *ks = nullptr;
         st = sort_pairs<uint32_t>((const uint32_t *)d_keys, nullptr, n, shift, shift + bits, ws, ws_bytes, &ks,
                                   &perm, s);
+        sorted_u32 = ks;
     } else {
         uint64_t *ks = nullptr;
         st = sort_pairs<uint64_t>((const uint64_t *)d_keys, nullptr, n, shift, shift + bits, ws, ws_bytes, &ks,
                                   &perm, s);
         bucket_keys_u64 = ks;
     }
-    (void)bucket_keys_u64;
     if (st == BPT_OK) {
         hipError_t e = hipMemcpyAsync(d_perm, perm, n * 4, hipMemcpyDeviceToDevice, s);
         if (e != hipSuccess) st = hip_fail(e, "perm copy", __FILE__, __LINE__);
     }
     if (st == BPT_OK && d_bucket_start) {
-        // bucket starts = lower bound of each bucket in the sorted digit sequence
-        std::vector<uint64_t> h(nb + 1, 0);
-        std::vector<uint8_t> keys_host;
-        keys_host.resize(n * key_bytes);
-        hipError_t e = hipMemcpyAsync(keys_host.data(), d_keys, n * key_bytes, hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
-        if (e != hipSuccess) st = hip_fail(e, "bucket count copy", __FILE__, __LINE__);
-        if (st == BPT_OK) {
-            for (uint64_t i = 0; i < n; ++i) {
-                uint64_t k = key_bytes == 4 ? ((const uint32_t *)keys_host.data())[i]
-                                            : ((const uint64_t *)keys_host.data())[i];
-                h[((k >> shift) & (nb - 1)) + 1]++;
-            }
-            for (uint32_t b = 0; b < nb; ++b) h[b + 1] += h[b];
-            e = hipMemcpyAsync(d_bucket_start, h.data(), (nb + 1) * 8ull, hipMemcpyHostToDevice, s);
-            if (e == hipSuccess) e = hipStreamSynchronize(s);
-            if (e != hipSuccess) st = hip_fail(e, "bucket start copy", __FILE__, __LINE__);
+        // bucket starts = lower bound of every bucket in the sorted digit sequence
+        const uint32_t g = (nb + 1 + 255) / 256;
+        if (key_bytes == 4) {
+            k_digit_starts<uint32_t><<<g, 256, 0, s>>>(sorted_u32, n, shift, nb - 1, nb, d_bucket_start);
+        } else {
+            k_digit_starts<uint64_t><<<g, 256, 0, s>>>(bucket_keys_u64, n, shift, nb - 1, nb, d_bucket_start);
         }
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) st = hip_fail(e, "k_digit_starts", __FILE__, __LINE__);
     }
     (void)hipStreamSynchronize(s);
     (void)hipFree(ws);
