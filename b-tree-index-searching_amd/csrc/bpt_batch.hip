@@ -85,8 +85,9 @@ __global__ __launch_bounds__(256) void k_forest_leaves(const uint64_t *__restric
         const uint64_t k = local < n ? keys[key_off[t] + local] : ~0ull;
         const uint64_t v = local < n ? vals[key_off[t] + local] : 0ull;
         uint8_t *lp = leaves + (s / LF) * 128;  // a leaf is one 128-byte line
-        reinterpret_cast<uint64_t *>(lp)[s % LF] = k;
-        reinterpret_cast<uint64_t *>(lp + 64)[s % LF] = v;
+        using Fmt = LeafFmt<uint64_t, uint64_t, LF>;
+        *reinterpret_cast<uint64_t *>(lp + Fmt::key_off((uint32_t)(s % LF))) = k;
+        *reinterpret_cast<uint64_t *>(lp + Fmt::val_off((uint32_t)(s % LF))) = v;
     }
 }
 

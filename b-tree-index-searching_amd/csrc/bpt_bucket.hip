@@ -292,6 +292,15 @@ __global__ __launch_bounds__(kPBlock) void k_bkt_scatter(const K *__restrict__ q
     uint16_t *sb = sl + d.tile;
     uint16_t *btab = sb + d.tile;
     const uint32_t tile = xcd_contiguous(blockIdx.x, gridDim.x);
+    constexpr int E = per_thread<K>();
+    const uint64_t lo = (uint64_t)tile * d.tile;
+    const uint32_t cnt = (uint32_t)(std::min<uint64_t>(lo + d.tile, nq) - lo);
+    K key[E];  // issued first: their latency overlaps the staging and the scan
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const uint32_t i = e * kPBlock + threadIdx.x;
+        key[e] = i < cnt ? __builtin_nontemporal_load(&q[lo + i]) : K(0);
+    }
     for (uint32_t b = threadIdx.x; b < d.nb; b += blockDim.x) {
         bsep[b] = bsep_g[b];
         const uint64_t e = (uint64_t)tile * d.nb + b;
@@ -304,15 +313,6 @@ __global__ __launch_bounds__(kPBlock) void k_bkt_scatter(const K *__restrict__ q
     block_exclusive_scan(lstart, d.nb, wsum);
     for (uint32_t b = threadIdx.x; b < d.nb; b += blockDim.x) lcur[b] = lstart[b];
     __syncthreads();
-    constexpr int E = per_thread<K>();
-    const uint64_t lo = (uint64_t)tile * d.tile;
-    const uint32_t cnt = (uint32_t)(std::min<uint64_t>(lo + d.tile, nq) - lo);
-    K key[E];
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-        const uint32_t i = e * kPBlock + threadIdx.x;
-        key[e] = i < cnt ? __builtin_nontemporal_load(&q[lo + i]) : K(0);
-    }
     uint32_t bk[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) bk[e] = bucket_of<K>(bsep, d.nb, btab, pi, key[e]);
@@ -420,7 +420,7 @@ __global__ __launch_bounds__(kSearchBlock) void k_bkt_search(TreeView t, BucketD
                 V r;
                 // leaves are reused by every query of the bucket: regular (temporal) loads
                 if constexpr (COOP)
-                    r = coop_finish64<K, V, F, LF, false, kOpValue, G, OFF32, true>(t, key, node, top, sentinel);
+                    r = coop_finish64<K, V, F, LF, false, kOpValue, G, OFF32>(t, key, node, top, sentinel);
                 else r = thread_finish<K, V, F, LF>(t, key, node, top, sentinel);
                 if (ok) __builtin_nontemporal_store(r, &res[my]);
             }

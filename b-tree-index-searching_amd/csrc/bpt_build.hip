@@ -32,8 +32,8 @@ __global__ __launch_bounds__(256) void k_build_leaves(const K *__restrict__ keys
         K k = s < n ? keys[s] : key_max<K>();
         V v = s < n ? vals[s] : V(0);
         uint8_t *lp = leaves + leaf * stride;
-        reinterpret_cast<K *>(lp)[j] = k;
-        reinterpret_cast<V *>(lp + LF * sizeof(K))[j] = v;
+        *reinterpret_cast<K *>(lp + LeafFmt<K, V, LF>::key_off(j)) = k;
+        *reinterpret_cast<V *>(lp + LeafFmt<K, V, LF>::val_off(j)) = v;
     }
 }
 

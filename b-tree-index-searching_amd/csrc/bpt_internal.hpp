@@ -175,35 +175,80 @@ __device__ __forceinline__ uint64_t descend(const TreeView &t, const uint8_t *ld
     return node;
 }
 
-// Loads the keys of a leaf into registers.
-template <class K, int LF>
-__device__ __forceinline__ void load_leaf_keys(const uint8_t *leaf, K (&k)[LF]) {
-    constexpr int NV = LF * (int)sizeof(K) / 16;
-    const uint4 *p = reinterpret_cast<const uint4 *>(leaf);
-#pragma unroll
-    for (int v = 0; v < NV; ++v) {
-        uint4 w = p[v];
-        if constexpr (sizeof(K) == 4) {
-            k[4 * v] = w.x; k[4 * v + 1] = w.y; k[4 * v + 2] = w.z; k[4 * v + 3] = w.w;
-        } else {
-            k[2 * v] = (uint64_t)w.x | ((uint64_t)w.y << 32);
-            k[2 * v + 1] = (uint64_t)w.z | ((uint64_t)w.w << 32);
-        }
+// ---------------------------------------------------------------- leaf format
+// A leaf holds LF (key, value) slots. With equal key and value widths the
+// slots are interleaved in 16-byte pieces -- [k0 k1 v0 v1] for 4-byte slots,
+// [k0 v0] for 8-byte slots -- so every 16-byte piece holds whole slots and a
+// lane that loads one piece can answer for its slots alone. Otherwise (8-byte
+// keys, 4-byte values) the LF keys come first, then the LF values.
+template <class K, class V, int LF>
+struct LeafFmt {
+    static constexpr bool kInter = sizeof(K) == sizeof(V);
+    static constexpr uint32_t kPair = kInter ? 16u / (2u * (uint32_t)sizeof(K)) : 1u;  // slots per piece
+    __host__ __device__ static constexpr uint32_t key_off(uint32_t j) {
+        return kInter ? (j / kPair) * 16u + (j % kPair) * (uint32_t)sizeof(K) : j * (uint32_t)sizeof(K);
     }
+    __host__ __device__ static constexpr uint32_t val_off(uint32_t j) {
+        return kInter ? (j / kPair) * 16u + kPair * (uint32_t)sizeof(K) + (j % kPair) * (uint32_t)sizeof(V)
+                      : LF * (uint32_t)sizeof(K) + j * (uint32_t)sizeof(V);
+    }
+};
+
+// Key and value of slot i of a run of whole 16-byte pieces held in 32-bit
+// words (interleaved format).
+template <class K>
+__device__ __forceinline__ K piece_key(const uint32_t *w, uint32_t i) {
+    if constexpr (sizeof(K) == 4) return w[(i / 2) * 4 + (i % 2)];
+    else return (uint64_t)w[i * 4] | ((uint64_t)w[i * 4 + 1] << 32);
+}
+template <class V>
+__device__ __forceinline__ V piece_val(const uint32_t *w, uint32_t i) {
+    if constexpr (sizeof(V) == 4) return w[(i / 2) * 4 + 2 + (i % 2)];
+    else return (uint64_t)w[i * 4 + 2] | ((uint64_t)w[i * 4 + 3] << 32);
 }
 
-template <class V, int LF>
-__device__ __forceinline__ void load_leaf_vals(const uint8_t *vals, V (&v)[LF]) {
-    constexpr int NV = LF * (int)sizeof(V) / 16;
-    const uint4 *p = reinterpret_cast<const uint4 *>(vals);
+// All keys (and values) of a leaf in registers, one lane.
+template <class K, class V, int LF>
+__device__ __forceinline__ void load_leaf(const uint8_t *lp, K (&k)[LF], V *v) {
+    using Fmt = LeafFmt<K, V, LF>;
+    if constexpr (Fmt::kInter) {
+        constexpr int NP = LF * 2 * (int)sizeof(K) / 16;
+        uint32_t w[NP * 4];
+        const uint4 *p4 = reinterpret_cast<const uint4 *>(lp);
 #pragma unroll
-    for (int i = 0; i < NV; ++i) {
-        uint4 w = p[i];
-        if constexpr (sizeof(V) == 4) {
-            v[4 * i] = w.x; v[4 * i + 1] = w.y; v[4 * i + 2] = w.z; v[4 * i + 3] = w.w;
-        } else {
-            v[2 * i] = (uint64_t)w.x | ((uint64_t)w.y << 32);
-            v[2 * i + 1] = (uint64_t)w.z | ((uint64_t)w.w << 32);
+        for (int i = 0; i < NP; ++i) {
+            const uint4 x = p4[i];
+            w[4 * i] = x.x; w[4 * i + 1] = x.y; w[4 * i + 2] = x.z; w[4 * i + 3] = x.w;
+        }
+#pragma unroll
+        for (int j = 0; j < LF; ++j) {
+            k[j] = piece_key<K>(w, j);
+            if (v) v[j] = piece_val<V>(w, j);
+        }
+    } else {
+        const uint4 *p4 = reinterpret_cast<const uint4 *>(lp);
+#pragma unroll
+        for (int i = 0; i < LF * (int)sizeof(K) / 16; ++i) {
+            const uint4 x = p4[i];
+            if constexpr (sizeof(K) == 4) {
+                k[4 * i] = x.x; k[4 * i + 1] = x.y; k[4 * i + 2] = x.z; k[4 * i + 3] = x.w;
+            } else {
+                k[2 * i] = (uint64_t)x.x | ((uint64_t)x.y << 32);
+                k[2 * i + 1] = (uint64_t)x.z | ((uint64_t)x.w << 32);
+            }
+        }
+        if (v) {
+            const uint4 *q4 = reinterpret_cast<const uint4 *>(lp + LF * sizeof(K));
+#pragma unroll
+            for (int i = 0; i < LF * (int)sizeof(V) / 16; ++i) {
+                const uint4 x = q4[i];
+                if constexpr (sizeof(V) == 4) {
+                    v[4 * i] = x.x; v[4 * i + 1] = x.y; v[4 * i + 2] = x.z; v[4 * i + 3] = x.w;
+                } else {
+                    v[2 * i] = (uint64_t)x.x | ((uint64_t)x.y << 32);
+                    v[2 * i + 1] = (uint64_t)x.z | ((uint64_t)x.w << 32);
+                }
+            }
         }
     }
 }
@@ -212,11 +257,9 @@ __device__ __forceinline__ void load_leaf_vals(const uint8_t *vals, V (&v)[LF]) 
 // (slots past n never match), else `sentinel`.
 template <class K, class V, int LF>
 __device__ __forceinline__ V leaf_lookup(const TreeView &t, uint64_t leaf, K q, V sentinel) {
-    const uint8_t *lp = t.leaves + leaf * t.leaf_stride;
     K k[LF];
     V v[LF];
-    load_leaf_keys<K, LF>(lp, k);
-    load_leaf_vals<V, LF>(lp + LF * sizeof(K), v);
+    load_leaf<K, V, LF>(t.leaves + leaf * t.leaf_stride, k, v);
     const uint64_t base = leaf * LF;
     V res = sentinel;
 #pragma unroll
@@ -239,11 +282,10 @@ __device__ __forceinline__ V thread_finish(const TreeView &t, K q, uint64_t node
 }
 
 // Global lower_bound position of q inside leaf `leaf`.
-template <class K, int LF>
+template <class K, class V, int LF>
 __device__ __forceinline__ uint64_t leaf_lower_bound(const TreeView &t, uint64_t leaf, K q) {
-    const uint8_t *lp = t.leaves + leaf * t.leaf_stride;
     K k[LF];
-    load_leaf_keys<K, LF>(lp, k);
+    load_leaf<K, V, LF>(t.leaves + leaf * t.leaf_stride, k, (V *)nullptr);
     uint32_t c = 0;
 #pragma unroll
     for (int j = 0; j < LF; ++j) c += (k[j] < q);
@@ -252,14 +294,15 @@ __device__ __forceinline__ uint64_t leaf_lower_bound(const TreeView &t, uint64_t
 }
 
 // Slot i of the sorted key / value sequence, read from the leaf layout.
-template <class K, int LF>
+template <class K, class V, int LF>
 __device__ __forceinline__ K slot_key(const TreeView &t, uint64_t i) {
-    return *reinterpret_cast<const K *>(t.leaves + (i / LF) * t.leaf_stride + (i % LF) * sizeof(K));
+    return *reinterpret_cast<const K *>(t.leaves + (i / LF) * t.leaf_stride +
+                                        LeafFmt<K, V, LF>::key_off((uint32_t)(i % LF)));
 }
 template <class K, class V, int LF>
 __device__ __forceinline__ V slot_val(const TreeView &t, uint64_t i) {
-    return *reinterpret_cast<const V *>(t.leaves + (i / LF) * t.leaf_stride + LF * sizeof(K) +
-                                        (i % LF) * sizeof(V));
+    return *reinterpret_cast<const V *>(t.leaves + (i / LF) * t.leaf_stride +
+                                        LeafFmt<K, V, LF>::val_off((uint32_t)(i % LF)));
 }
 
 // Copies the LDS prefix of the internal levels into shared memory.
@@ -429,9 +472,9 @@ __device__ __forceinline__ uint32_t lower_bound_flat(const K *__restrict__ a, ui
 // Phase B (G lanes per query, G = 8 or 4): group g (lanes Gg..Gg+G-1) walks
 // queries Gg..Gg+G-1 through the global levels and the leaf, all G in flight
 // at each level; lane s of the group reads bytes [s*NB, (s+1)*NB) of a node,
-// so one load instruction fetches 64/G node pieces of whole lines. Leaf: the
-// first G/2 lanes hold the keys, the others the values (sizeof(K) ==
-// sizeof(V)). Every step is branch-free so the G queries interleave.
+// so one load instruction fetches 64/G node pieces of whole lines; the leaf
+// step is coop_leaf below. Every step is branch-free so the G queries
+// interleave.
 // OFF32: node and leaf byte offsets fit 32 bits (levels and leaves < 4 GiB),
 // so addresses are a uniform base plus a 32-bit lane offset.
 // OP kOpValue returns the value of the first key equal to the lane's query
@@ -439,25 +482,88 @@ __device__ __forceinline__ uint32_t lower_bound_flat(const K *__restrict__ a, ui
 constexpr int kOpValue = 0;
 constexpr int kOpLowerBound = 1;
 
+// Leaf step of the cooperative lookup. Group g (lanes Gg..Gg+G-1) holds the
+// queries qk[0..G) of its lanes and their leaves node[0..G). Lane s of the
+// group loads bytes [s*LB, (s+1)*LB) of each leaf -- whole (key, value) slots
+// in the interleaved format -- so one load instruction fetches 64/G leaf
+// pieces of whole lines, and the G loads of a query's leaf cover it once.
+// The rank r of the query in its leaf is the DPP sum of the lanes' counts; the
+// lane whose slots contain r checks the key there and hands its value (or the
+// sentinel) to the query's lane with one ds_bpermute. Returns the result of
+// the lane's own query.
+template <class K, class V, int LF, bool NT, int OP, int G, bool OFF32>
+__device__ __forceinline__ typename std::conditional<OP == kOpValue, V, uint64_t>::type
+coop_leaf(const TreeView &t, const K (&qk)[G], const uint32_t (&node)[G], V sentinel) {
+    static_assert(LeafFmt<K, V, LF>::kInter, "cooperative leaf needs the interleaved format");
+    constexpr uint32_t STRIDE = 2 * LF * sizeof(K);
+    constexpr int GL = (int)(STRIDE / 16) < G ? (int)(STRIDE / 16) : G;  // lanes holding leaf pieces
+    constexpr int LB = (int)STRIDE / GL;               // leaf bytes per such lane
+    constexpr uint32_t PPL = LB / (2 * sizeof(K));     // slots per such lane
+    static_assert(LB >= 16 && LB % 16 == 0, "whole 16-byte pieces per lane");
+    using R = typename std::conditional<OP == kOpValue, V, uint64_t>::type;
+    const uint32_t lane = threadIdx.x & 63, g = lane / G, sub = lane % G;
+    const bool holds = sub < (uint32_t)GL;
+    Chunk<LB> c[G];
+#pragma unroll
+    for (int k = 0; k < G; ++k)
+#pragma unroll
+        for (int i = 0; i < LB / 4; ++i) c[k].w[i] = ~0u;  // lanes past GL hold no slots
+    if (holds) {
+        if constexpr (OFF32) {
+            const __amdgpu_buffer_rsrc_t rs = raw_rsrc(t.leaves);
+#pragma unroll
+            for (int k = 0; k < G; ++k) load_chunk_buf<LB, NT>(rs, node[k] * STRIDE + sub * LB, c[k]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < G; ++k) {
+                const uint8_t *lp = t.leaves + (uint64_t)node[k] * STRIDE + sub * LB;
+                if constexpr (NT) load_chunk_nt<LB>(lp, c[k]);
+                else load_chunk<LB>(lp, c[k]);
+            }
+        }
+    }
+    // slot leaf*LF + r holds a key iff leaf < last_leaf or r < last_cnt
+    const uint32_t last_leaf = (uint32_t)(t.count[0] - 1);
+    const uint32_t last_cnt = (uint32_t)(t.n - (uint64_t)last_leaf * LF);
+    R res = OP == kOpValue ? (R)sentinel : (R)0;
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+        uint32_t cnt = 0;
+#pragma unroll
+        for (uint32_t i = 0; i < PPL; ++i) cnt += (piece_key<K>(c[k].w, i) < qk[k]);
+        const uint32_t r = group_sum<G>(holds ? cnt : 0u);  // lower bound inside the leaf, 0..LF
+        if constexpr (OP == kOpValue) {
+            const uint32_t local = r - sub * PPL;  // r's slot in this lane, if < PPL
+            K key = piece_key<K>(c[k].w, 0);
+            V val = piece_val<V>(c[k].w, 0);
+#pragma unroll
+            for (uint32_t i = 1; i < PPL; ++i) {
+                key = local == i ? piece_key<K>(c[k].w, i) : key;
+                val = local == i ? piece_val<V>(c[k].w, i) : val;
+            }
+            const bool hit = holds && local < PPL && key == qk[k] && (node[k] < last_leaf || r < last_cnt);
+            const V cand = hit ? val : sentinel;
+            const uint32_t owner = r / PPL < (uint32_t)GL ? r / PPL : (uint32_t)GL - 1;  // r = LF: sentinel
+            const V got = __shfl(cand, (int)(g * G + owner), 64);
+            res = sub == (uint32_t)k ? got : res;
+        } else {
+            const uint64_t p = (uint64_t)node[k] * LF + r;
+            res = sub == (uint32_t)k ? (p < t.n ? p : t.n) : res;
+        }
+    }
+    return res;
+}
+
 // Phase B alone: the lane's query `myq` sits in node `mynode` of level `top`
 // (top = 0: mynode is its leaf); walks levels top..1 and the leaf.
-// KEYS: at the leaf the G lanes share only its LF keys; the rank comes with
-// the count of equal keys in the same sum, and the lane that owns the query
-// then reads the one value it needs (a value lookup is a hit iff an equal key
-// exists in the leaf at a real slot). Fewer instructions per query than
-// splitting the whole line, for one more dependent load.
-template <class K, class V, int F, int LF, bool NT, int OP, int G = 8, bool OFF32 = false, bool KEYS = false>
+template <class K, class V, int F, int LF, bool NT, int OP, int G = 8, bool OFF32 = false>
 __device__ __forceinline__ typename std::conditional<OP == kOpValue, V, uint64_t>::type
 coop_finish64(const TreeView &t, K myq, uint32_t mynode, int top, V sentinel) {
-    static_assert(sizeof(K) == sizeof(V), "cooperative leaf split needs equal key/value widths");
+    static_assert(sizeof(K) == sizeof(V), "cooperative lookup needs equal key/value widths");
     static_assert(G == 4 || G == 8, "4 or 8 lanes per query");
-    constexpr int NB = F * (int)sizeof(K) / G;       // node bytes per lane
-    constexpr int LB = 2 * LF * (int)sizeof(K) / G;  // leaf bytes per lane (keys in lanes 0..G/2-1)
-    constexpr int KPC = LB / (int)sizeof(K);         // keys per leaf chunk
+    constexpr int NB = F * (int)sizeof(K) / G;  // node bytes per lane
     constexpr uint32_t NODE = F * sizeof(K);
-    constexpr uint32_t STRIDE = 2 * LF * sizeof(K);  // leaf line: LF keys then LF values
-    static_assert(NB >= 8 && LB >= 8, "chunks of at least 8 bytes");
-    using R = typename std::conditional<OP == kOpValue, V, uint64_t>::type;
+    static_assert(NB >= 8, "chunks of at least 8 bytes");
     const uint32_t lane = threadIdx.x & 63, g = lane / G, sub = lane % G;
     K qk[G];
     uint32_t node[G];
@@ -484,98 +590,7 @@ coop_finish64(const TreeView &t, K myq, uint32_t mynode, int top, V sentinel) {
             node[k] = ch < last ? ch : last;
         }
     }
-    // slot leaf*LF + r holds a key iff leaf < last_leaf or r < last_cnt
-    const uint32_t last_leaf = (uint32_t)(t.count[0] - 1);
-    const uint32_t last_cnt = (uint32_t)(t.n - (uint64_t)last_leaf * LF);
-    constexpr int KB = LF * (int)sizeof(K) / G;  // key bytes per lane when only keys are shared
-    if constexpr (KEYS && KB >= 8) {
-        Chunk<KB> c[G];
-        if constexpr (OFF32) {
-            const __amdgpu_buffer_rsrc_t rs = raw_rsrc(t.leaves);
-#pragma unroll
-            for (int k = 0; k < G; ++k) load_chunk_buf<KB, NT>(rs, node[k] * STRIDE + sub * KB, c[k]);
-        } else {
-#pragma unroll
-            for (int k = 0; k < G; ++k) {
-                const uint8_t *lp = t.leaves + (uint64_t)node[k] * STRIDE + sub * KB;
-                if constexpr (NT) load_chunk_nt<KB>(lp, c[k]);
-                else load_chunk<KB>(lp, c[k]);
-            }
-        }
-        if constexpr (OP == kOpValue) {
-            uint32_t myr = 0, myleaf = 0;
-            bool myhit = false;
-#pragma unroll
-            for (int k = 0; k < G; ++k) {
-                const uint32_t tot =
-                    group_sum<G>(chunk_count_lt<K, KB>(c[k], qk[k]) | (chunk_count_eq<K, KB>(c[k], qk[k]) << 16));
-                const uint32_t r = tot & 0xFFFFu;  // lower bound inside the leaf
-                const bool hit = (tot >> 16) != 0 && (node[k] < last_leaf || r < last_cnt);
-                const bool own = sub == (uint32_t)k;
-                myr = own ? r : myr;
-                myleaf = own ? node[k] : myleaf;
-                myhit = own ? hit : myhit;
-            }
-            V res = sentinel;
-            if (myhit) {
-                const uint64_t off = (uint64_t)myleaf * STRIDE + LF * sizeof(K) + (uint64_t)myr * sizeof(V);
-                if constexpr (OFF32) {
-                    const __amdgpu_buffer_rsrc_t rs = raw_rsrc(t.leaves);
-                    if constexpr (sizeof(V) == 4) res = (V)__builtin_amdgcn_raw_buffer_load_b32(rs, (uint32_t)off, 0, 0);
-                    else {
-                        const u32x2_t v2 = __builtin_amdgcn_raw_buffer_load_b64(rs, (uint32_t)off, 0, 0);
-                        res = (V)((uint64_t)v2.x | ((uint64_t)v2.y << 32));
-                    }
-                } else {
-                    res = *reinterpret_cast<const V *>(t.leaves + off);
-                }
-            }
-            return res;
-        } else {
-            uint64_t res = 0;
-#pragma unroll
-            for (int k = 0; k < G; ++k) {
-                const uint32_t r = group_sum<G>(chunk_count_lt<K, KB>(c[k], qk[k]));
-                const uint64_t p = (uint64_t)node[k] * LF + r;
-                res = sub == (uint32_t)k ? (p < t.n ? p : t.n) : res;
-            }
-            return res;
-        }
-    }
-    Chunk<LB> c[G];
-    if constexpr (OFF32) {
-        const __amdgpu_buffer_rsrc_t rs = raw_rsrc(t.leaves);
-#pragma unroll
-        for (int k = 0; k < G; ++k) load_chunk_buf<LB, NT>(rs, node[k] * STRIDE + sub * LB, c[k]);
-    } else {
-#pragma unroll
-        for (int k = 0; k < G; ++k) {
-            const uint8_t *lp = t.leaves + (uint64_t)node[k] * STRIDE + sub * LB;
-            if constexpr (NT) load_chunk_nt<LB>(lp, c[k]);
-            else load_chunk<LB>(lp, c[k]);
-        }
-    }
-    const uint32_t keymask = sub < (uint32_t)(G / 2) ? ~0u : 0u;  // value lanes count nothing
-    R res = OP == kOpValue ? (R)sentinel : (R)0;
-#pragma unroll
-    for (int k = 0; k < G; ++k) {
-        const uint32_t r = group_sum<G>(chunk_count_lt<K, LB>(c[k], qk[k]) & keymask);  // 0..LF
-        if constexpr (OP == kOpValue) {
-            // slot r sits at element r % KPC of key lane r / KPC and of value lane r / KPC + G/2
-            const K e = chunk_elem<K, LB>(c[k], r);
-            const uint32_t src = g * G + (r / KPC);
-            const K key = __shfl(e, (int)src, 64);
-            const V val = __shfl((V)e, (int)(src + G / 2), 64);
-            const bool hit = r < LF && key == qk[k] && (node[k] < last_leaf || r < last_cnt);
-            const V mine = hit ? val : sentinel;
-            res = sub == (uint32_t)k ? mine : res;
-        } else {
-            const uint64_t p = (uint64_t)node[k] * LF + r;
-            const uint64_t mine = p < t.n ? p : t.n;
-            res = sub == (uint32_t)k ? mine : res;
-        }
-    }
-    return res;
+    return coop_leaf<K, V, LF, NT, OP, G, OFF32>(t, qk, node, sentinel);
 }
 
 // Phase A (LDS levels, one lane per query) then phase B.

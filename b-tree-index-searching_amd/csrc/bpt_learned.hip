@@ -260,7 +260,7 @@ static size_t learned_lds(const LearnedView &v) {
 }
 
 
-template <class K, int LF>
+template <class K, class V, int LF>
 __global__ __launch_bounds__(256) void k_route_keys(TreeView t, LearnedView v, uint32_t *__restrict__ routes,
                                                     uint32_t *__restrict__ pmin, uint32_t *__restrict__ pmax,
                                                     uint32_t *__restrict__ cnt) {
@@ -268,7 +268,7 @@ __global__ __launch_bounds__(256) void k_route_keys(TreeView t, LearnedView v, u
     stage_learned(v, sh_l);
     const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < t.n; i += step) {
-        const uint32_t j = route<K>(v, sh_l, slot_key<K, LF>(t, i));
+        const uint32_t j = route<K>(v, sh_l, slot_key<K, V, LF>(t, i));
         routes[i] = j;
         atomicMin(&pmin[j], (uint32_t)i);
         atomicMax(&pmax[j], (uint32_t)i);
@@ -276,7 +276,7 @@ __global__ __launch_bounds__(256) void k_route_keys(TreeView t, LearnedView v, u
     }
 }
 
-template <class K, int LF>
+template <class K, class V, int LF>
 __global__ void k_fit_models(TreeView t, uint32_t nmodels, const uint32_t *__restrict__ pmin,
                              const uint32_t *__restrict__ pmax, const uint32_t *__restrict__ cnt,
                              LeafModel *__restrict__ models) {
@@ -288,8 +288,8 @@ __global__ void k_fit_models(TreeView t, uint32_t nmodels, const uint32_t *__res
         m.flags = kModelEmpty;
     } else {
         const uint64_t a = pmin[j], b = pmax[j];
-        m.k0 = (uint64_t)slot_key<K, LF>(t, a);
-        m.k1 = (uint64_t)slot_key<K, LF>(t, b);
+        m.k0 = (uint64_t)slot_key<K, V, LF>(t, a);
+        m.k1 = (uint64_t)slot_key<K, V, LF>(t, b);
         m.p0 = (double)a;
         m.slope = m.k1 > m.k0 ? (double)(b - a) / (double)(m.k1 - m.k0) : 0.0;
         m.elo = 0x7fffffff;
@@ -298,12 +298,12 @@ __global__ void k_fit_models(TreeView t, uint32_t nmodels, const uint32_t *__res
     models[j] = m;
 }
 
-template <class K, int LF>
+template <class K, class V, int LF>
 __global__ void k_model_errors(TreeView t, const uint32_t *__restrict__ routes, LeafModel *__restrict__ models) {
     const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < t.n; i += step) {
         LeafModel &m = models[routes[i]];
-        const int64_t e = (int64_t)i - model_pred(m, (uint64_t)slot_key<K, LF>(t, i));
+        const int64_t e = (int64_t)i - model_pred(m, (uint64_t)slot_key<K, V, LF>(t, i));
         const int32_t ec = e > 0x3fffffff ? 0x3fffffff : (e < -0x3fffffff ? -0x3fffffff : (int32_t)e);
         atomicMin(&m.elo, ec);
         atomicMax(&m.ehi, ec);
@@ -354,11 +354,11 @@ __global__ __launch_bounds__(256) void k_point_learned(TreeView t, LearnedView v
                 uint64_t base = (uint64_t)lo, len = (uint64_t)(hi - lo + 1);
                 while (len > 1) {
                     const uint64_t half = len >> 1;
-                    base = (slot_key<K, LF>(t, base + half) < key) ? base + half : base;
+                    base = (slot_key<K, V, LF>(t, base + half) < key) ? base + half : base;
                     len -= half;
                 }
-                const uint64_t p = base + (slot_key<K, LF>(t, base) < key);
-                if (p <= (uint64_t)hi && slot_key<K, LF>(t, p) == key) res = slot_val<K, V, LF>(t, p);
+                const uint64_t p = base + (slot_key<K, V, LF>(t, base) < key);
+                if (p <= (uint64_t)hi && slot_key<K, V, LF>(t, p) == key) res = slot_val<K, V, LF>(t, p);
             }
         }
         __builtin_nontemporal_store(res, &out[i]);
@@ -379,9 +379,7 @@ __global__ __launch_bounds__(256) void k_point_learned_coop(TreeView t, LearnedV
                                                             V *__restrict__ out, uint64_t nq, V sentinel,
                                                             uint32_t lds_tree_bytes, uint32_t *__restrict__ fb_count,
                                                             uint32_t *__restrict__ fb_list, uint32_t fb_cap) {
-    static_assert(sizeof(K) == sizeof(V), "cooperative leaf split needs equal key/value widths");
-    constexpr int LB = LF * (int)sizeof(K) / 4;
-    constexpr int KPC = LB / (int)sizeof(K);
+    static_assert(sizeof(K) == sizeof(V), "cooperative leaf needs equal key/value widths");
     constexpr uint32_t SPL = 4;  // separators per lane, 32 leaves per group step
     extern __shared__ uint4 lds_raw[];
     uint8_t *lds = reinterpret_cast<uint8_t *>(lds_raw);
@@ -460,22 +458,9 @@ __global__ __launch_bounds__(256) void k_point_learned_coop(TreeView t, LearnedV
             const uint32_t lf = aa + r;
             leaf[k] = a[k] > b[k] ? 0u : (lf > b[k] ? b[k] : lf);
         }
-        // phase B.2: leaf lines
-        Chunk<LB> c[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) load_chunk_nt<LB>(t.leaves + (uint64_t)leaf[k] * t.leaf_stride + sub * LB, c[k]);
-        V res = sentinel;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const uint32_t part = sub < 4 ? chunk_count_lt<K, LB>(c[k], qk[k]) : 0u;
-            const uint32_t r = group8_sum(part);
-            const uint32_t rl = r < LF ? r : LF - 1;
-            const uint32_t ksrc = g * 8 + rl / KPC, comp = rl % KPC;
-            const K key = __shfl(chunk_elem<K, LB>(c[k], comp), (int)ksrc, 64);
-            const V val = __shfl((V)chunk_elem<K, LB>(c[k], comp), (int)(ksrc + 4), 64);
-            const bool hit = a[k] <= b[k] && r < LF && (uint64_t)leaf[k] * LF + r < t.n && key == qk[k];
-            if (sub == (uint32_t)k && hit) res = val;
-        }
+        // phase B.2: leaf lines (an empty window, la > lb, is a miss)
+        const V got = coop_leaf<K, V, LF, true, kOpValue, 8, false>(t, qk, leaf, sentinel);
+        const V res = la <= lb ? got : sentinel;
         if (my < nq && !fb) __builtin_nontemporal_store(res, &out[my]);
     }
     __syncthreads();
@@ -588,13 +573,13 @@ static bpt_status mlp_grad_dev(const float *d_params, const float *d_x, const fl
     return BPT_OK;
 }
 
-template <class K, int LF>
+template <class K, class V, int LF>
 __global__ void k_training_samples(TreeView t, uint32_t m, float *__restrict__ xs, float *__restrict__ ts) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
     const uint64_t pos = (uint64_t)(((double)i + 0.5) * (double)t.n / (double)m);
     const uint64_t p = pos < t.n ? pos : t.n - 1;
-    xs[i] = key_feature<K>(slot_key<K, LF>(t, p));
+    xs[i] = key_feature<K>(slot_key<K, V, LF>(t, p));
     ts[i] = (float)((double)p / (double)t.n);
 }
 
@@ -624,7 +609,7 @@ static bpt_status learned_build_impl(bpt_tree *t, const bpt_learned_config &cfg,
         H(hipMemsetAsync(d_vel, 0, kNP * 4, s), "memset");
         H(hipMemsetAsync(d_step, 0, 4, s), "memset");
         if (st == BPT_OK) {
-            k_training_samples<K, LF><<<(m + 255) / 256, 256, 0, s>>>(t->view, m, d_x, d_t);
+            k_training_samples<K, V, LF><<<(m + 255) / 256, 256, 0, s>>>(t->view, m, d_x, d_t);
             H(hipGetLastError(), "samples");
         }
         if (st == BPT_OK) st = mlp_grad_dev(lm->d_params, d_x, d_t, m, d_partial, blocks, d_grad, s);
@@ -701,9 +686,9 @@ static bpt_status learned_build_impl(bpt_tree *t, const bpt_learned_config &cfg,
     v.models = lm->d_models;
     if (st == BPT_OK) {
         const uint32_t g = (uint32_t)std::min<uint64_t>((t->n + 255) / 256, (uint64_t)device_cu_count() * 16);
-        k_route_keys<K, LF><<<g, 256, learned_lds(v), s>>>(t->view, v, d_routes, d_pmin, d_pmax, d_cnt);
-        k_fit_models<K, LF><<<(M + 255) / 256, 256, 0, s>>>(t->view, M, d_pmin, d_pmax, d_cnt, lm->d_models);
-        k_model_errors<K, LF><<<g, 256, 0, s>>>(t->view, d_routes, lm->d_models);
+        k_route_keys<K, V, LF><<<g, 256, learned_lds(v), s>>>(t->view, v, d_routes, d_pmin, d_pmax, d_cnt);
+        k_fit_models<K, V, LF><<<(M + 255) / 256, 256, 0, s>>>(t->view, M, d_pmin, d_pmax, d_cnt, lm->d_models);
+        k_model_errors<K, V, LF><<<g, 256, 0, s>>>(t->view, d_routes, lm->d_models);
         k_finalize_models<<<(M + 255) / 256, 256, 0, s>>>(lm->d_models, M, cfg.max_window, d_acc);
         H(hipGetLastError(), "leaf models");
     }

@@ -73,7 +73,7 @@ __device__ __forceinline__ uint64_t bigmin(uint64_t z, uint64_t zmin, uint64_t z
 
 template <int F, int LF>
 __device__ __forceinline__ uint64_t lb_u64(const TreeView &t, const uint8_t *lds, uint64_t z) {
-    return leaf_lower_bound<uint64_t, LF>(t, descend<uint64_t, F>(t, lds, z), z);
+    return leaf_lower_bound<uint64_t, uint32_t, LF>(t, descend<uint64_t, F>(t, lds, z), z);
 }
 
 // Walks rectangle r; calls emit(slot) for every point inside, in code order.
@@ -85,7 +85,7 @@ __device__ __forceinline__ void rect_walk(const TreeView &t, const uint8_t *lds,
     const uint64_t zmax = spread(x1) | (spread(y1) << 1);
     uint64_t p = lb_u64<F, LF>(t, lds, zmin);
     while (p < t.n) {
-        const uint64_t k = slot_key<uint64_t, LF>(t, p);
+        const uint64_t k = slot_key<uint64_t, uint32_t, LF>(t, p);
         if (k > zmax) break;
         const uint32_t x = compact(k), y = compact(k >> 1);
         if (x >= x0 && x <= x1 && y >= y0 && y <= y1) {
@@ -98,7 +98,7 @@ __device__ __forceinline__ void rect_walk(const TreeView &t, const uint8_t *lds,
         // stay inside the current leaf while its keys are below the target
         const uint64_t leaf_end = (p / LF + 1) * LF;
         uint64_t q = p + 1;
-        while (q < leaf_end && q < t.n && slot_key<uint64_t, LF>(t, q) < nz) ++q;
+        while (q < leaf_end && q < t.n && slot_key<uint64_t, uint32_t, LF>(t, q) < nz) ++q;
         if (q < leaf_end || q >= t.n) {
             p = q;
         } else {

@@ -52,7 +52,7 @@ __global__ __launch_bounds__(256) void k_point_coop(TreeView t, const K *__restr
     }
 }
 
-template <class K, int F, int LF>
+template <class K, class V, int F, int LF>
 __global__ __launch_bounds__(256) void k_lower_bound(TreeView t, const K *__restrict__ q,
                                                      uint64_t *__restrict__ pos, uint64_t nq) {
     extern __shared__ uint4 lds_raw[];
@@ -62,13 +62,13 @@ __global__ __launch_bounds__(256) void k_lower_bound(TreeView t, const K *__rest
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += step) {
         const K key = q[i];
         const uint64_t leaf = descend<K, F>(t, lds, key);
-        pos[i] = leaf_lower_bound<K, LF>(t, leaf, key);
+        pos[i] = leaf_lower_bound<K, V, LF>(t, leaf, key);
     }
 }
 
 // count = upper_bound(hi) - lower_bound(lo); upper_bound(hi) is
 // lower_bound(hi + 1), or n when hi is the largest key value.
-template <class K, int F, int LF>
+template <class K, class V, int F, int LF>
 __global__ __launch_bounds__(256) void k_range_count(TreeView t, const K *__restrict__ lo,
                                                      const K *__restrict__ hi,
                                                      const uint32_t *__restrict__ order,
@@ -81,11 +81,11 @@ __global__ __launch_bounds__(256) void k_range_count(TreeView t, const K *__rest
     for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nq; j += step) {
         const uint64_t i = order ? order[j] : j;
         const K a = lo[i], b = hi[i];
-        const uint64_t pa = leaf_lower_bound<K, LF>(t, descend<K, F>(t, lds, a), a);
+        const uint64_t pa = leaf_lower_bound<K, V, LF>(t, descend<K, F>(t, lds, a), a);
         uint64_t pb = t.n;
         if (b != key_max<K>()) {
             const K b1 = b + 1;
-            pb = leaf_lower_bound<K, LF>(t, descend<K, F>(t, lds, b1), b1);
+            pb = leaf_lower_bound<K, V, LF>(t, descend<K, F>(t, lds, b1), b1);
         }
         if (first) first[i] = pa;
         count[i] = (a > b || pb < pa) ? 0 : pb - pa;
@@ -314,9 +314,10 @@ bpt_status bpt_lower_bound(const bpt_tree *t, const void *d_q, uint64_t nq, uint
         BPT_HIP(hipGetLastError());
         return BPT_OK;
     }
-    return dispatch_tree(t, [&](auto kt, auto, auto fc, auto lc) -> bpt_status {
+    return dispatch_tree(t, [&](auto kt, auto vt, auto fc, auto lc) -> bpt_status {
         using K = typename decltype(kt)::type;
-        k_lower_bound<K, decltype(fc)::value, decltype(lc)::value>
+        using V = typename decltype(vt)::type;
+        k_lower_bound<K, V, decltype(fc)::value, decltype(lc)::value>
             <<<search_grid(t, nq), t->block, lds_of(t), s>>>(t->view, (const K *)d_q, d_pos, nq);
         BPT_HIP(hipGetLastError());
         return BPT_OK;
@@ -336,9 +337,10 @@ bpt_status range_count_ordered(const bpt_tree *t, const void *d_lo, const void *
         BPT_HIP(hipGetLastError());
         return BPT_OK;
     }
-    return dispatch_tree(t, [&](auto kt, auto, auto fc, auto lc) -> bpt_status {
+    return dispatch_tree(t, [&](auto kt, auto vt, auto fc, auto lc) -> bpt_status {
         using K = typename decltype(kt)::type;
-        k_range_count<K, decltype(fc)::value, decltype(lc)::value>
+        using V = typename decltype(vt)::type;
+        k_range_count<K, V, decltype(fc)::value, decltype(lc)::value>
             <<<search_grid(t, nq), t->block, lds_of(t), s>>>(t->view, (const K *)d_lo, (const K *)d_hi,
                                                              d_order, d_first, d_count, nq);
         BPT_HIP(hipGetLastError());

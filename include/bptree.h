@@ -73,9 +73,11 @@ bpt_status bpt_timer_ms(bpt_timer *t, float *ms); /* waits for the stop event */
 /* ------------------------------------------------------------------ */
 /* Replaces genBPlusTree (BPTree.h:26-29, BPTree.cu:127-306): page pool +
  * leaf copy + one kernel per internal level. Here: a flat, level-ordered
- * layout, leaves of `leaf_cap` (key, value) slots (keys then values in one
- * line), internal nodes of `fanout` separators (child maximum, as the
- * reference's kernelMaxIntermidatePage, BPTree.cu:54), children implicit. */
+ * layout; leaves of `leaf_cap` (key, value) slots, interleaved in 16-byte
+ * pieces when key and value widths are equal ([k0 k1 v0 v1] for 4-byte,
+ * [k0 v0] for 8-byte slots), otherwise all keys then all values; internal
+ * nodes of `fanout` separators (child maximum, as the reference's
+ * kernelMaxIntermidatePage, BPTree.cu:54), children implicit. */
 typedef struct bpt_build_desc {
     uint32_t key_bytes;    /* 4 or 8 */
     uint32_t val_bytes;    /* 4 or 8 */
