@@ -272,7 +272,9 @@ __global__ __launch_bounds__(kBBlock) void k_bkt_colprefix(const uint32_t *__res
 }
 
 // LDS (widest first, so every array is aligned):
-//   sk[T] | bsep[nb] | gstart[nb] | lstart[nb] | lcur[nb] | sl[T] | sb[T] | btab[2^10 + 1]
+//   sk[T] | bsep[nb] | delta[nb] | lcur[nb] | sbl[T] | btab[2^10 + 1]
+// sbl[j] = (bucket << 16) | position in the tile of tile-sorted element j;
+// delta[b] = first global slot of bucket b's run minus its first tile slot.
 template <class K>
 __global__ __launch_bounds__(kPBlock) void k_bkt_scatter(const K *__restrict__ q, uint64_t nq, BucketDev d,
                                                          const K *__restrict__ bsep_g,
@@ -285,12 +287,10 @@ __global__ __launch_bounds__(kPBlock) void k_bkt_scatter(const K *__restrict__ q
     __shared__ uint32_t wsum[kPBlock / 64];
     K *sk = reinterpret_cast<K *>(lds_raw);
     K *bsep = sk + d.tile;
-    uint32_t *gstart = reinterpret_cast<uint32_t *>(bsep + d.nb);
-    uint32_t *lstart = gstart + d.nb;
-    uint32_t *lcur = lstart + d.nb;
-    uint16_t *sl = reinterpret_cast<uint16_t *>(lcur + d.nb);
-    uint16_t *sb = sl + d.tile;
-    uint16_t *btab = sb + d.tile;
+    uint32_t *delta = reinterpret_cast<uint32_t *>(bsep + d.nb);
+    uint32_t *lcur = delta + d.nb;
+    uint32_t *sbl = lcur + d.nb;
+    uint16_t *btab = reinterpret_cast<uint16_t *>(sbl + d.tile);
     const uint32_t tile = xcd_contiguous(blockIdx.x, gridDim.x);
     constexpr int E = per_thread<K>();
     const uint64_t lo = (uint64_t)tile * d.tile;
@@ -303,15 +303,13 @@ __global__ __launch_bounds__(kPBlock) void k_bkt_scatter(const K *__restrict__ q
     }
     for (uint32_t b = threadIdx.x; b < d.nb; b += blockDim.x) {
         bsep[b] = bsep_g[b];
-        const uint64_t e = (uint64_t)tile * d.nb + b;
-        gstart[b] = off[e];
-        lstart[b] = hist[e];  // this tile's count of bucket b
+        lcur[b] = hist[(uint64_t)tile * d.nb + b];  // this tile's count of bucket b
     }
     for (uint32_t p = threadIdx.x; p <= (1u << kBucketTabBits); p += blockDim.x) btab[p] = btab_g[p];
     const PrefixIndex<K> pi{(K)hdr[0], (uint32_t)hdr[1]};
     __syncthreads();
-    block_exclusive_scan(lstart, d.nb, wsum);
-    for (uint32_t b = threadIdx.x; b < d.nb; b += blockDim.x) lcur[b] = lstart[b];
+    block_exclusive_scan(lcur, d.nb, wsum);  // first tile slot of every bucket
+    for (uint32_t b = threadIdx.x; b < d.nb; b += blockDim.x) delta[b] = off[(uint64_t)tile * d.nb + b] - lcur[b];
     __syncthreads();
     uint32_t bk[E];
 #pragma unroll
@@ -322,8 +320,7 @@ __global__ __launch_bounds__(kPBlock) void k_bkt_scatter(const K *__restrict__ q
         if (i < cnt) {
             const uint32_t slot = atomicAdd(&lcur[bk[e]], 1u);
             sk[slot] = key[e];
-            sl[slot] = (uint16_t)i;
-            sb[slot] = (uint16_t)bk[e];
+            sbl[slot] = (bk[e] << 16) | i;
         }
     }
     __syncthreads();
@@ -332,10 +329,10 @@ __global__ __launch_bounds__(kPBlock) void k_bkt_scatter(const K *__restrict__ q
     for (int e = 0; e < E; ++e) {
         const uint32_t j = e * kPBlock + threadIdx.x;
         if (j < cnt) {
-            const uint32_t b = sb[j];
-            const uint32_t g = gstart[b] + (j - lstart[b]);
+            const uint32_t x = sbl[j];
+            const uint32_t g = delta[x >> 16] + j;
             qs[g] = sk[j];
-            lidx[g] = sl[j];
+            lidx[g] = (uint16_t)(x & 0xFFFFu);
         }
     }
 }
@@ -429,7 +426,10 @@ __global__ __launch_bounds__(kSearchBlock) void k_bkt_search(TreeView t, BucketD
     }
 }
 
-// LDS: gstart[nb] | lstart[nb + 1] | outt[T]
+// LDS: delta[nb] | lstart[nb + 1] | outt[T] (V), whose first 2T bytes hold
+// bid[T] (u16, bucket of every tile-sorted position) before the results
+// arrive. Each wave paints whole buckets' runs of bid with its 64 lanes, so the
+// cost follows the run lengths, not a search per element.
 template <class K, class V>
 __global__ __launch_bounds__(kPBlock) void k_bkt_unpermute(const uint32_t *__restrict__ hist,
                                                            const uint32_t *__restrict__ off, BucketDev d,
@@ -438,18 +438,20 @@ __global__ __launch_bounds__(kPBlock) void k_bkt_unpermute(const uint32_t *__res
                                                            V *__restrict__ out) {
     extern __shared__ uint4 lds_raw[];
     __shared__ uint32_t wsum[kPBlock / 64];
-    uint32_t *gstart = reinterpret_cast<uint32_t *>(lds_raw);
-    uint32_t *lstart = gstart + d.nb;
+    uint32_t *delta = reinterpret_cast<uint32_t *>(lds_raw);
+    uint32_t *lstart = delta + d.nb;
     V *outt = reinterpret_cast<V *>(lds_raw + (2 * d.nb + 1 + 3) / 4);
+    uint16_t *bid = reinterpret_cast<uint16_t *>(outt);
     const uint32_t tile = xcd_contiguous(blockIdx.x, gridDim.x);
-    for (uint32_t b = threadIdx.x; b < d.nb; b += blockDim.x) {
-        const uint64_t e = (uint64_t)tile * d.nb + b;
-        gstart[b] = off[e];
-        lstart[b] = hist[e];
-    }
+    for (uint32_t b = threadIdx.x; b < d.nb; b += blockDim.x) lstart[b] = hist[(uint64_t)tile * d.nb + b];
     __syncthreads();
     const uint32_t total = block_exclusive_scan(lstart, d.nb, wsum);
+    for (uint32_t b = threadIdx.x; b < d.nb; b += blockDim.x) delta[b] = off[(uint64_t)tile * d.nb + b] - lstart[b];
     if (threadIdx.x == 0) lstart[d.nb] = total;
+    __syncthreads();
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (uint32_t b = w; b < d.nb; b += nw)
+        for (uint32_t j = lstart[b] + lane; j < lstart[b + 1]; j += 64) bid[j] = (uint16_t)b;
     __syncthreads();
     constexpr int E = per_thread<K>();
     const uint64_t lo = (uint64_t)tile * d.tile;
@@ -458,16 +460,9 @@ __global__ __launch_bounds__(kPBlock) void k_bkt_unpermute(const uint32_t *__res
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const uint32_t j = e * kPBlock + threadIdx.x;
-        // bucket of tile-sorted element j: the last b with lstart[b] <= j
-        // (lstart[nb] = cnt bounds it; trip count depends on nb only)
-        uint32_t a = 0, n = d.nb;
-        while (n > 1) {
-            const uint32_t half = n >> 1;
-            a = lstart[a + half] <= j ? a + half : a;
-            n -= half;
-        }
-        g[e] = gstart[a] + (j - lstart[a]);
+        g[e] = j < cnt ? delta[bid[j]] + j : 0u;
     }
+    __syncthreads();  // bid is dead: outt reuses its bytes
     uint16_t li[E];
     V rv[E];
 #pragma unroll
@@ -541,7 +536,7 @@ static BucketPlan bucket_plan(const bpt_tree *t, uint64_t nq) {
     const size_t kb = t->key_bytes, vb = t->val_bytes;
     const size_t btab_bytes = ((1u << kBucketTabBits) + 1) * 2, stab_bytes = ((1u << kSliceTabBits) + 1) * 2;
     p.lds_hist = d.nb * (kb + 4) + btab_bytes;
-    p.lds_scatter = d.nb * (kb + 12) + (size_t)d.tile * (kb + 4) + btab_bytes;
+    p.lds_scatter = d.nb * (kb + 8) + (size_t)d.tile * (kb + 4) + btab_bytes;
     p.lds_search = ((d.nb + 1 + 3) / 4) * 16 + (size_t)d.epb * kb + stab_bytes;
     p.lds_unpermute = ((2 * d.nb + 1 + 3) / 4) * 16 + (size_t)d.tile * vb;
     return p;
