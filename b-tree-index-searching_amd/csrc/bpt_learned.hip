@@ -89,7 +89,9 @@ __global__ __launch_bounds__(kTile) void k_mlp_grad(const float *__restrict__ pa
     __syncthreads();
     const float *w1 = P, *b1 = P + kH, *W2 = P + 2 * kH, *b2 = W2 + kH * kH, *w3 = b2 + kH, *b3 = w3 + kH;
     constexpr int PER = kH * kH / kTile;  // W2 entries per thread
+    static_assert(kH * kH % kTile == 0 && kH % PER == 0, "a thread's W2 entries lie in one row");
     float gW2[PER];  // W2 entries tid*PER .. tid*PER+PER-1
+#pragma unroll
     for (int r = 0; r < PER; ++r) gW2[r] = 0.f;
     float gv[4] = {0.f, 0.f, 0.f, 0.f};  // tid < 32: w1, b1, b2, w3 of unit tid
     float gb3 = 0.f, gloss = 0.f;
@@ -97,19 +99,23 @@ __global__ __launch_bounds__(kTile) void k_mlp_grad(const float *__restrict__ pa
         const uint64_t s = tile * kTile + tid;
         const bool valid = s < m;
         const float x = valid ? xs[s] : 0.f, t = valid ? ts[s] : 0.f;
-        float a1[kH], h1[kH], a2[kH];
+        // ReLU masks are kept as bit words: register arrays are indexed only in
+        // fully unrolled loops (a variable index would put them in scratch)
+        float h1[kH];
+        uint32_t m1 = 0, m2 = 0;
 #pragma unroll
         for (int k = 0; k < kH; ++k) {
-            a1[k] = fmaf(w1[k], x, b1[k]);
-            h1[k] = a1[k] > 0.f ? a1[k] : 0.f;
+            const float a = fmaf(w1[k], x, b1[k]);
+            h1[k] = a > 0.f ? a : 0.f;
+            m1 |= (a > 0.f ? 1u : 0u) << k;
         }
         float y = b3[0];
-#pragma unroll 4
+#pragma unroll 1
         for (int j = 0; j < kH; ++j) {
             float a = b2[j];
 #pragma unroll
             for (int k = 0; k < kH; ++k) a = fmaf(W2[j * kH + k], h1[k], a);
-            a2[j] = a;
+            m2 |= (a > 0.f ? 1u : 0u) << j;
             const float h = a > 0.f ? a : 0.f;
             s_h2[tid][j] = h;
             y = fmaf(w3[j], h, y);
@@ -120,25 +126,30 @@ __global__ __launch_bounds__(kTile) void k_mlp_grad(const float *__restrict__ pa
         float d2[kH];
 #pragma unroll
         for (int j = 0; j < kH; ++j) {
-            d2[j] = a2[j] > 0.f ? dy * w3[j] : 0.f;
+            d2[j] = ((m2 >> j) & 1u) ? dy * w3[j] : 0.f;
             s_d2[tid][j] = d2[j];
             s_h1[tid][j] = h1[j];
         }
-#pragma unroll 4
+#pragma unroll 1
         for (int k = 0; k < kH; ++k) {
             float d = 0.f;
 #pragma unroll
             for (int j = 0; j < kH; ++j) d = fmaf(d2[j], W2[j * kH + k], d);
-            s_d1[tid][k] = a1[k] > 0.f ? d : 0.f;
+            s_d1[tid][k] = ((m1 >> k) & 1u) ? d : 0.f;
         }
         s_x[tid] = x;
         s_dy[tid] = dy;
         __syncthreads();
-        for (int r = 0; r < PER; ++r) {
-            const int e2 = tid * PER + r, j = e2 / kH, k = e2 % kH;
-            float acc = 0.f;
-            for (int q = 0; q < kTile; ++q) acc = fmaf(s_d2[q][j], s_h1[q][k], acc);
-            gW2[r] += acc;
+        {
+            // this thread's W2 entries share one row j and run over PER
+            // consecutive columns from k0
+            const int j = (tid * PER) / kH, k0 = (tid * PER) % kH;
+#pragma unroll 2
+            for (int q = 0; q < kTile; ++q) {
+                const float dj = s_d2[q][j];
+#pragma unroll
+                for (int r = 0; r < PER; ++r) gW2[r] = fmaf(dj, s_h1[q][k0 + r], gW2[r]);
+            }
         }
         if (tid < kH) {
             float a = 0.f, b = 0.f, c = 0.f, d = 0.f;
@@ -158,6 +169,7 @@ __global__ __launch_bounds__(kTile) void k_mlp_grad(const float *__restrict__ pa
         __syncthreads();
     }
     float *out = partial + (uint64_t)blockIdx.x * (kNP + 1);
+#pragma unroll
     for (int r = 0; r < PER; ++r) out[2 * kH + tid * PER + r] = gW2[r];
     if (tid < kH) {
         out[tid] = gv[0];                              // w1
