@@ -60,6 +60,13 @@ def build(force: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"bptree_main failed to build:\n{r.stderr[-6000:]}")
     os.replace(exe + ".tmp", exe)
+    # seeded generators of the reference's input files (host code only)
+    gen = os.path.join(HERE, "bptree_gen")
+    cmd = ["g++", "-O3", "-std=c++17", "-Wall", os.path.join(HERE, "examples", "bptree_gen.cpp"), "-o", gen + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"bptree_gen failed to build:\n{r.stderr[-6000:]}")
+    os.replace(gen + ".tmp", gen)
     return LIB
 
 

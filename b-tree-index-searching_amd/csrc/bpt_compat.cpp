@@ -265,6 +265,51 @@ void *SearchingBTreeGPURangeWithGroupingZorder(ulli *input_key_start, ulli *inpu
     return range_counts(input_key_start, input_key_end, inputSize, dev_globalPointer, BPT_GROUP_ZORDER);
 }
 
+void *SearchingBTreeGPURangeResults(ulli *input_key_start, ulli *input_key_end, ulli inputSize, ulli *dev_Btree,
+                                    BTPage *dev_globalPointer, int groupMode, ulli *&output_offset, ulli *&ids) {
+    (void)dev_Btree;
+    output_offset = nullptr;
+    ids = nullptr;
+    if (!dev_globalPointer || groupMode < BPT_GROUP_NONE || groupMode > BPT_GROUP_ZORDER) return nullptr;
+    const ulli n = inputSize;
+    ulli *counts = new ulli[n ? n : 1];
+    ulli *offs = new ulli[n + 1];
+    ulli *dlo = to_device(input_key_start, n), *dhi = to_device(input_key_end, n);
+    void *dfirst = nullptr, *dcount = nullptr, *doff = nullptr, *dids = nullptr;
+    uint64_t total = 0;
+    bool good = dlo && dhi && ok(bpt_device_alloc(n * 8, &dfirst), "device alloc") &&
+                ok(bpt_device_alloc(n * 8, &dcount), "device alloc") && ok(bpt_device_alloc(n * 8, &doff), "device alloc") &&
+                ok(bpt_range_count_grouped(dev_globalPointer, dlo, dhi, n, groupMode, static_cast<uint64_t *>(dfirst),
+                                           static_cast<uint64_t *>(dcount), nullptr),
+                   "range count") &&
+                ok(bpt_exclusive_scan_u64(static_cast<const uint64_t *>(dcount), n, static_cast<uint64_t *>(doff), &total,
+                                          nullptr),
+                   "count scan");
+    ulli *h_ids = good ? new ulli[total ? total : 1] : nullptr;
+    good = good && ok(bpt_device_alloc(total ? total * 8 : 8, &dids), "device alloc") &&
+           ok(bpt_range_ids(dev_globalPointer, static_cast<const uint64_t *>(dfirst), static_cast<const uint64_t *>(dcount),
+                            static_cast<const uint64_t *>(doff), n, dids, nullptr),
+              "range ids") &&
+           to_host(counts, static_cast<const ulli *>(dcount), n) && to_host(offs, static_cast<const ulli *>(doff), n) &&
+           to_host(h_ids, static_cast<const ulli *>(dids), total);
+    bpt_device_free(dlo);
+    bpt_device_free(dhi);
+    bpt_device_free(dfirst);
+    bpt_device_free(dcount);
+    bpt_device_free(doff);
+    bpt_device_free(dids);
+    if (!good) {
+        delete[] counts;
+        delete[] offs;
+        delete[] h_ids;
+        return nullptr;
+    }
+    offs[n] = total;
+    output_offset = offs;
+    ids = h_ids;
+    return counts;
+}
+
 // ------------------------------------------------------------------ grouping.h
 int bucketIdentify(ulli input, int nPass) {
     const int shift = 8 * (7 - nPass);

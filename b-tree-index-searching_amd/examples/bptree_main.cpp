@@ -6,7 +6,9 @@
 // their data on the device first (main.cu:106-118), the tree is built with
 // genBPlusTree, and the searches run through the reference-named functions.
 // Extra (no reference counterpart): `-o <prefix>` writes each search's
-// results as raw uint64 to <prefix>.<form>.bin.
+// results as raw uint64 to <prefix>.<form>.bin; with `--ids`, range searches
+// also write the two-phase ids of SearchingBTreeGPURangeResults
+// (<prefix>.range_offsets.bin, inputSize + 1 entries, and <prefix>.range_ids.bin).
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -36,14 +38,17 @@ static void write_u64(const std::string &path, const ulli *v, ulli n) {
 
 int main(int argc, char **argv) {
     std::string out_prefix;
+    bool want_ids = false;
     std::vector<char *> args;
     for (int i = 0; i < argc; ++i) {
         if (!strcmp(argv[i], "-o") && i + 1 < argc) { out_prefix = argv[++i]; continue; }
+        if (!strcmp(argv[i], "--ids")) { want_ids = true; continue; }
         args.push_back(argv[i]);
     }
     const int na = (int)args.size();
     if (na < 4) {
-        fprintf(stderr, "usage: %s inputSize fileNameKey filenameData [nq qfile | nq startfile endfile] [-o prefix]\n", argv[0]);
+        fprintf(stderr, "usage: %s inputSize fileNameKey filenameData [nq qfile | nq startfile endfile] "
+                        "[-o prefix [--ids]]\n", argv[0]);
         return -1;
     }
     const ulli n = strtoull(args[1], nullptr, 10);
@@ -103,6 +108,20 @@ int main(int argc, char **argv) {
             if (!r) { rc = 1; continue; }
             if (!out_prefix.empty()) write_u64(out_prefix + "." + names[f] + ".bin", r, nq);
             delete[] r;
+        }
+        if (!out_prefix.empty() && want_ids) {
+            ulli *offsets = nullptr, *ids = nullptr;
+            ulli *r = static_cast<ulli *>(
+                SearchingBTreeGPURangeResults(s.data(), e.data(), nq, dev_btree, pool, 0, offsets, ids));
+            if (!r) rc = 1;
+            else {
+                write_u64(out_prefix + ".range_offsets.bin", offsets, nq + 1);
+                write_u64(out_prefix + ".range_ids.bin", ids, offsets[nq]);
+                printf("Range ids: %llu\n", offsets[nq]);
+            }
+            delete[] r;
+            delete[] offsets;
+            delete[] ids;
         }
     }
     freeBPlusTree(pool, dev_btree, dev_currentPage);

@@ -54,4 +54,17 @@ void *SearchingBTreeGPURangeWithGrouping(ulli *input_key_start, ulli *input_key_
 void *SearchingBTreeGPURangeWithGroupingZorder(ulli *input_key_start, ulli *input_key_end,
                                                ulli inputSize, ulli *dev_Btree,
                                                BTPage *dev_globalPointer);
+
+/* No reference counterpart: the range search with its results on the host.
+ * The reference's state 3 (BtreeRangeSearch.cu:503-510, kernelGenerateSearchResult
+ * :297-374) fills a device uni_result of inputSize * DOB_PAGE_SIZE slots, one
+ * count per query, and frees it. Here the returned new[]'d array is that
+ * uni_result layout (the inclusive count of query i in slot i * DOB_PAGE_SIZE),
+ * and the ids come two-phase: output_offset (new[]'d, inputSize + 1 entries)
+ * is the exclusive scan of the counts, and ids (new[]'d, output_offset[inputSize]
+ * entries, at least one) holds the data of every key in [start, end], query
+ * by query, in key order. groupMode: 0 plain, 1 2-D grouping, 2 Z-order. */
+void *SearchingBTreeGPURangeResults(ulli *input_key_start, ulli *input_key_end, ulli inputSize,
+                                    ulli *dev_Btree, BTPage *dev_globalPointer, int groupMode,
+                                    ulli *&output_offset, ulli *&ids);
 #endif

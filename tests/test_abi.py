@@ -34,6 +34,32 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def compat_declared():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "compat", "*.h")):
+        text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        names.update(re.findall(r"^\s*[A-Za-z_][\w\s\*&]*?\b([A-Za-z_]\w*)\s*\(", text, re.M))
+    return sorted(names)
+
+
+def test_library_exports_every_compat_function():
+    """The reference-named C++ surface (include/compat/*.h) is defined in the
+    library under its C++ names."""
+    import shutil
+    import subprocess
+    import bptree
+    nm = shutil.which("nm")
+    if nm is None:
+        pytest.skip("nm not available")
+    out = subprocess.run([nm, "-DC", "--defined-only", bptree.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    exported = {line.split(" ", 2)[2].split("(")[0] for line in out.splitlines() if " T " in line}
+    declared = compat_declared()
+    assert "genBPlusTree" in declared and "SearchingBTreeGPURangeResults" in declared
+    missing = [n for n in declared if n not in exported]
+    assert not missing, missing
+
+
 def test_python_bindings_cover_header():
     import bptree
     bound = {name for name, _, _ in bptree.SIGNATURES}

@@ -54,6 +54,62 @@ def test_driver_point_and_range(tmp_path, oracle):
         assert np.array_equal(got, want_c), form
 
 
+def test_driver_range_ids(tmp_path, oracle):
+    """SearchingBTreeGPURangeResults through the driver: the uni_result
+    counts, their exclusive scan and the two-phase ids (data of every key in
+    [start, end], query by query, in key order) against the oracle."""
+    rng = np.random.default_rng(23)
+    n = 200_000
+    keys = np.unique(rng.integers(1, 1 << 40, size=n, dtype=np.uint64))
+    n = len(keys)
+    data = rng.integers(0, 1 << 62, size=n, dtype=np.uint64)
+    perm = rng.permutation(n)
+    _write(tmp_path / "k", keys[perm])
+    _write(tmp_path / "d", data[perm])
+    nq = 20_000
+    s = rng.integers(0, 1 << 40, size=nq, dtype=np.uint64)
+    e = s + np.floor(np.exp2(rng.uniform(0, 30, size=nq))).astype(np.uint64)
+    s[:100] = e[:100] + np.uint64(5)            # empty ranges (start > end)
+    s[100:200] = keys[rng.integers(0, n, 100)]  # bounds on keys
+    e[100:200] = s[100:200]
+    _write(tmp_path / "s", s)
+    _write(tmp_path / "e", e)
+    out = str(tmp_path / "res")
+    r = subprocess.run([EXE, str(n), str(tmp_path / "k"), str(tmp_path / "d"), str(nq), str(tmp_path / "s"),
+                        str(tmp_path / "e"), "-o", out, "--ids"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    first, count = oracle.range_closed(keys, s, e)
+    offsets = np.fromfile(f"{out}.range_offsets.bin", dtype=np.uint64)
+    ids = np.fromfile(f"{out}.range_ids.bin", dtype=np.uint64)
+    assert np.array_equal(np.fromfile(f"{out}.range.bin", dtype=np.uint64), count)
+    assert np.array_equal(offsets, np.concatenate([[0], np.cumsum(count)]).astype(np.uint64))
+    want = np.concatenate([data[int(f):int(f) + int(c)] for f, c in zip(first, count)])
+    assert np.array_equal(ids, want)
+    assert int(count[100:200].min()) == 1 and int(count[:100].max()) == 0
+
+
+def test_driver_with_generated_files(tmp_path, oracle):
+    """The reference's workflow: bptree_gen writes the key file and Zipf and
+    normal query files (randomgenator/), and the driver is run with the key
+    file as the data file too, as run.sh does; results equal the oracle."""
+    gen = os.path.join(ROOT, "b-tree-index-searching_amd", "bptree_gen")
+    n, nq = 300_000, 200_000
+    for args in (["keys", n, tmp_path / "k", 11], ["queries", tmp_path / "k", nq, 2, tmp_path / "qz", 12],
+                 ["queries", tmp_path / "k", nq, 1, tmp_path / "qn", 13]):
+        r = subprocess.run([gen, *map(str, args)], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+    keys = np.sort(np.fromfile(tmp_path / "k", dtype=np.uint64))
+    for qf in ("qz", "qn"):
+        q = np.fromfile(tmp_path / qf, dtype=np.uint64)
+        out = str(tmp_path / f"res_{qf}")
+        r = subprocess.run([EXE, str(n), str(tmp_path / "k"), str(tmp_path / "k"), str(nq), str(tmp_path / qf),
+                            "-o", out], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        want = oracle.point_closed(keys, keys, q, SENT)
+        for form in ("plain", "grouped"):
+            assert np.array_equal(np.fromfile(f"{out}.{form}.bin", dtype=np.uint64), want), (qf, form)
+
+
 def test_sort_pairs(bpt):
     rng = np.random.default_rng(22)
     for dt in (np.uint32, np.uint64):
