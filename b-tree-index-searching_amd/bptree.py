@@ -78,6 +78,8 @@ SIGNATURES = [
     ("bpt_tree_get_info", C.c_int, [vp, C.POINTER(TreeInfo)]),
     ("bpt_tree_get_layout", C.c_int, [vp, C.POINTER(vp), C.POINTER(vp)]),
     ("bpt_tree_set_launch", C.c_int, [vp, C.c_uint32, C.c_uint32]),
+    ("bpt_tree_upsert", C.c_int, [vp, vp, vp, C.c_uint64, vp]),
+    ("bpt_tree_erase", C.c_int, [vp, vp, C.c_uint64, vp]),
     ("bpt_tree_set_option", C.c_int, [vp, C.c_char_p, C.c_int64]),
     ("bpt_point_search", C.c_int, [vp, vp, C.c_uint64, vp, C.c_int, vp]),
     ("bpt_point_search_ws_size", C.c_uint64, [vp, C.c_uint64, C.c_int]),
@@ -362,6 +364,27 @@ class Tree:
         if own:
             dq.free()
         return r
+
+    def upsert(self, keys, vals, stream=None):
+        """Each key takes its value or is added (last of equal keys wins)."""
+        dk, ok = _dev(keys, self.kdt)
+        dv, ov = _dev(vals, self.vdt)
+        try:
+            check(lib().bpt_tree_upsert(self.h, dk.ptr, dv.ptr, dk.n, stream))
+        finally:
+            if ok:
+                dk.free()
+            if ov:
+                dv.free()
+
+    def erase(self, keys, stream=None):
+        """Removes every slot whose key is one of `keys`."""
+        dk, ok = _dev(keys, self.kdt)
+        try:
+            check(lib().bpt_tree_erase(self.h, dk.ptr, dk.n, stream))
+        finally:
+            if ok:
+                dk.free()
 
     def lower_bound(self, q, stream=None):
         dq, own = _dev(q, self.kdt)

@@ -170,6 +170,47 @@ static bpt_status check_sorted(const void *d_keys, uint64_t n, hipStream_t s, bo
     return BPT_OK;
 }
 
+bpt_status tree_rebuild(bpt_tree *t, const void *d_keys, const void *d_vals, uint64_t n, hipStream_t s) {
+    if (t->learned) {
+        learned_free(t->learned);
+        t->learned = nullptr;
+    }
+    (void)hipFree(t->d_leaves);
+    (void)hipFree(t->d_inner);
+    t->d_leaves = t->d_inner = nullptr;
+    t->n = n;
+    bpt_status st = compute_layout(t);
+    if (st == BPT_OK && t->leaf_bytes) {
+        hipError_t e = hipMalloc(&t->d_leaves, t->leaf_bytes);
+        if (e != hipSuccess) st = hip_fail(e, "hipMalloc(leaves)", __FILE__, __LINE__);
+    }
+    if (st == BPT_OK && t->inner_bytes) {
+        hipError_t e = hipMalloc(&t->d_inner, t->inner_bytes);
+        if (e != hipSuccess) st = hip_fail(e, "hipMalloc(inner)", __FILE__, __LINE__);
+    }
+    t->view.inner = t->d_inner;
+    t->view.leaves = t->d_leaves;
+    if (st == BPT_OK) {
+        st = dispatch_tree(t, [&](auto kt, auto vt, auto fc, auto lc) {
+            using K = typename decltype(kt)::type;
+            using V = typename decltype(vt)::type;
+            return build_impl<K, V, decltype(fc)::value, decltype(lc)::value>(t, d_keys, d_vals, s);
+        });
+    }
+    if (st == BPT_OK) {
+        hipError_t e = hipStreamSynchronize(s);
+        if (e != hipSuccess) st = hip_fail(e, "rebuild sync", __FILE__, __LINE__);
+    }
+    if (st != BPT_OK) {  // leave an empty, usable tree behind
+        (void)hipFree(t->d_leaves);
+        (void)hipFree(t->d_inner);
+        t->d_leaves = t->d_inner = nullptr;
+        t->n = 0;
+        (void)compute_layout(t);
+    }
+    return st;
+}
+
 }  // namespace bpt
 
 using namespace bpt;

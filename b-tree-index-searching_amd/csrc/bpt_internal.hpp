@@ -625,5 +625,8 @@ bpt_status point_learned(const bpt_tree *t, const void *d_q, uint64_t nq, void *
                          void *d_ws, uint64_t ws_bytes, hipStream_t s);
 uint64_t point_learned_ws(const bpt_tree *t, uint64_t nq);
 void learned_free(LearnedModel *m);
+// Replaces the tree's contents with n sorted (key, value) pairs on the device,
+// keeping its widths, shape and options; drops a learned model. Synchronizes s.
+bpt_status tree_rebuild(bpt_tree *t, const void *d_keys, const void *d_vals, uint64_t n, hipStream_t s);
 
 }  // namespace bpt

@@ -1,0 +1,243 @@
+// Batched updates of a built tree: upsert (key, value) pairs, erase keys.
+//
+// No reference counterpart: the reference's pages carry previous/next links
+// and a bump allocator (BPTree.h:14-23, BPTree.cu:164) that suggest updates,
+// but it never changes a built tree. Here a batch is merged with the tree's
+// sorted slots on the device and the flat tree is rebuilt from the merge
+// (one level-parallel build, a few milliseconds at 64M keys), so every search
+// form keeps its layout and its speed:
+//   1. the tree's slots are read out of the leaves in key order;
+//   2. the batch is copied and stably sorted by key; for upserts only the last
+//      pair of each run of equal keys is kept (the last in the caller's order);
+//   3. a slot survives when its key is not in the batch; its place in the
+//      result is its rank among the surviving slots plus the number of batch
+//      keys below it, and an upserted pair's place is its rank in the batch plus
+//      the number of surviving slots below it (a merge with no ties left);
+//   4. the tree is rebuilt from the result with its own options.
+#include <algorithm>
+#include <vector>
+
+#include "bpt_internal.hpp"
+
+namespace bpt {
+
+template <class K, class V, int LF>
+__global__ __launch_bounds__(256) void k_extract_slots(TreeView t, K *__restrict__ keys, V *__restrict__ vals) {
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < t.n; i += step) {
+        keys[i] = slot_key<K, V, LF>(t, i);
+        vals[i] = slot_val<K, V, LF>(t, i);
+    }
+}
+
+// flag[j] = 1 for the last pair of every run of equal keys in sorted b.
+template <class K>
+__global__ __launch_bounds__(256) void k_run_last(const K *__restrict__ b, uint64_t m, uint32_t *__restrict__ flag) {
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m; j += step)
+        flag[j] = (j + 1 == m || b[j] != b[j + 1]) ? 1u : 0u;
+}
+
+template <class K, class V>
+__global__ __launch_bounds__(256) void k_compact(const K *__restrict__ k, const V *__restrict__ v,
+                                                 const uint32_t *__restrict__ flag, const uint32_t *__restrict__ pos,
+                                                 uint64_t m, K *__restrict__ ko, V *__restrict__ vo) {
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m; j += step)
+        if (flag[j]) {
+            ko[pos[j]] = k[j];
+            if (vo) vo[pos[j]] = v[j];
+        }
+}
+
+template <class K>
+__device__ __forceinline__ uint64_t lower_bound_g(const K *__restrict__ a, uint64_t n, K q) {
+    uint64_t lo = 0, len = n;
+    while (len > 0) {
+        const uint64_t half = len >> 1;
+        if (a[lo + half] < q) {
+            lo += half + 1;
+            len -= half + 1;
+        } else {
+            len = half;
+        }
+    }
+    return lo;
+}
+
+// keep[i] = 1 when slot i's key is not among the m sorted batch keys.
+template <class K>
+__global__ __launch_bounds__(256) void k_survivors(const K *__restrict__ a, uint64_t n, const K *__restrict__ b,
+                                                   uint64_t m, uint32_t *__restrict__ keep) {
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) {
+        const uint64_t p = lower_bound_g<K>(b, m, a[i]);
+        keep[i] = (p < m && b[p] == a[i]) ? 0u : 1u;
+    }
+}
+
+template <class K, class V>
+__global__ __launch_bounds__(256) void k_place_slots(const K *__restrict__ a, const V *__restrict__ av, uint64_t n,
+                                                     const uint32_t *__restrict__ keep, const uint32_t *__restrict__ rank,
+                                                     const K *__restrict__ b, uint64_t m, K *__restrict__ ko,
+                                                     V *__restrict__ vo) {
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step)
+        if (keep[i]) {
+            const uint64_t p = (uint64_t)rank[i] + lower_bound_g<K>(b, m, a[i]);
+            ko[p] = a[i];
+            vo[p] = av[i];
+        }
+}
+
+// Upserted pair j goes after every surviving slot with a smaller key.
+template <class K, class V>
+__global__ __launch_bounds__(256) void k_place_batch(const K *__restrict__ b, const V *__restrict__ bv, uint64_t m,
+                                                     const K *__restrict__ a, uint64_t n,
+                                                     const uint32_t *__restrict__ rank, uint64_t survivors,
+                                                     K *__restrict__ ko, V *__restrict__ vo) {
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m; j += step) {
+        const uint64_t q = lower_bound_g<K>(a, n, b[j]);
+        const uint64_t p = j + (q < n ? (uint64_t)rank[q] : survivors);
+        ko[p] = b[j];
+        vo[p] = bv[j];
+    }
+}
+
+namespace {
+
+// Device buffers of one update, freed together.
+struct Bufs {
+    std::vector<void *> p;
+    ~Bufs() {
+        for (void *x : p) (void)hipFree(x);
+    }
+    template <class T>
+    bpt_status get(uint64_t count, T **out) {
+        void *x = nullptr;
+        BPT_HIP(hipMalloc(&x, std::max<uint64_t>(count, 1) * sizeof(T)));
+        p.push_back(x);
+        *out = static_cast<T *>(x);
+        return BPT_OK;
+    }
+};
+
+uint32_t grid_for(uint64_t n) {
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, (uint64_t)device_cu_count() * 16));
+}
+
+// Exclusive scan of n u32 flags into rank; returns the total in *total.
+bpt_status scan_flags(const uint32_t *flag, uint64_t n, uint32_t *rank, uint64_t *total, Bufs &b, hipStream_t s) {
+    *total = 0;
+    if (n == 0) return BPT_OK;
+    const uint64_t tb = scan_tmp_bytes(n, 4);
+    uint8_t *tmp = nullptr;
+    BPT_TRY(b.get<uint8_t>(tb, &tmp));
+    BPT_TRY(exclusive_scan<uint32_t>(flag, n, rank, tmp, tb, s));
+    uint32_t last_rank = 0, last_flag = 0;
+    BPT_HIP(hipMemcpyAsync(&last_rank, rank + n - 1, 4, hipMemcpyDeviceToHost, s));
+    BPT_HIP(hipMemcpyAsync(&last_flag, flag + n - 1, 4, hipMemcpyDeviceToHost, s));
+    BPT_HIP(hipStreamSynchronize(s));
+    *total = (uint64_t)last_rank + last_flag;
+    return BPT_OK;
+}
+
+template <class K, class V, int LF>
+bpt_status update_impl(bpt_tree *t, const void *d_keys, const void *d_vals, uint64_t m, bool upsert, hipStream_t s) {
+    Bufs b;
+    const uint64_t n = t->n;
+    // 1. the tree's slots in key order
+    K *a = nullptr;
+    V *av = nullptr;
+    BPT_TRY(b.get<K>(n, &a));
+    BPT_TRY(b.get<V>(n, &av));
+    if (n) {
+        k_extract_slots<K, V, LF><<<grid_for(n), 256, 0, s>>>(t->view, a, av);
+        BPT_HIP(hipGetLastError());
+    }
+    // 2. the batch, sorted (stable), deduplicated for upserts
+    K *bk = nullptr;
+    V *bv = nullptr;  // erase: keys only
+    BPT_TRY(b.get<K>(m, &bk));
+    BPT_HIP(hipMemcpyAsync(bk, d_keys, m * sizeof(K), hipMemcpyDeviceToDevice, s));
+    if (upsert) {
+        BPT_TRY(b.get<V>(m, &bv));
+        BPT_HIP(hipMemcpyAsync(bv, d_vals, m * sizeof(V), hipMemcpyDeviceToDevice, s));
+    }
+    BPT_TRY(bpt_sort_pairs(bk, bv, sizeof(K), sizeof(V), m, reinterpret_cast<bpt_stream>(s)));
+    uint64_t mb = m;
+    if (upsert && m > 1) {
+        uint32_t *flag = nullptr, *pos = nullptr;
+        BPT_TRY(b.get<uint32_t>(m, &flag));
+        BPT_TRY(b.get<uint32_t>(m, &pos));
+        k_run_last<K><<<grid_for(m), 256, 0, s>>>(bk, m, flag);
+        BPT_HIP(hipGetLastError());
+        BPT_TRY(scan_flags(flag, m, pos, &mb, b, s));
+        K *ck = nullptr;
+        V *cv = nullptr;
+        BPT_TRY(b.get<K>(mb, &ck));
+        BPT_TRY(b.get<V>(mb, &cv));
+        k_compact<K, V><<<grid_for(m), 256, 0, s>>>(bk, bv, flag, pos, m, ck, cv);
+        BPT_HIP(hipGetLastError());
+        bk = ck;
+        bv = cv;
+    }
+    // 3. survivors and the merge
+    uint32_t *keep = nullptr, *rank = nullptr;
+    BPT_TRY(b.get<uint32_t>(n, &keep));
+    BPT_TRY(b.get<uint32_t>(n, &rank));
+    uint64_t survivors = 0;
+    if (n) {
+        k_survivors<K><<<grid_for(n), 256, 0, s>>>(a, n, bk, mb, keep);
+        BPT_HIP(hipGetLastError());
+        BPT_TRY(scan_flags(keep, n, rank, &survivors, b, s));
+    }
+    const uint64_t total = survivors + (upsert ? mb : 0);
+    K *ok = nullptr;
+    V *ov = nullptr;
+    BPT_TRY(b.get<K>(total, &ok));
+    BPT_TRY(b.get<V>(total, &ov));
+    if (n) {
+        // erased keys take no places: only upserted keys count below a slot
+        k_place_slots<K, V><<<grid_for(n), 256, 0, s>>>(a, av, n, keep, rank, bk, upsert ? mb : 0, ok, ov);
+        BPT_HIP(hipGetLastError());
+    }
+    if (upsert && mb) {
+        k_place_batch<K, V><<<grid_for(mb), 256, 0, s>>>(bk, bv, mb, a, n, rank, survivors, ok, ov);
+        BPT_HIP(hipGetLastError());
+    }
+    // 4. rebuild with the tree's options
+    return tree_rebuild(t, ok, ov, total, s);
+}
+
+bpt_status update(bpt_tree *t, const void *d_keys, const void *d_vals, uint64_t m, bool upsert, bpt_stream stream) {
+    BPT_CHECK(t, BPT_ERR_INVALID, "tree is NULL");
+    BPT_CHECK(m == 0 || d_keys, BPT_ERR_INVALID, "keys are NULL");
+    BPT_CHECK(!upsert || m == 0 || d_vals, BPT_ERR_INVALID, "values are NULL");
+    BPT_CHECK(t->n + m < (1ull << 32), BPT_ERR_CAPACITY, "updated tree would exceed 2^32 - 1 keys");
+    if (m == 0) return BPT_OK;
+    hipStream_t s = as_stream(stream);
+    return dispatch_tree(t, [&](auto kt, auto vt, auto, auto lc) -> bpt_status {
+        using K = typename decltype(kt)::type;
+        using V = typename decltype(vt)::type;
+        return update_impl<K, V, decltype(lc)::value>(t, d_keys, d_vals, m, upsert, s);
+    });
+}
+
+}  // namespace
+}  // namespace bpt
+
+using namespace bpt;
+
+extern "C" {
+
+bpt_status bpt_tree_upsert(bpt_tree *t, const void *d_keys, const void *d_vals, uint64_t n, bpt_stream s) {
+    return update(t, d_keys, d_vals, n, true, s);
+}
+
+bpt_status bpt_tree_erase(bpt_tree *t, const void *d_keys, uint64_t n, bpt_stream s) {
+    return update(t, d_keys, nullptr, n, false, s);
+}
+
+}  // extern "C"

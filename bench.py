@@ -478,6 +478,22 @@ def main():
                                  "speedup_vs_plain": wall_pz / wall_gz,
                                  "mismatches_vs_plain": int((dout.to_host() != plain_z).sum())}
         dqz.free()
+        # batched update: 2^20 upserts (half of them existing keys) merged into
+        # the tree, which is rebuilt; wall time of the whole call (allocations included)
+        rng_u = np.random.default_rng(7)
+        nu = 1 << 20
+        ku = np.concatenate([keys[rng_u.integers(0, n, nu // 2)],
+                             rng_u.integers(0, 1 << 32, nu - nu // 2, dtype=np.uint64).astype(np.uint32)])
+        vu = rng_u.integers(0, 1 << 32, nu, dtype=np.uint64).astype(np.uint32)
+        dku, dvu = bptree.DeviceArray.from_host(ku), bptree.DeviceArray.from_host(vu)
+        bptree.sync()
+        t0 = time.perf_counter()
+        tree.upsert(dku, dvu)
+        wall_u = time.perf_counter() - t0
+        forms["upsert_2^20"] = {"ms": wall_u * 1e3, "pairs_per_s": nu / wall_u, "keys_before": n,
+                                "keys_after": tree.info()["n"]}
+        dku.free()
+        dvu.free()
         log("forms: " + json.dumps(forms))
 
     traffic = None

@@ -112,6 +112,18 @@ bpt_status bpt_tree_get_layout(const bpt_tree *t, const void **d_leaves, const v
  * blocks per CU of the persistent grid (0 keeps the default). */
 bpt_status bpt_tree_set_launch(bpt_tree *t, uint32_t block, uint32_t blocks_per_cu);
 
+/* Batched updates (no reference counterpart: its pages carry next/previous
+ * links and a bump allocator, BPTree.h:14-23 and BPTree.cu:164, but a built
+ * tree is never changed). Device arrays of the tree's key and value widths.
+ * bpt_tree_upsert: each key takes its new value, or is added; of equal keys
+ * in the batch the last one (in array order) wins. bpt_tree_erase: every slot
+ * whose key is one of d_keys is removed (absent keys are ignored). The batch
+ * is merged with the tree's slots on the device and the tree is rebuilt with
+ * its own options (O(tree + batch) per call); a trained learned model is
+ * dropped. Synchronize s. On failure the tree is left empty. */
+bpt_status bpt_tree_upsert(bpt_tree *t, const void *d_keys, const void *d_vals, uint64_t n, bpt_stream s);
+bpt_status bpt_tree_erase(bpt_tree *t, const void *d_keys, uint64_t n, bpt_stream s);
+
 /* Search-kernel options: "coop" (1 = several lanes share each node, the
  * default when key and value widths match; 0 = one lane per query),
  * "lanes" (lanes per query of the cooperative descent: 8, the default, or 4),
