@@ -379,13 +379,16 @@ __global__ __launch_bounds__(256) void k_point_learned(TreeView t, LearnedView v
 
 // Cooperative form (equal key/value widths). Phase A, one lane per query:
 // route, model and window [lo, hi] in slots -> leaf range [la, lb]; queries of
-// fallback models take the plain descent here (one lane each). Phase B,
-// eight lanes per query, the group's eight queries in flight together: the
-// level-1 separators (leaf maxima, one flat array) of leaves la..la+31 are
-// ranked with four per lane, which picks the leaf (wider windows continue
-// 32 leaves at a time); the leaves are then matched as in the plain
-// cooperative descent. The fallback descent reads every level from global
-// memory, so the kernel's LDS holds only the root's tables.
+// fallback models are queued for the plain descent. Phase B, eight lanes per
+// query, the group's eight queries in flight together: the window's level-1
+// nodes [la/F, lb/F] are ranked by their maxima (the level-2 entries, two per
+// lane: a window of 256 slots spans a few nodes, one line), which picks the
+// node; one cooperative step through that node picks the leaf, and the leaves
+// are matched as in the plain cooperative descent. A window thus costs one
+// line of node maxima instead of a line per 16 leaves of leaf maxima. Trees
+// of one internal level rank the leaf maxima over [la, lb] directly. The
+// fallback descent reads every level from global memory, so the kernel's LDS
+// holds only the root's tables.
 template <class K, class V, int F, int LF>
 __global__ __launch_bounds__(256) void k_point_learned_coop(TreeView t, LearnedView v, const K *__restrict__ q,
                                                             V *__restrict__ out, uint64_t nq, V sentinel,
@@ -440,35 +443,86 @@ __global__ __launch_bounds__(256) void k_point_learned_coop(TreeView t, LearnedV
             a[k] = (uint32_t)__shfl((int)la, (int)(g * 8 + k), 64);
             b[k] = (uint32_t)__shfl((int)lb, (int)(g * 8 + k), 64);
         }
-        // phase B.1: rank among leaf maxima (all eight queries' loads in flight)
-        K sv[8][SPL];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-#pragma unroll
-            for (uint32_t j = 0; j < SPL; ++j) {
-                const uint32_t l = a[k] + sub * SPL + j;
-                sv[k][j] = (l <= b[k] && l < nleaves) ? seps[l] : key_max<K>();
-            }
-        }
         uint32_t leaf[8];
+        if (t.height >= 2) {
+            // phase B.1: the window's level-1 nodes ranked by their maxima (the
+            // level-2 entries: a few, in one line), then one cooperative step
+            // through the chosen level-1 node, as in the plain descent
+            constexpr uint32_t EPL = 2;  // node maxima per lane, 16 per group step
+            constexpr int NB = F * (int)sizeof(K) / 8;
+            const K *nmax = reinterpret_cast<const K *>(t.inner + t.off[2]);
+            const uint32_t nn = (uint32_t)t.count[1];
+            K ev[8][EPL];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            uint32_t c = 0;
+            for (int k = 0; k < 8; ++k) {
+                const uint32_t ja = a[k] / F, jb = b[k] / F;
 #pragma unroll
-            for (uint32_t j = 0; j < SPL; ++j) c += (uint32_t)(sv[k][j] < qk[k]) & (uint32_t)(a[k] + sub * SPL + j <= b[k]);
-            uint32_t r = group8_sum(c);
-            uint32_t aa = a[k];
-            while (r == 8 * SPL && aa + 8 * SPL <= b[k]) {  // windows wider than 64 leaves
-                aa += 8 * SPL;
-                c = 0;
-                for (uint32_t j = 0; j < SPL; ++j) {
-                    const uint32_t l = aa + sub * SPL + j;
-                    c += (l <= b[k] && l < nleaves) ? (uint32_t)(seps[l] < qk[k]) : 0u;
+                for (uint32_t j = 0; j < EPL; ++j) {
+                    const uint32_t e = ja + sub * EPL + j;
+                    ev[k][j] = (a[k] <= b[k] && e <= jb && e < nn) ? nmax[e] : key_max<K>();
                 }
-                r = group8_sum(c);
             }
-            const uint32_t lf = aa + r;
-            leaf[k] = a[k] > b[k] ? 0u : (lf > b[k] ? b[k] : lf);
+            uint32_t node[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const uint32_t ja = a[k] / F, jb = b[k] / F;
+                uint32_t c = 0;
+#pragma unroll
+                for (uint32_t j = 0; j < EPL; ++j) c += (uint32_t)(ev[k][j] < qk[k]) & (uint32_t)(ja + sub * EPL + j <= jb);
+                uint32_t r = group8_sum(c);
+                uint32_t jj = ja;
+                while (r == 8 * EPL && jj + 8 * EPL <= jb) {  // windows over more than 16 level-1 nodes
+                    jj += 8 * EPL;
+                    c = 0;
+                    for (uint32_t j = 0; j < EPL; ++j) {
+                        const uint32_t e = jj + sub * EPL + j;
+                        c += (e <= jb && e < nn) ? (uint32_t)(nmax[e] < qk[k]) : 0u;
+                    }
+                    r = group8_sum(c);
+                }
+                const uint32_t nd = jj + r;
+                node[k] = a[k] > b[k] ? 0u : (nd > jb ? jb : nd);
+            }
+            const uint8_t *lv = t.inner + t.off[1];
+            Chunk<NB> c1[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) load_chunk<NB>(lv + (uint64_t)node[k] * (F * sizeof(K)) + sub * NB, c1[k]);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const uint32_t ch = node[k] * F + group8_sum(chunk_count_lt<K, NB>(c1[k], qk[k]));
+                leaf[k] = ch < nleaves - 1 ? ch : nleaves - 1;
+            }
+        } else {
+            // phase B.1 of a one-level tree: rank among the leaf maxima over the
+            // window (all eight queries' loads in flight)
+            K sv[8][SPL];
+    #pragma unroll
+            for (int k = 0; k < 8; ++k) {
+    #pragma unroll
+                for (uint32_t j = 0; j < SPL; ++j) {
+                    const uint32_t l = a[k] + sub * SPL + j;
+                    sv[k][j] = (l <= b[k] && l < nleaves) ? seps[l] : key_max<K>();
+                }
+            }
+    #pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                uint32_t c = 0;
+    #pragma unroll
+                for (uint32_t j = 0; j < SPL; ++j) c += (uint32_t)(sv[k][j] < qk[k]) & (uint32_t)(a[k] + sub * SPL + j <= b[k]);
+                uint32_t r = group8_sum(c);
+                uint32_t aa = a[k];
+                while (r == 8 * SPL && aa + 8 * SPL <= b[k]) {  // windows wider than 64 leaves
+                    aa += 8 * SPL;
+                    c = 0;
+                    for (uint32_t j = 0; j < SPL; ++j) {
+                        const uint32_t l = aa + sub * SPL + j;
+                        c += (l <= b[k] && l < nleaves) ? (uint32_t)(seps[l] < qk[k]) : 0u;
+                    }
+                    r = group8_sum(c);
+                }
+                const uint32_t lf = aa + r;
+                leaf[k] = a[k] > b[k] ? 0u : (lf > b[k] ? b[k] : lf);
+            }
         }
         // phase B.2: leaf lines (an empty window, la > lb, is a miss)
         const V got = coop_leaf<K, V, LF, true, kOpValue, 8, false>(t, qk, leaf, sentinel);
