@@ -306,6 +306,11 @@ def bench_c5(args):
         log(f"c5 {name}: {wall * 1e3:.2f} ms, {nq / wall / 1e9:.2f} G lookups/s")
     forms["learned"]["speedup_vs_plain"] = forms["plain"]["kernel_ms"] / forms["learned"]["kernel_ms"]
     forms["learned"]["mismatches_vs_plain"] = int((outs["plain"] != outs["learned"]).sum())
+    if args.learned_only:
+        print(json.dumps({"form": "c5_lookups", "forms": forms, "learned_model": lst, "steps": args.steps}),
+              flush=True)
+        tree.close()
+        return
     # layer-by-layer root evaluation, for comparison with the piecewise-linear form
     tree.learned_build(leaf_models=65536, max_window=256, train_samples=1 << 20, train_steps=2000, root=1)
     wall, ev = time_form(tree, dq, dout, bptree.LEARNED, max(2, args.steps // 2), 1)
@@ -399,6 +404,8 @@ def main():
                     help="time only the grouped form on the uniform set (for profiling its kernels)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run the 1 GiB device copy kernel (HBM counter calibration)")
+    ap.add_argument("--learned-only", action="store_true",
+                    help="with --config c5: only the plain and learned lookups (for profiling)")
     ap.add_argument("--config", default="c2", choices=["c2", "c3", "c4", "c5"],
                     help="c2 (default): 64M-key point search; c3: 2-D rectangles; c4: 4096-tree batch; "
                          "c5: learned index + serving")
