@@ -205,9 +205,12 @@ __global__ void k_morton(const uint32_t *__restrict__ x, const uint32_t *__restr
         z[i] = spread32(x[i]) | (spread32(y[i]) << 1);
 }
 
+// Maxima of a and b: wave shuffles, then the block's waves through LDS, then
+// one atomic per block (atomics of every wave on two words serialize).
 template <class KT>
-__global__ void k_max2(const KT *__restrict__ a, const KT *__restrict__ b, uint64_t n,
-                       unsigned long long *__restrict__ out) {
+__global__ __launch_bounds__(256) void k_max2(const KT *__restrict__ a, const KT *__restrict__ b, uint64_t n,
+                                              unsigned long long *__restrict__ out) {
+    __shared__ unsigned long long wa[4], wb[4];
     const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
     unsigned long long ma = 0, mb = 0;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) {
@@ -218,7 +221,17 @@ __global__ void k_max2(const KT *__restrict__ a, const KT *__restrict__ b, uint6
         ma = std::max<unsigned long long>(ma, __shfl_xor(ma, d, 64));
         mb = std::max<unsigned long long>(mb, __shfl_xor(mb, d, 64));
     }
+    const uint32_t w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
+        wa[w] = ma;
+        wb[w] = mb;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (uint32_t k = 1; k < blockDim.x / 64; ++k) {
+            ma = std::max<unsigned long long>(ma, wa[k]);
+            mb = std::max<unsigned long long>(mb, wb[k]);
+        }
         atomicMax(&out[0], ma);
         atomicMax(&out[1], mb);
     }
@@ -253,7 +266,7 @@ __global__ void k_group2d_keys(const KT *__restrict__ a, const KT *__restrict__ 
     }
 }
 
-static int bitwidth(uint64_t v) { int b = 0; while (v) { v >>= 1; ++b; } return b; }
+__host__ __device__ static int bitwidth(uint64_t v) { int b = 0; while (v) { v >>= 1; ++b; } return b; }
 
 template <class KT>
 bpt_status pair_max(const KT *a, const KT *b, uint64_t n, uint64_t *ma, uint64_t *mb, hipStream_t s) {
@@ -262,7 +275,7 @@ bpt_status pair_max(const KT *a, const KT *b, uint64_t n, uint64_t *ma, uint64_t
     unsigned long long h[2] = {0, 0};
     hipError_t e = hipMemsetAsync(d, 0, 16, s);
     if (e == hipSuccess) {
-        uint32_t g = (uint32_t)std::min<uint64_t>((n + 255) / 256, (uint64_t)device_cu_count() * 8);
+        uint32_t g = (uint32_t)std::min<uint64_t>((n + 255) / 256, (uint64_t)device_cu_count() * 2);
         k_max2<KT><<<std::max<uint32_t>(g, 1), 256, 0, s>>>(a, b, n, d);
         e = hipGetLastError();
     }
@@ -275,18 +288,49 @@ bpt_status pair_max(const KT *a, const KT *b, uint64_t n, uint64_t *ma, uint64_t
     return BPT_OK;
 }
 
-// zOrder2d.cu:129-130
-static uint64_t zorder_interval(uint64_t m) {
-    float r = (float)m / 4294967296.0f;
-    int iv = (int)ceilf(r);
+// zOrder2d.cu:129-130: ceil of max / 2^32 in float, as an int. A quotient of
+// 2^31 or more does not fit the int; the reference's host (x86) conversion
+// then yields INT_MIN, which the floor of 1 turns into 1. That outcome is
+// written out here, so host and device (whose conversion saturates) agree.
+__host__ __device__ static uint64_t zorder_interval(uint64_t m) {
+    const float r = (float)m / 4294967296.0f;
+    if (r >= 2147483648.0f) return 1;
+    const int iv = (int)ceilf(r);
     return iv < 1 ? 1 : (uint64_t)iv;
 }
 
 // grouping2d.cu:610-635
-static int group2d_start_pass(uint64_t ma, uint64_t mb) {
+__host__ __device__ static int group2d_start_pass(uint64_t ma, uint64_t mb) {
     int a = 16 - (int)ceilf((float)bitwidth(ma) / 3.0f);
     int b = 16 - (int)ceilf((float)bitwidth(mb) / 3.0f);
     return a > b ? a : b;
+}
+
+// The same keys with their parameters taken from the maxima in device memory
+// (mx[0], mx[1], written by k_max2), so no host round trip sits between.
+template <class KT>
+__global__ void k_zorder_keys_dev(const KT *__restrict__ a, const KT *__restrict__ b, uint64_t n,
+                                  const unsigned long long *__restrict__ mx, uint64_t *__restrict__ out) {
+    const uint64_t ix = zorder_interval(mx[0]), iy = zorder_interval(mx[1]);
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) {
+        const uint32_t xg = (uint32_t)((uint64_t)a[i] / ix);
+        const uint32_t yg = (uint32_t)((uint64_t)b[i] / iy);
+        out[i] = spread32(xg) | (spread32(yg) << 1);
+    }
+}
+
+template <class KT>
+__global__ void k_group2d_keys_dev(const KT *__restrict__ a, const KT *__restrict__ b, uint64_t n,
+                                   const unsigned long long *__restrict__ mx, uint32_t *__restrict__ out) {
+    int p0 = group2d_start_pass(mx[0], mx[1]);
+    if (p0 < 0) p0 = 0;
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) {
+        const uint64_t x = a[i], y = b[i];
+        out[i] = (bucket2d(x, y, p0) << 24) | (bucket2d(x, y, p0 + 1) << 16) | (bucket2d(x, y, p0 + 2) << 8) |
+                 bucket2d(x, y, p0 + 3);
+    }
 }
 
 template <class KT>
@@ -319,29 +363,44 @@ template bpt_status group2d_keys<uint32_t>(const uint32_t *, const uint32_t *, u
 template bpt_status group2d_keys<uint64_t>(const uint64_t *, const uint64_t *, uint64_t, uint32_t *, hipStream_t);
 
 // Order of n (lo, hi) queries under a grouping mode: *order points into ws.
+// Layout: maxima (256 B) | keys (n * 8) | sort scratch.
 uint64_t query_order_ws(uint64_t n) {
-    return align256(n * 8) + sort_ws_bytes(n, 4);
+    return 256 + align256(n * 8) + sort_ws_bytes(n, 4);
 }
 
 template <class KT>
 bpt_status query_order(const KT *lo, const KT *hi, uint64_t n, int mode, void *ws, uint64_t ws_bytes,
                        uint32_t **order, hipStream_t s) {
     BPT_CHECK(ws_bytes >= query_order_ws(n), BPT_ERR_INVALID, "order workspace too small");
-    uint8_t *p = reinterpret_cast<uint8_t *>(ws);
+    BPT_CHECK(mode == BPT_GROUP_GRID2D || mode == BPT_GROUP_ZORDER, BPT_ERR_INVALID, "unknown group mode %d", mode);
+    unsigned long long *mx = reinterpret_cast<unsigned long long *>(ws);
+    uint8_t *p = reinterpret_cast<uint8_t *>(ws) + 256;
     uint8_t *sort_ws = p + align256(n * 8);
+    ws_bytes -= 256;
     uint32_t *ksorted = nullptr;
+    *order = nullptr;
+    if (n == 0) return BPT_OK;
+    const uint32_t g = (uint32_t)std::min<uint64_t>((n + 255) / 256, (uint64_t)device_cu_count() * 16);
+    BPT_HIP(hipMemsetAsync(mx, 0, 16, s));
+    k_max2<KT><<<(uint32_t)std::min<uint64_t>(g, (uint64_t)device_cu_count() * 2), 256, 0, s>>>(lo, hi, n, mx);
+    BPT_HIP(hipGetLastError());
     if (mode == BPT_GROUP_GRID2D) {
         uint32_t *keys = reinterpret_cast<uint32_t *>(p);
-        BPT_TRY(group2d_keys<KT>(lo, hi, n, keys, s));
-        return sort_pairs<uint32_t>(keys, nullptr, n, 0, 32, sort_ws, ws_bytes - align256(n * 8), &ksorted,
+        k_group2d_keys_dev<KT><<<g, 256, 0, s>>>(lo, hi, n, mx, keys);
+        BPT_HIP(hipGetLastError());
+        // the first two passes' buckets (the top 16 bits) order the groups: 65536
+        // groups are enough for neighbouring lanes to share subtrees, and two
+        // radix passes cost half of four
+        return sort_pairs<uint32_t>(keys, nullptr, n, 16, 32, sort_ws, ws_bytes - align256(n * 8), &ksorted,
                                     order, s);
     }
     if (mode == BPT_GROUP_ZORDER) {
         uint64_t *z = reinterpret_cast<uint64_t *>(p);
-        BPT_TRY(zorder_keys<KT>(lo, hi, n, z, s));
-        // the top 32 bits of the 64-bit Z key order the groups
+        k_zorder_keys_dev<KT><<<g, 256, 0, s>>>(lo, hi, n, mx, z);
+        BPT_HIP(hipGetLastError());
+        // the top 16 bits of the 64-bit Z key order the groups (as above)
         uint64_t *zs = nullptr;
-        return sort_pairs<uint64_t>(z, nullptr, n, 32, 64, sort_ws, ws_bytes - align256(n * 8), &zs, order, s);
+        return sort_pairs<uint64_t>(z, nullptr, n, 48, 64, sort_ws, ws_bytes - align256(n * 8), &zs, order, s);
     }
     set_error("unknown group mode %d", mode);
     return BPT_ERR_INVALID;
@@ -353,7 +412,7 @@ template bpt_status query_order<uint64_t>(const uint64_t *, const uint64_t *, ui
                                           uint32_t **, hipStream_t);
 
 uint64_t query_order_ws_any(uint64_t n) {
-    return align256(n * 8) + std::max(sort_ws_bytes(n, 4), sort_ws_bytes(n, 8));
+    return 256 + align256(n * 8) + std::max(sort_ws_bytes(n, 4), sort_ws_bytes(n, 8));
 }
 
 }  // namespace bpt

@@ -186,25 +186,23 @@ __global__ void k_rect_zbounds(const uint4 *__restrict__ rects, uint64_t nr, uin
     }
 }
 
-// Grouped query order for rectangles: keys from (Zmin, Zmax).
-static bpt_status rect_order(const bpt_tree *t, const uint32_t *d_rects, uint64_t nr, int mode,
-                             void **ws_out, uint32_t **order, hipStream_t s) {
-    *ws_out = nullptr;
+// Grouped query order for rectangles: keys from (Zmin, Zmax). The scratch
+// space is the tree's own workspace (kept between calls, grown on demand).
+static bpt_status rect_order(bpt_tree *t, const uint32_t *d_rects, uint64_t nr, int mode, uint32_t **order,
+                             hipStream_t s) {
     *order = nullptr;
     if (mode == BPT_GROUP_NONE) return BPT_OK;
     BPT_CHECK(mode == BPT_GROUP_GRID2D || mode == BPT_GROUP_ZORDER, BPT_ERR_INVALID, "group mode %d", mode);
     BPT_CHECK(nr < (1ull << 32), BPT_ERR_CAPACITY, "more than 2^32 rectangles");
     const uint64_t zb = (nr * 8 + 255) / 256 * 256;
     const uint64_t bytes = 2 * zb + query_order_ws_any(nr);
-    void *ws = nullptr;
-    BPT_HIP(hipMalloc(&ws, bytes));
-    *ws_out = ws;
+    BPT_TRY(t->ws.reserve(bytes));
+    void *ws = t->ws.ptr;
     uint64_t *zlo = reinterpret_cast<uint64_t *>(ws);
     uint64_t *zhi = reinterpret_cast<uint64_t *>(reinterpret_cast<uint8_t *>(ws) + zb);
     uint32_t g = (uint32_t)std::min<uint64_t>((nr + 255) / 256, (uint64_t)device_cu_count() * 16);
     k_rect_zbounds<<<std::max<uint32_t>(g, 1), 256, 0, s>>>((const uint4 *)d_rects, nr, zlo, zhi);
     BPT_HIP(hipGetLastError());
-    (void)t;
     return query_order<uint64_t>(zlo, zhi, nr, mode, reinterpret_cast<uint8_t *>(ws) + 2 * zb,
                                  bytes - 2 * zb, order, s);
 }
@@ -218,9 +216,8 @@ static bpt_status rect_run(bpt_tree *t, const uint32_t *d_rects, uint64_t nr, in
         if (d_counts) BPT_HIP(hipMemsetAsync(d_counts, 0, nr * 8, s));
         return BPT_OK;
     }
-    void *ws = nullptr;
     uint32_t *order = nullptr;
-    bpt_status st = rect_order(t, d_rects, nr, mode, &ws, &order, s);
+    bpt_status st = rect_order(t, d_rects, nr, mode, &order, s);
     if (st == BPT_OK) {
         st = dispatch_tree(t, [&](auto, auto, auto fc, auto lc) -> bpt_status {
             constexpr int F = decltype(fc)::value, LF = decltype(lc)::value;
@@ -235,10 +232,6 @@ static bpt_status rect_run(bpt_tree *t, const uint32_t *d_rects, uint64_t nr, in
             BPT_HIP(hipGetLastError());
             return BPT_OK;
         });
-    }
-    if (ws) {
-        (void)hipStreamSynchronize(s);
-        (void)hipFree(ws);
     }
     return st;
 }

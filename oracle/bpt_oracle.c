@@ -308,6 +308,9 @@ void ora_demorton2d(u64 z, uint32_t *x, uint32_t *y) {
  * A zero interval (max == 0) would divide by zero there; treated as 1. */
 int ora_zorder_interval(u64 maxval) {
     float r = (float)maxval / 4294967296.0f;
+    /* a quotient of 2^31 or more does not fit the int: x86 gives INT_MIN,
+     * which the floor below turns into 1; written out to avoid the overflow */
+    if (r >= 2147483648.0f) return 1;
     int iv = (int)ceilf(r);
     return iv < 1 ? 1 : iv;
 }

@@ -153,6 +153,11 @@ def test_zorder_keys_reference_semantics(oracle):
     assert oracle.zorder_interval(1 << 33) == 2
     assert oracle.zorder_interval((1 << 33) + 1) == 2       # float rounding
     assert oracle.zorder_interval(3 << 40) == 3 << 8
+    # a quotient of 2^31 or more overflows the reference's int: the x86 result
+    # (INT_MIN) falls below the floor of 1
+    assert oracle.zorder_interval((1 << 31) << 32) == 1
+    assert oracle.zorder_interval((1 << 64) - 1) == 1
+    assert oracle.zorder_interval((1 << 62)) == 1 << 30
     a = np.array([5, 7, 0xFFFFFFFF], dtype=np.uint64)
     b = np.array([9, 0, 3], dtype=np.uint64)
     got = oracle.zorder_keys(a, b)
