@@ -240,7 +240,9 @@ bpt_status bpt_group2d_keys(const uint64_t *d_a, const uint64_t *d_b, uint64_t n
 /* rects: n x {x0, y0, x1, y1} uint32, inclusive. The tree holds
  * key = Morton(x, y), value = point id. counts[i] = points inside rect i;
  * bpt_rect_ids writes their ids ascending at ids[offsets[i] ...].
- * group_mode orders the queries (bpt_group_mode); results do not change. */
+ * group_mode orders the queries (bpt_group_mode); results do not change.
+ * The grouped orders use the tree's workspace, like bpt_point_search:
+ * concurrent calls on one tree must share one stream. */
 bpt_status bpt_rect_count(bpt_tree *t, const uint32_t *d_rects, uint64_t nr, int group_mode,
                           uint64_t *d_counts, bpt_stream s);
 bpt_status bpt_rect_ids(bpt_tree *t, const uint32_t *d_rects, uint64_t nr, int group_mode,
