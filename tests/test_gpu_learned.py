@@ -87,3 +87,31 @@ def test_learned_requires_model(bpt):
     t = bpt.Tree(keys, keys, 8, 8)
     with pytest.raises(bpt.BptError):
         t.point_search(keys, bpt.LEARNED)
+
+
+def test_workload_c5_full_size(bpt, oracle, wl):
+    """256M unique lognormal uint64 keys, the learned model trained as in the
+    benchmark, 2^27 queries (90 % hits): learned equals plain on every query,
+    and plain equals the oracle on a 2^20-query sample."""
+    n, nq = 1 << 28, 1 << 27
+    keys = wl.lognormal_keys(n, sort=lambda a: bpt.sort_pairs(a)[0])
+    vals = np.arange(n, dtype=np.uint64)
+    print("c5 keys generated", flush=True)
+    t = bpt.Tree(keys, vals, 8, 8)
+    st = t.learned_build(leaf_models=65536, max_window=256, train_samples=1 << 20, train_steps=2000, root=0)
+    assert st["models_used"] > 60000 and st["mean_window"] < 400
+    rng = wl.rng_for(0, 5, 1)
+    q = np.where(rng.random(nq) < 0.9, keys[rng.integers(0, n, nq)],
+                 rng.integers(0, 1 << 50, nq, dtype=np.uint64)).astype(np.uint64)
+    dq = bpt.DeviceArray.from_host(q)
+    out = bpt.DeviceArray(nq, np.uint64)
+    t.point_search_dev(dq, out, bpt.PLAIN)
+    plain = out.to_host()
+    t.point_search_dev(dq, out, bpt.LEARNED)
+    learned = out.to_host()
+    assert int((plain != learned).sum()) == 0
+    s = q[: 1 << 20]
+    assert np.array_equal(plain[: 1 << 20], oracle.point_closed(keys, vals, s, (1 << 64) - 1))
+    dq.free()
+    out.free()
+    t.close()

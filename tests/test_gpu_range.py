@@ -127,3 +127,23 @@ def test_exclusive_scan(bpt):
         off, total = bpt.exclusive_scan(c)
         assert total == int(c.sum())
         assert np.array_equal(off, np.concatenate([[0], np.cumsum(c)[:-1]]).astype(np.uint64))
+
+
+def test_workload_c3_full_size(bpt, oracle, wl):
+    """32M Morton-keyed points, 4M rectangles: the three query orders return
+    identical counts and ids, and both equal the oracle on a sample."""
+    npts, nr = 1 << 25, 1 << 22
+    x, y, ids, keys, kid = wl.points_2d(npts)
+    r = wl.rects(nr)
+    print("c3 generated", flush=True)
+    t = bpt.Tree(keys, kid, 8, 4)
+    c0, o0, i0 = t.rect_search(r, group_mode=MODES[0])
+    for mode in MODES[1:]:
+        c, o, got = t.rect_search(r, group_mode=mode)
+        assert np.array_equal(c, c0) and np.array_equal(got, i0), mode
+    sample = 2048
+    want_c, want_o, want_ids = oracle.rect_morton(keys, kid, r[:sample], threads=8)
+    assert np.array_equal(c0[:sample], want_c)
+    end = int(o0[sample - 1] + c0[sample - 1])
+    assert np.array_equal(i0[:end], want_ids)
+    t.close()
