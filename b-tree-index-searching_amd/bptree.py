@@ -195,11 +195,16 @@ class DeviceArray:
             check(lib().bpt_copy_h2d(d.ptr, a.ctypes.data, d.nbytes, stream))
         return d
 
-    def to_host(self, stream=None, count=None):
-        n = self.n if count is None else int(count)
+    def to_host(self, stream=None, count=None, offset=0):
+        """Elements [offset, offset + count) (all by default) as a numpy array."""
+        offset = int(offset)
+        n = self.n - offset if count is None else int(count)
+        if offset < 0 or n < 0 or offset + n > self.n:
+            raise ValueError(f"range [{offset}, {offset + n}) outside a device array of {self.n}")
         out = np.empty(n, dtype=self.dtype)
         if n:
-            check(lib().bpt_copy_d2h(out.ctypes.data, self.ptr, n * self.dtype.itemsize, stream))
+            check(lib().bpt_copy_d2h(out.ctypes.data, self.ptr + offset * self.dtype.itemsize,
+                                     n * self.dtype.itemsize, stream))
         return out
 
     def fill_bytes(self, value, stream=None):

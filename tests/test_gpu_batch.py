@@ -85,3 +85,42 @@ def test_autotune_returns_fastest_valid_config(bpt, oracle, wl):
     for t, (po, fe, ce) in exp.items():
         assert np.array_equal(out[q_off[t]:q_off[t + 1]], po)
         assert np.array_equal(count[r_off[t]:r_off[t + 1]], ce)
+
+
+def test_workload_c4_full_size(bpt, oracle, wl):
+    """The benchmarked forest: 4096 uint64 trees of 2^10..2^20 keys (632M in
+    all), 4 point queries per key and size/16 ranges per tree, one
+    graph-captured launch; 32 trees (the smallest, the largest and a spread)
+    against the oracle on their slices of the device inputs."""
+    sizes = wl.forest_sizes(4096, 10, 20)
+    key_off, r_off = wl.forest_offsets(sizes)
+    dk, dv, dq, dlo, dhi = bpt.synth_forest(0, key_off, qpk=4, hit_pct=80, r_off=r_off)
+    q_off = key_off * np.uint64(4)
+    nq, nr = int(q_off[-1]), int(r_off[-1])
+    print(f"c4 generated: {int(key_off[-1])} keys, {nq} points, {nr} ranges", flush=True)
+    f = bpt.Forest(dk, dv, key_off, fanout=16, lds_bytes=32768)
+    dout = bpt.DeviceArray(nq, np.uint64)
+    dfirst = bpt.DeviceArray(nr, np.uint64)
+    dcount = bpt.DeviceArray(nr, np.uint64)
+    dout.fill_bytes(0)
+    dcount.fill_bytes(0xAB)
+    b = bpt.Batch(f, dq, q_off, dout, dlo, dhi, r_off, dfirst, dcount, chunk=2048)
+    b.run(graph=True)
+    bpt.sync()
+    rng = np.random.default_rng(11)
+    trees = sorted({int(np.argmin(sizes)), int(np.argmax(sizes)), *rng.choice(len(sizes), 30, replace=False).tolist()})
+    hits = []
+    for t in trees:
+        k0, k1 = int(key_off[t]), int(key_off[t + 1])
+        q0, q1 = int(q_off[t]), int(q_off[t + 1])
+        r0, r1 = int(r_off[t]), int(r_off[t + 1])
+        k, v = dk.to_host(offset=k0, count=k1 - k0), dv.to_host(offset=k0, count=k1 - k0)
+        q = dq.to_host(offset=q0, count=q1 - q0)
+        want = oracle.point_closed(k, v, q, SENT)
+        assert np.array_equal(dout.to_host(offset=q0, count=q1 - q0), want), t
+        fe, ce = oracle.range_closed(k, dlo.to_host(offset=r0, count=r1 - r0), dhi.to_host(offset=r0, count=r1 - r0))
+        assert np.array_equal(dfirst.to_host(offset=r0, count=r1 - r0), fe), t
+        assert np.array_equal(dcount.to_host(offset=r0, count=r1 - r0), ce), t
+        hits.append(np.mean(want != SENT))
+    assert 0.75 < float(np.mean(hits)) < 0.85
+    b.close()
