@@ -514,9 +514,17 @@ static BucketPlan bucket_plan(const bpt_tree *t, uint64_t nq) {
     d.tile = t->key_bytes == 4 ? tile_of<uint32_t>() : tile_of<uint64_t>();
     d.ntiles = (uint32_t)std::max<uint64_t>(1, (nq + d.tile - 1) / d.tile);
     d.ngroups = (d.ntiles + kColGroup - 1) / kColGroup;
-    d.chunk = t->group_chunk;
-    const uint64_t nchunks = std::max<uint64_t>(1, (nq + d.chunk - 1) / d.chunk);
     const uint64_t cap = (uint64_t)device_cu_count() * t->group_blocks;
+    if (t->group_chunk) {
+        d.chunk = t->group_chunk;
+    } else {
+        // about 16 chunks per search block: long enough runs of one bucket per
+        // slice staging, enough chunks for every block (32768 at 2^28 queries)
+        uint64_t c = 1024;
+        while (c < 32768 && c * 2 * cap * 16 <= nq) c *= 2;
+        d.chunk = (uint32_t)c;
+    }
+    const uint64_t nchunks = std::max<uint64_t>(1, (nq + d.chunk - 1) / d.chunk);
     p.grid = (uint32_t)std::min(cap, nchunks);
     // split per XCD only when every XCD gets whole strides of chunks
     d.xcd = (t->group_xcd && p.grid % kXcds == 0 && nchunks >= 2ull * p.grid) ? 1u : 0u;

@@ -339,8 +339,8 @@ bpt_status bpt_tree_set_option(bpt_tree *t, const char *key, int64_t value) {
         return BPT_OK;
     }
     if (strcmp(key, "group_chunk") == 0) {
-        BPT_CHECK(value >= 256 && value <= (1 << 20) && value % 64 == 0, BPT_ERR_INVALID,
-                  "group_chunk must be a multiple of 64 in [256, 2^20]");
+        BPT_CHECK(value == 0 || (value >= 256 && value <= (1 << 20) && value % 64 == 0), BPT_ERR_INVALID,
+                  "group_chunk must be 0 (by batch size) or a multiple of 64 in [256, 2^20]");
         t->group_chunk = (uint32_t)value;
         return BPT_OK;
     }

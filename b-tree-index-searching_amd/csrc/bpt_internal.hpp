@@ -83,7 +83,7 @@ struct bpt_tree {
     uint32_t leaf_nt = 1;            // non-temporal leaf loads
     uint32_t coop_lanes = 8;         // lanes per query of the cooperative descent (4 or 8)
     uint32_t addr32 = 1;             // 32-bit node / leaf offsets when the tree allows them
-    uint32_t group_chunk = 16384;    // grouped search: queries per work chunk
+    uint32_t group_chunk = 0;        // grouped search: queries per work chunk (0: by batch size)
     uint32_t group_xcd = 1;          // grouped search: chunks split per XCD
     uint32_t group_lanes = 4;        // grouped search: lanes per query at the leaf (4 or 8)
     uint32_t group_blocks = 2;       // grouped search: 512-thread blocks per CU

@@ -130,7 +130,8 @@ bpt_status bpt_tree_erase(bpt_tree *t, const void *d_keys, uint64_t n, bpt_strea
  * "leaf_nt" (1 = non-temporal leaf loads, the default),
  * "addr32" (1 = 32-bit byte offsets into levels and leaves below 4 GiB, the
  * default; 0 = 64-bit addresses throughout), and for the grouped mode
- * "group_chunk" (queries per search work chunk, default 16384),
+ * "group_chunk" (queries per search work chunk; 0, the default, picks about
+ * 16 chunks per search block, a power of two in [1024, 32768]),
  * "group_lanes" (lanes per query at the leaf: 4, the default, or 8),
  * "group_blocks" (search blocks of 512 threads per CU, default 2) and
  * "group_xcd" (1 = each XCD searches its own contiguous part of the

@@ -122,7 +122,7 @@ def test_grouped_skewed_queries(bpt, oracle):
                         rng.integers(0, 1 << 32, size=100_000, dtype=np.uint64).astype(np.uint32)])
     rng.shuffle(q)
     want = _expect(oracle, keys, vals, q, 4)
-    for chunk, xcd in ((16384, 1), (256, 1), (4096, 0)):
+    for chunk, xcd in ((0, 1), (16384, 1), (256, 1), (4096, 0)):
         t.set_option("group_chunk", chunk)
         t.set_option("group_xcd", xcd)
         assert np.array_equal(t.point_search(q, bpt.GROUPED), want), (chunk, xcd)
