@@ -51,7 +51,7 @@ struct bpt_batch {
     bpt_forest *forest = nullptr;
     bpt::WorkItem *d_items = nullptr;
     uint32_t nitems = 0, chunk = 8192, blocks_per_cu = 8;
-    uint32_t *d_counter = nullptr;
+    uint32_t *d_counter = nullptr;  // kBatchXcds cursors, then kBatchXcds + 1 range bounds
     const uint64_t *q = nullptr, *lo = nullptr, *hi = nullptr;
     uint64_t *out = nullptr, *first = nullptr, *count = nullptr;
     uint64_t npoints = 0, nranges = 0;
@@ -120,7 +120,28 @@ __global__ __launch_bounds__(256) void k_forest_inner(const uint64_t *__restrict
     }
 }
 
-__global__ void k_reset_counter(uint32_t *c) { *c = 0; }
+// The items (ordered by tree) are cut into one contiguous range per XCD,
+// balanced by lookups, so that an XCD works through its own trees and their
+// lines stay in its L2. Blocks are dispatched to the XCDs round-robin (block
+// i on XCD i % 8): a block takes items from its XCD's range first, then from
+// the others' (in turn from the next XCD), so no XCD idles while work remains.
+constexpr uint32_t kBatchXcds = 8;
+
+__global__ void k_reset_counter(uint32_t *c) {
+    if (threadIdx.x < kBatchXcds) c[threadIdx.x] = 0;
+}
+
+// Next item for a block whose XCD is x, or nitems when every range is done.
+__device__ __forceinline__ uint32_t next_item(uint32_t *cursor, const uint32_t *bound, uint32_t x) {
+    for (uint32_t k = 0; k < kBatchXcds; ++k) {
+        const uint32_t r = (x + k) % kBatchXcds;
+        const uint32_t span = bound[r + 1] - bound[r];
+        if (cursor[r] >= span) continue;  // a stale read only costs one wasted atomic below
+        const uint32_t i = atomicAdd(&cursor[r], 1u);
+        if (i < span) return bound[r] + i;
+    }
+    return 0xFFFFFFFFu;
+}
 
 template <int F>
 __global__ __launch_bounds__(256) void k_forest_batch(const TreeView *__restrict__ views,
@@ -136,8 +157,9 @@ __global__ __launch_bounds__(256) void k_forest_batch(const TreeView *__restrict
     __shared__ uint32_t item_sh;
     uint32_t cur = 0xFFFFFFFFu;
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const uint32_t *bound = counter + kBatchXcds;
     for (;;) {
-        if (threadIdx.x == 0) item_sh = atomicAdd(counter, 1u);
+        if (threadIdx.x == 0) item_sh = next_item(counter, bound, blockIdx.x % kBatchXcds);
         __syncthreads();
         const uint32_t it = item_sh;
         if (it >= nitems) break;  // the same value in every thread of the block
@@ -402,11 +424,21 @@ bpt_status bpt_batch_prepare(bpt_forest *f, const uint32_t chunk, const uint32_t
     b->npoints = h_q_off ? h_q_off[f->ntrees] - h_q_off[0] : 0;
     b->nranges = h_r_off ? h_r_off[f->ntrees] - h_r_off[0] : 0;
     b->nitems = (uint32_t)items.size();
-    bpt_status st = upload(items, &b->d_items);
-    if (st == BPT_OK) {
-        hipError_t e = hipMalloc(&b->d_counter, 4);
-        if (e != hipSuccess) st = hip_fail(e, "hipMalloc(counter)", __FILE__, __LINE__);
+    // per-XCD item ranges balanced by lookups (a range query costs two descents)
+    std::vector<uint32_t> cnt(2 * kBatchXcds + 1, 0);
+    {
+        uint64_t total = 0;
+        for (const WorkItem &w : items) total += (w.end - w.begin) * (w.kind ? 2 : 1);
+        uint64_t acc = 0;
+        uint32_t x = 1;
+        for (uint32_t i = 0; i < b->nitems && x < kBatchXcds; ++i) {
+            acc += (items[i].end - items[i].begin) * (items[i].kind ? 2 : 1);
+            while (x < kBatchXcds && acc * kBatchXcds >= total * x) cnt[kBatchXcds + x++] = i + 1;
+        }
+        while (x <= kBatchXcds) cnt[kBatchXcds + x++] = b->nitems;
     }
+    bpt_status st = upload(items, &b->d_items);
+    if (st == BPT_OK) st = upload(cnt, &b->d_counter);
     if (st != BPT_OK) {
         bpt_batch_free(b);
         return st;
@@ -417,7 +449,7 @@ bpt_status bpt_batch_prepare(bpt_forest *f, const uint32_t chunk, const uint32_t
 
 static bpt_status batch_enqueue(bpt_batch *b, hipStream_t s) {
     bpt_forest *f = b->forest;
-    k_reset_counter<<<1, 1, 0, s>>>(b->d_counter);
+    k_reset_counter<<<1, 64, 0, s>>>(b->d_counter);
     BPT_HIP(hipGetLastError());
     const uint32_t grid = (uint32_t)std::min<uint64_t>((uint64_t)device_cu_count() * b->blocks_per_cu,
                                                        std::max<uint32_t>(b->nitems, 1));
