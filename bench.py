@@ -184,8 +184,12 @@ def bench_c4(args):
     dfirst = bptree.DeviceArray(nr, np.uint64)
     dcount = bptree.DeviceArray(nr, np.uint64)
     t0 = time.perf_counter()
-    best, report = bptree.forest_autotune(dk, dv, key_off, dq, q_off, dout, dlo, dhi, r_off, dfirst, dcount,
-                                          reps=1)
+    if args.no_autotune:  # the configuration the autotuner picks on MI355X
+        best = {"fanout": 16, "leaf_cap": 8, "lds_bytes": 32768, "chunk": 8192, "blocks_per_cu": 8}
+        report = []
+    else:
+        best, report = bptree.forest_autotune(dk, dv, key_off, dq, q_off, dout, dlo, dhi, r_off, dfirst, dcount,
+                                              reps=1)
     log(f"autotune {time.perf_counter() - t0:.1f} s -> {best}")
     forest = bptree.Forest(dk, dv, key_off, fanout=best["fanout"], lds_bytes=best["lds_bytes"])
     batch = bptree.Batch(forest, dq, q_off, dout, dlo, dhi, r_off, dfirst, dcount, chunk=best["chunk"])
@@ -404,6 +408,8 @@ def main():
                     help="time only the grouped form on the uniform set (for profiling its kernels)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run the 1 GiB device copy kernel (HBM counter calibration)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="with --config c4: skip the autotuner and use its usual choice (for profiling)")
     ap.add_argument("--learned-only", action="store_true",
                     help="with --config c5: only the plain and learned lookups (for profiling)")
     ap.add_argument("--config", default="c2", choices=["c2", "c3", "c4", "c5"],
