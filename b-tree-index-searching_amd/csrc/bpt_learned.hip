@@ -45,8 +45,19 @@ struct LeafModel {  // 48 bytes
     uint32_t flags, cnt;
 };
 
+// The lookup's copy of a LeafModel, 32 bytes: the base position fits 32 bits
+// (n < 2^31), the error bounds 16 bits (a model whose bounds do not is
+// answered by the fallback). Empty: k0 > k1. Fallback: elo > ehi.
+struct PackedModel {
+    uint64_t k0, k1;
+    double slope;
+    uint32_t p0;
+    int16_t elo, ehi;
+};
+static_assert(sizeof(PackedModel) == 32, "32-byte packed leaf model");
+
 struct LearnedView {
-    const LeafModel *models;
+    const PackedModel *models;
     const float *params;  // device, layer-by-layer mode
     const float *bx;      // breakpoints (nbreak)
     const float *alpha, *beta;  // nbreak + 1 pieces
@@ -56,15 +67,18 @@ struct LearnedView {
 struct LearnedModel {
     LearnedView view{};
     LeafModel *d_models = nullptr;
+    PackedModel *d_packed = nullptr;
     float *d_params = nullptr, *d_pwl = nullptr;
     std::vector<float> params;
     bpt_learned_stats stats{};
     uint32_t max_window = 256;
+    uint64_t packed_fallbacks = 0;  // models whose queries the fallback pass answers
 };
 
 void learned_free(LearnedModel *m) {
     if (!m) return;
     (void)hipFree(m->d_models);
+    (void)hipFree(m->d_packed);
     (void)hipFree(m->d_params);
     (void)hipFree(m->d_pwl);
     delete m;
@@ -252,6 +266,13 @@ __device__ __forceinline__ int64_t model_pred(const LeafModel &m, uint64_t key) 
     return (int64_t)floor(m.p0 + m.slope * (double)(key - m.k0));
 }
 
+// Same prediction from the packed copy ((double)p0 is exact: p0 < 2^31).
+__device__ __forceinline__ int64_t model_pred(const PackedModel &m, uint64_t key) {
+    return (int64_t)floor((double)m.p0 + m.slope * (double)(key - m.k0));
+}
+
+__device__ __forceinline__ bool packed_fallback(const PackedModel &m) { return m.elo > m.ehi; }
+
 __device__ void stage_learned(const LearnedView &v, float *sh) {
     if (v.root == 0) {
         for (uint32_t i = threadIdx.x; i < v.nbreak; i += blockDim.x) sh[i] = v.bx[i];
@@ -339,6 +360,33 @@ __global__ void k_finalize_models(LeafModel *__restrict__ models, uint32_t nmode
     atomicAdd(&acc[4], 1ull);
 }
 
+// Also counts the packed models the lookup sends to the fallback pass (those
+// over the window cap and those whose bounds exceed 16 bits) in *fallbacks.
+__global__ void k_pack_models(const LeafModel *__restrict__ models, uint32_t nmodels, PackedModel *__restrict__ out,
+                              unsigned long long *__restrict__ fallbacks) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nmodels) return;
+    const LeafModel m = models[j];
+    PackedModel p;
+    if (m.flags & kModelEmpty) {
+        p.k0 = ~0ull;
+        p.k1 = 0;
+        p.slope = 0.0;
+        p.p0 = 0;
+        p.elo = p.ehi = 0;
+    } else {
+        p.k0 = m.k0;
+        p.k1 = m.k1;
+        p.slope = m.slope;
+        p.p0 = (uint32_t)m.p0;
+        const bool fb = (m.flags & kModelFallback) || m.elo < -32767 || m.ehi > 32767;
+        p.elo = fb ? (int16_t)32767 : (int16_t)m.elo;
+        p.ehi = fb ? (int16_t)-32768 : (int16_t)m.ehi;
+        if (fb) atomicAdd(fallbacks, 1ull);
+    }
+    out[j] = p;
+}
+
 // ------------------------------------------------------------------ query
 template <class K, class V, int F, int LF>
 __global__ __launch_bounds__(256) void k_point_learned(TreeView t, LearnedView v, const K *__restrict__ q,
@@ -352,11 +400,11 @@ __global__ __launch_bounds__(256) void k_point_learned(TreeView t, LearnedView v
     const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += step) {
         const K key = __builtin_nontemporal_load(&q[i]);
-        const LeafModel m = v.models[route<K>(v, sh, key)];
+        const PackedModel m = v.models[route<K>(v, sh, key)];
         V res = sentinel;
-        if (m.flags & kModelFallback) {
+        if (packed_fallback(m)) {
             res = leaf_lookup<K, V, LF>(t, descend<K, F>(t, lds, key), key, sentinel);
-        } else if (!(m.flags & kModelEmpty) && (uint64_t)key >= m.k0 && (uint64_t)key <= m.k1) {
+        } else if ((uint64_t)key >= m.k0 && (uint64_t)key <= m.k1) {
             const int64_t pred = model_pred(m, (uint64_t)key);
             int64_t lo = pred + m.elo, hi = pred + m.ehi;
             lo = lo < 0 ? 0 : lo;
@@ -413,9 +461,9 @@ __global__ __launch_bounds__(256) void k_point_learned_coop(TreeView t, LearnedV
         const uint64_t my = base + lane;
         const K myq = my < nq ? __builtin_nontemporal_load(&q[my]) : K(0);
         // phase A: la > lb marks a miss (or a query handed to the fallback pass)
-        const LeafModel m = v.models[route<K>(v, sh, myq)];
+        const PackedModel m = v.models[route<K>(v, sh, myq)];
         uint32_t la = 1, lb = 0;
-        const bool fb = my < nq && (m.flags & kModelFallback);
+        const bool fb = my < nq && packed_fallback(m);
         const uint64_t fbm = __ballot(fb);
         if (fbm) {  // one LDS atomic per wave appends this wave's fallback queries
             uint32_t basei = 0;
@@ -425,7 +473,7 @@ __global__ __launch_bounds__(256) void k_point_learned_coop(TreeView t, LearnedV
         }
         if (fb) {
             // answered by the fallback pass
-        } else if (!(m.flags & kModelEmpty) && (uint64_t)myq >= m.k0 && (uint64_t)myq <= m.k1) {
+        } else if ((uint64_t)myq >= m.k0 && (uint64_t)myq <= m.k1) {
             const int64_t pred = model_pred(m, (uint64_t)myq);
             int64_t lo = pred + m.elo, hi = pred + m.ehi;
             lo = lo < 0 ? 0 : lo;
@@ -744,21 +792,23 @@ static bpt_status learned_build_impl(bpt_tree *t, const bpt_learned_config &cfg,
     H(hipMalloc(&d_pmin, M * 4ull), "hipMalloc");
     H(hipMalloc(&d_pmax, M * 4ull), "hipMalloc");
     H(hipMalloc(&d_cnt, M * 4ull), "hipMalloc");
-    H(hipMalloc(&d_acc, 5 * 8), "hipMalloc");
+    H(hipMalloc(&d_acc, 6 * 8), "hipMalloc");
     H(hipMemsetAsync(d_pmin, 0xFF, M * 4ull, s), "memset");
     H(hipMemsetAsync(d_pmax, 0, M * 4ull, s), "memset");
     H(hipMemsetAsync(d_cnt, 0, M * 4ull, s), "memset");
-    H(hipMemsetAsync(d_acc, 0, 5 * 8, s), "memset");
-    v.models = lm->d_models;
+    H(hipMemsetAsync(d_acc, 0, 6 * 8, s), "memset");
+    H(hipMalloc(&lm->d_packed, (size_t)M * sizeof(PackedModel)), "hipMalloc");
+    v.models = lm->d_packed;
     if (st == BPT_OK) {
         const uint32_t g = (uint32_t)std::min<uint64_t>((t->n + 255) / 256, (uint64_t)device_cu_count() * 16);
         k_route_keys<K, V, LF><<<g, 256, learned_lds(v), s>>>(t->view, v, d_routes, d_pmin, d_pmax, d_cnt);
         k_fit_models<K, V, LF><<<(M + 255) / 256, 256, 0, s>>>(t->view, M, d_pmin, d_pmax, d_cnt, lm->d_models);
         k_model_errors<K, V, LF><<<g, 256, 0, s>>>(t->view, d_routes, lm->d_models);
         k_finalize_models<<<(M + 255) / 256, 256, 0, s>>>(lm->d_models, M, cfg.max_window, d_acc);
+        k_pack_models<<<(M + 255) / 256, 256, 0, s>>>(lm->d_models, M, lm->d_packed, d_acc + 5);
         H(hipGetLastError(), "leaf models");
     }
-    unsigned long long acc[5] = {0, 0, 0, 0, 0};
+    unsigned long long acc[6] = {0, 0, 0, 0, 0, 0};
     H(hipMemcpyAsync(acc, d_acc, sizeof acc, hipMemcpyDeviceToHost, s), "stats");
     H(hipStreamSynchronize(s), "sync");
     auto t2 = std::chrono::steady_clock::now();
@@ -768,6 +818,7 @@ static bpt_status learned_build_impl(bpt_tree *t, const bpt_learned_config &cfg,
     lm->stats.mean_window = t->n ? (double)acc[2] / (double)t->n : 0.0;
     lm->stats.max_window_seen = (uint32_t)acc[3];
     lm->stats.models_used = (uint32_t)acc[4];
+    lm->packed_fallbacks = acc[5];
     lm->max_window = cfg.max_window;
     for (void *p : {(void *)d_x, (void *)d_t, (void *)d_partial, (void *)d_grad, (void *)d_mom, (void *)d_vel,
                     (void *)d_step, (void *)d_routes, (void *)d_pmin, (void *)d_pmax, (void *)d_cnt, (void *)d_acc})
@@ -815,7 +866,7 @@ bpt_status point_learned(const bpt_tree *t, const void *d_q, uint64_t nq, void *
                                                                  ((uint64_t)blocks * 4 + 255) / 256 * 256);
                 k_point_learned_coop<K, V, F, LF><<<blocks, t->block, lds, s>>>(
                     gv, t->learned->view, (const K *)d_q, (V *)d_out, nq, ~V(0), 0, fb_count, fb_list, cap);
-                if (t->learned->stats.fallback_models) {
+                if (t->learned->packed_fallbacks) {
                     const size_t tl = t->view.lds_bytes ? t->view.lds_bytes : 16;
                     k_point_coop_list<K, V, F, LF><<<blocks, t->block, tl, s>>>(
                         t->view, (const K *)d_q, (V *)d_out, fb_count, fb_list, cap, ~V(0));
