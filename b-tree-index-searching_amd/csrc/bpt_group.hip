@@ -308,15 +308,16 @@ __host__ __device__ static int group2d_start_pass(uint64_t ma, uint64_t mb) {
 
 // The same keys with their parameters taken from the maxima in device memory
 // (mx[0], mx[1], written by k_max2), so no host round trip sits between.
+// Only the top 32 bits of the Z key are kept: the order uses its top 16.
 template <class KT>
 __global__ void k_zorder_keys_dev(const KT *__restrict__ a, const KT *__restrict__ b, uint64_t n,
-                                  const unsigned long long *__restrict__ mx, uint64_t *__restrict__ out) {
+                                  const unsigned long long *__restrict__ mx, uint32_t *__restrict__ out) {
     const uint64_t ix = zorder_interval(mx[0]), iy = zorder_interval(mx[1]);
     const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) {
         const uint32_t xg = (uint32_t)((uint64_t)a[i] / ix);
         const uint32_t yg = (uint32_t)((uint64_t)b[i] / iy);
-        out[i] = spread32(xg) | (spread32(yg) << 1);
+        out[i] = (uint32_t)((spread32(xg) | (spread32(yg) << 1)) >> 32);
     }
 }
 
@@ -395,12 +396,11 @@ bpt_status query_order(const KT *lo, const KT *hi, uint64_t n, int mode, void *w
                                     order, s);
     }
     if (mode == BPT_GROUP_ZORDER) {
-        uint64_t *z = reinterpret_cast<uint64_t *>(p);
+        uint32_t *z = reinterpret_cast<uint32_t *>(p);
         k_zorder_keys_dev<KT><<<g, 256, 0, s>>>(lo, hi, n, mx, z);
         BPT_HIP(hipGetLastError());
         // the top 16 bits of the 64-bit Z key order the groups (as above)
-        uint64_t *zs = nullptr;
-        return sort_pairs<uint64_t>(z, nullptr, n, 48, 64, sort_ws, ws_bytes - align256(n * 8), &zs, order, s);
+        return sort_pairs<uint32_t>(z, nullptr, n, 16, 32, sort_ws, ws_bytes - align256(n * 8), &ksorted, order, s);
     }
     set_error("unknown group mode %d", mode);
     return BPT_ERR_INVALID;
