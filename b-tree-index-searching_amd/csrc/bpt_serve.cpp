@@ -2,15 +2,17 @@
 // with continuous batching (no reference counterpart; the reference runs one
 // synchronous batch per process).
 //
-//  * `slots` HIP streams (default priority); each in-flight request owns one.
+//  * `slots` HIP streams (default priority), each driven by its own host
+//    thread; each in-flight request owns one slot. The threads take requests
+//    from one FIFO queue, so the host work of different requests (staging
+//    copies, HIP calls, waiting) runs in parallel.
 //  * Device memory for requests comes from one pool of `mem_budget` bytes
 //    (first-fit, 256-byte granules). A request is admitted when a slot is
 //    free AND its bytes fit in the pool; otherwise it waits in FIFO order, so
 //    a small budget makes requests queue, never fail (a request larger than
 //    the whole budget is rejected at submit).
 //  * Admitted work overlaps on the device: copy-in, search kernels and
-//    copy-out of different requests run on different streams while the
-//    dispatcher thread keeps admitting as earlier requests complete.
+//    copy-out of different requests run on different streams.
 //  * Pinned staging buffers (hipHostMalloc) per slot carry the host copies.
 //  * Latency = completion time - submit time; stats give p50/p99.
 #include <hip/hip_runtime.h>
@@ -45,10 +47,10 @@ struct Request {
 
 struct Slot {
     hipStream_t stream = nullptr;
-    hipEvent_t ev = nullptr;
     uint8_t *pin = nullptr;  // pinned staging
     uint64_t pin_bytes = 0;
     Request *cur = nullptr;
+    std::thread worker;
 };
 
 // First-fit allocator over [0, size) in 256-byte granules.
@@ -105,7 +107,6 @@ struct bpt_server {
     std::deque<Request *> queue;
     std::unordered_map<uint64_t, Request *> reqs;
     uint64_t next_ticket = 1;
-    std::thread worker;
     bool stop = false;
     uint32_t inflight = 0, max_inflight = 0;
     std::vector<double> lat_ms;
@@ -162,81 +163,68 @@ bpt_status launch(bpt_server *S, Slot &sl, Request *q) {
         BPT_HIP(hipMemcpyAsync(sl.pin + 2 * r.n * kb, df, r.n * 8, hipMemcpyDeviceToHost, s));
         BPT_HIP(hipMemcpyAsync(sl.pin + 2 * r.n * kb + r.n * 8, dc, r.n * 8, hipMemcpyDeviceToHost, s));
     }
-    BPT_HIP(hipEventRecord(sl.ev, s));
     return BPT_OK;
 }
 
-void finish(bpt_server *S, Slot &sl) {
-    Request *q = sl.cur;
+// Copies a finished request's results to the caller's buffers (no lock held).
+void copy_out(const bpt_server *S, const Slot &sl, const Request *q) {
     const bpt_request &r = q->req;
     const uint64_t kb = S->tree->key_bytes, vb = S->tree->val_bytes;
-    if (q->status == BPT_OK) {
-        if (r.kind == BPT_REQ_POINT) {
-            memcpy(r.out, sl.pin + r.n * kb, r.n * vb);
-        } else {
-            if (r.first) memcpy(r.first, sl.pin + 2 * r.n * kb, r.n * 8);
-            memcpy(r.count, sl.pin + 2 * r.n * kb + r.n * 8, r.n * 8);
-        }
+    if (q->status != BPT_OK) return;
+    if (r.kind == BPT_REQ_POINT) {
+        memcpy(r.out, sl.pin + r.n * kb, r.n * vb);
+    } else {
+        if (r.first) memcpy(r.first, sl.pin + 2 * r.n * kb, r.n * 8);
+        memcpy(r.count, sl.pin + 2 * r.n * kb + r.n * 8, r.n * 8);
     }
+}
+
+// Bookkeeping of a finished request (S->mu held).
+void finish(bpt_server *S, Slot &sl) {
+    Request *q = sl.cur;
     q->t_done = clk::now();
     q->done = true;
     S->pool.release(q->dev_off, q->bytes);
     S->used -= (q->bytes + 255) / 256 * 256;
     S->inflight--;
     S->lat_ms.push_back(std::chrono::duration<double, std::milli>(q->t_done - q->t_submit).count());
-    S->lookups += r.n;
+    S->lookups += q->req.n;
     S->completed++;
     if (!S->any || q->t_done > S->t_last) S->t_last = q->t_done;
     sl.cur = nullptr;
 }
 
-void dispatcher(bpt_server *S) {
+// One thread per slot: admission is FIFO -- a thread takes the head of the
+// queue only when its bytes fit the pool, otherwise every thread waits for
+// memory to be released -- then stages, launches and waits on its own stream.
+void slot_worker(bpt_server *S, Slot *sl) {
     (void)hipSetDevice(S->tree->device);
     std::unique_lock<std::mutex> lk(S->mu);
     for (;;) {
-        bool progress = false;
-        for (Slot &sl : S->slots) {
-            if (sl.cur && hipEventQuery(sl.ev) == hipSuccess) {
-                finish(S, sl);
-                progress = true;
-            }
-        }
-        if (progress) S->cv_done.notify_all();
-        // admission: FIFO while a slot is free and the head fits in memory
-        while (!S->queue.empty()) {
-            Slot *free_slot = nullptr;
-            for (Slot &sl : S->slots)
-                if (!sl.cur) { free_slot = &sl; break; }
-            if (!free_slot) break;
-            Request *q = S->queue.front();
-            uint64_t off = 0;
-            if (!S->pool.alloc(q->bytes, &off)) break;
-            S->queue.pop_front();
-            q->dev_off = off;
-            S->used += (q->bytes + 255) / 256 * 256;
-            S->peak = std::max(S->peak, S->used);
-            free_slot->cur = q;
-            S->inflight++;
-            S->max_inflight = std::max(S->max_inflight, S->inflight);
-            q->status = launch(S, *free_slot, q);
-            if (q->status != BPT_OK) {
-                // record the failure; the event still marks the slot complete
-                (void)hipEventRecord(free_slot->ev, free_slot->stream);
-            }
-            progress = true;
-        }
-        bool busy = false;
-        for (Slot &sl : S->slots) busy |= sl.cur != nullptr;
-        if (S->stop && !busy && S->queue.empty()) break;
-        if (!progress) {
-            if (busy) {
-                lk.unlock();
-                std::this_thread::sleep_for(std::chrono::microseconds(20));
-                lk.lock();
-            } else {
-                S->cv_work.wait_for(lk, std::chrono::milliseconds(5));
-            }
-        }
+        uint64_t off = 0;
+        S->cv_work.wait(lk, [&] {
+            return (S->stop && S->queue.empty()) ||
+                   (!S->queue.empty() && S->pool.alloc(S->queue.front()->bytes, &off));
+        });
+        if (S->queue.empty()) break;  // stopping
+        Request *q = S->queue.front();
+        S->queue.pop_front();
+        q->dev_off = off;
+        S->used += (q->bytes + 255) / 256 * 256;
+        S->peak = std::max(S->peak, S->used);
+        sl->cur = q;
+        S->inflight++;
+        S->max_inflight = std::max(S->max_inflight, S->inflight);
+        if (!S->queue.empty()) S->cv_work.notify_one();  // the next head may fit too
+        lk.unlock();
+        q->status = launch(S, *sl, q);
+        const hipError_t e = hipStreamSynchronize(sl->stream);
+        if (q->status == BPT_OK && e != hipSuccess) q->status = bpt::hip_fail(e, "request", __FILE__, __LINE__);
+        copy_out(S, *sl, q);
+        lk.lock();
+        finish(S, *sl);
+        S->cv_done.notify_all();
+        S->cv_work.notify_all();  // memory was released
     }
 }
 
@@ -266,13 +254,12 @@ bpt_status bpt_server_create(bpt_tree *t, uint32_t slots, uint64_t mem_budget, i
     S->slots.resize(slots);
     for (Slot &sl : S->slots) {
         e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
-        if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming);
         if (e != hipSuccess) {
             bpt_server_destroy(S);
             return hip_fail(e, "server stream", __FILE__, __LINE__);
         }
     }
-    S->worker = std::thread(dispatcher, S);
+    for (Slot &sl : S->slots) sl.worker = std::thread(slot_worker, S, &sl);
     *out = S;
     return BPT_OK;
 }
@@ -363,11 +350,11 @@ bpt_status bpt_server_destroy(bpt_server *S) {
         S->stop = true;
     }
     S->cv_work.notify_all();
-    if (S->worker.joinable()) S->worker.join();
+    for (Slot &sl : S->slots)
+        if (sl.worker.joinable()) sl.worker.join();
     for (auto &kv : S->reqs) delete kv.second;
     for (Slot &sl : S->slots) {
         if (sl.stream) (void)hipStreamSynchronize(sl.stream);
-        if (sl.ev) (void)hipEventDestroy(sl.ev);
         if (sl.stream) (void)hipStreamDestroy(sl.stream);
         if (sl.pin) (void)hipHostFree(sl.pin);
     }
