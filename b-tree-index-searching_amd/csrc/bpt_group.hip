@@ -571,8 +571,8 @@ bpt_status bpt_group2d_keys(const uint64_t *d_a, const uint64_t *d_b, uint64_t n
     return BPT_OK;
 }
 
-bpt_status bpt_range_count_grouped(const bpt_tree *t, const void *d_lo, const void *d_hi, uint64_t nq,
-                                   int group_mode, uint64_t *d_first, uint64_t *d_count, bpt_stream stream) {
+bpt_status bpt_range_count_grouped(bpt_tree *t, const void *d_lo, const void *d_hi, uint64_t nq, int group_mode,
+                                   uint64_t *d_first, uint64_t *d_count, bpt_stream stream) {
     BPT_CHECK(t, BPT_ERR_INVALID, "tree is NULL");
     BPT_CHECK(nq == 0 || (d_lo && d_hi && d_count), BPT_ERR_INVALID, "NULL argument");
     hipStream_t s = as_stream(stream);
@@ -581,18 +581,16 @@ bpt_status bpt_range_count_grouped(const bpt_tree *t, const void *d_lo, const vo
     BPT_CHECK(group_mode == BPT_GROUP_GRID2D || group_mode == BPT_GROUP_ZORDER, BPT_ERR_INVALID,
               "unknown group mode %d", group_mode);
     BPT_CHECK(nq < (1ull << 32), BPT_ERR_CAPACITY, "grouped range search of more than 2^32 queries");
+    // the order's scratch is the tree's workspace (kept between calls)
     const uint64_t ws_bytes = query_order_ws_any(nq);
-    void *ws = nullptr;
-    BPT_HIP(hipMalloc(&ws, ws_bytes));
+    BPT_TRY(t->ws.reserve(ws_bytes));
     uint32_t *order = nullptr;
     bpt_status st = t->key_bytes == 4
                         ? query_order<uint32_t>((const uint32_t *)d_lo, (const uint32_t *)d_hi, nq, group_mode,
-                                                ws, ws_bytes, &order, s)
+                                                t->ws.ptr, ws_bytes, &order, s)
                         : query_order<uint64_t>((const uint64_t *)d_lo, (const uint64_t *)d_hi, nq, group_mode,
-                                                ws, ws_bytes, &order, s);
+                                                t->ws.ptr, ws_bytes, &order, s);
     if (st == BPT_OK) st = range_count_ordered(t, d_lo, d_hi, nq, order, d_first, d_count, s);
-    (void)hipStreamSynchronize(s);
-    (void)hipFree(ws);
     return st;
 }
 

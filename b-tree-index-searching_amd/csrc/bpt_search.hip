@@ -92,18 +92,133 @@ __global__ __launch_bounds__(256) void k_range_count(TreeView t, const K *__rest
     }
 }
 
-// One wave per query: lanes copy consecutive value slots.
+// The same counts through the cooperative descent (equal key and value
+// widths): each wave takes 64 queries at a time and runs both lower bounds
+// with G lanes per node, so the two descents of a group are in flight
+// together.
+template <class K, class V, int F, int LF, int G, bool OFF32>
+__global__ __launch_bounds__(256) void k_range_count_coop(TreeView t, const K *__restrict__ lo,
+                                                          const K *__restrict__ hi,
+                                                          const uint32_t *__restrict__ order,
+                                                          uint64_t *__restrict__ first,
+                                                          uint64_t *__restrict__ count, uint64_t nq) {
+    extern __shared__ uint4 lds_raw[];
+    uint8_t *lds = reinterpret_cast<uint8_t *>(lds_raw);
+    stage_top_levels(t, lds);
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t stride = ((uint64_t)gridDim.x * blockDim.x) & ~63ull;
+    for (uint64_t base = wave * 64; base < nq; base += stride) {
+        const uint64_t j = base + lane;
+        const bool valid = j < nq;
+        const uint64_t i = valid ? (order ? (uint64_t)order[j] : j) : 0;
+        const K a = valid ? lo[i] : K(0), b = valid ? hi[i] : K(0);
+        const bool top = b == key_max<K>();
+        const uint64_t pa = coop_lookup64<K, V, F, LF, false, kOpLowerBound, G, OFF32>(t, lds, a, V(0));
+        const uint64_t pb1 =
+            coop_lookup64<K, V, F, LF, false, kOpLowerBound, G, OFF32>(t, lds, top ? b : K(b + 1), V(0));
+        const uint64_t pb = top ? t.n : pb1;
+        if (valid) {
+            if (first) first[i] = pa;
+            count[i] = (a > b || pb < pa) ? 0 : pb - pa;
+        }
+    }
+}
+
+__device__ __forceinline__ uint64_t wave_max_u64(uint64_t x) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        const uint64_t y = __shfl_xor(x, d, 64);
+        x = y > x ? y : x;
+    }
+    return x;
+}
+
+constexpr int kRangeIdsUnroll = 4;  // output chunks of 64 ids in flight per wave
+
+// Ids of a group of ranges whose output runs exceed 32-bit relative
+// positions: one range at a time, lanes copying consecutive value slots.
 template <class K, class V, int LF>
+__device__ __forceinline__ void range_ids_wide(const TreeView &t, const uint64_t *__restrict__ first,
+                                               const uint64_t *__restrict__ count,
+                                               const uint64_t *__restrict__ offsets, uint64_t g0,
+                                               uint64_t g1, V *__restrict__ ids) {
+    const uint32_t lane = threadIdx.x & 63;
+    for (uint64_t i = g0; i < g1; ++i) {
+        const uint64_t f = first[i], c = count[i], o = offsets[i];
+        for (uint64_t k = lane; k < c; k += 64) ids[o + k] = slot_val<K, V, LF>(t, f + k);
+    }
+}
+
+// One wave per group of 64 consecutive ranges, range r of the group held by
+// lane r. With offsets the exclusive scan of the counts, the group's ids are
+// one run of output positions [o0, o0 + span); the wave walks it 64
+// positions at a time, U chunks per round so that U loads per lane are in
+// flight. A lane finds the range of its position p (the last range starting
+// at or before p) between the chunk's first and last range -- two ballots --
+// by a binary search over the lanes' starts. Positions outside every
+// [offset, offset + count) are skipped, so any non-decreasing offsets are
+// accepted; every write stays inside a caller's [offset, offset + count).
+// S32: slot numbers fit 32 bits (n <= 2^32).
+template <class K, class V, int LF, int U, bool S32>
 __global__ __launch_bounds__(256) void k_range_ids(TreeView t, const uint64_t *__restrict__ first,
                                                    const uint64_t *__restrict__ count,
                                                    const uint64_t *__restrict__ offsets, uint64_t nq,
                                                    V *__restrict__ ids) {
+    using S = typename std::conditional<S32, uint32_t, uint64_t>::type;
     const uint32_t lane = threadIdx.x & 63;
     const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint64_t waves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-    for (uint64_t i = wave; i < nq; i += waves) {
-        const uint64_t f = first[i], c = count[i], o = offsets[i];
-        for (uint64_t k = lane; k < c; k += 64) ids[o + k] = slot_val<K, V, LF>(t, f + k);
+    for (uint64_t g0 = wave * 64; g0 < nq; g0 += waves * 64) {
+        const uint64_t i = g0 + lane;
+        const bool valid = i < nq;
+        const uint64_t f = valid ? first[i] : 0, c = valid ? count[i] : 0, o = valid ? offsets[i] : 0;
+        const uint64_t o0 = __shfl(o, 0, 64);  // lane 0 always holds a range
+        const uint64_t span = wave_max_u64(valid && o >= o0 ? o + c : o0) - o0;
+        if (span > 0xFFFFFFFFull - 64 * U) {
+            range_ids_wide<K, V, LF>(t, first, count, offsets, g0, g0 + 64 < nq ? g0 + 64 : nq, ids);
+            continue;
+        }
+        // relative start; a lane without a range (or starting before o0) never matches
+        const uint32_t rel = valid && o >= o0 ? (uint32_t)(o - o0) : 0xFFFFFFFFu;
+        const uint32_t cnt = rel != 0xFFFFFFFFu ? (uint32_t)c : 0u;
+        const S sb = (S)f - (S)rel;  // slot of position p in this range: sb + p
+        const uint32_t nxt = __shfl(rel, (int)((lane + 1) & 63), 64);
+        // dense: every range starts at or after o0 and ends where the next
+        // one starts (offsets are the exclusive scan of the counts)
+        const bool has_next = lane < 63 && i + 1 < nq;
+        const bool dense = __all(!valid || (rel != 0xFFFFFFFFu && (!has_next || nxt == rel + cnt)));
+        const uint32_t end = (uint32_t)span;
+        for (uint32_t base = 0; base < end; base += 64 * U) {
+            V v[U];
+            uint32_t pos[U];
+            bool ok[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t P = base + 64u * u, p = P + lane;
+                const uint32_t ra = (uint32_t)__popcll(__ballot(rel <= P)) - 1;
+                const uint32_t rb = (uint32_t)__popcll(__ballot(rel <= P + 63)) - 1;
+                uint32_t r = ra;
+                if (rb > ra) {
+                    for (uint32_t st = 1u << (31 - __clz(rb - ra)); st; st >>= 1) {
+                        const uint32_t cand = r + st < rb ? r + st : rb;
+                        r = __shfl(rel, (int)cand, 64) <= p ? cand : r;
+                    }
+                }
+                bool in = p < end;
+                if (!dense) {
+                    const uint32_t rr = __shfl(rel, (int)r, 64), cr = __shfl(cnt, (int)r, 64);
+                    in = in && p >= rr && p - rr < cr;
+                }
+                const S slot = __shfl(sb, (int)r, 64) + (S)p;
+                v[u] = in ? slot_val<K, V, LF>(t, (uint64_t)slot) : V(0);
+                pos[u] = p;
+                ok[u] = in;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (ok[u]) ids[o0 + pos[u]] = v[u];
+        }
     }
 }
 
@@ -340,9 +455,30 @@ bpt_status range_count_ordered(const bpt_tree *t, const void *d_lo, const void *
     return dispatch_tree(t, [&](auto kt, auto vt, auto fc, auto lc) -> bpt_status {
         using K = typename decltype(kt)::type;
         using V = typename decltype(vt)::type;
-        k_range_count<K, V, decltype(fc)::value, decltype(lc)::value>
-            <<<search_grid(t, nq), t->block, lds_of(t), s>>>(t->view, (const K *)d_lo, (const K *)d_hi,
-                                                             d_order, d_first, d_count, nq);
+        constexpr int F = decltype(fc)::value, LF = decltype(lc)::value;
+        const uint32_t g = search_grid(t, nq);
+        const size_t l = lds_of(t);
+        if constexpr (sizeof(K) == sizeof(V)) {
+            if (t->coop && t->view.count[0] < (1ull << 32)) {
+                const bool off32 = t->addr32 && t->leaf_bytes <= (1ull << 32) && t->inner_bytes <= (1ull << 32);
+                auto go = [&](auto lanes, auto o32) {
+                    k_range_count_coop<K, V, F, LF, decltype(lanes)::value, decltype(o32)::value>
+                        <<<g, t->block, l, s>>>(t->view, (const K *)d_lo, (const K *)d_hi, d_order, d_first,
+                                                d_count, nq);
+                };
+                if (t->coop_lanes == 4) {
+                    if (off32) go(IC<4>{}, std::true_type{});
+                    else go(IC<4>{}, std::false_type{});
+                } else {
+                    if (off32) go(IC<8>{}, std::true_type{});
+                    else go(IC<8>{}, std::false_type{});
+                }
+                BPT_HIP(hipGetLastError());
+                return BPT_OK;
+            }
+        }
+        k_range_count<K, V, F, LF><<<g, t->block, l, s>>>(t->view, (const K *)d_lo, (const K *)d_hi, d_order,
+                                                          d_first, d_count, nq);
         BPT_HIP(hipGetLastError());
         return BPT_OK;
     });
@@ -367,10 +503,15 @@ bpt_status bpt_range_ids(const bpt_tree *t, const uint64_t *d_first, const uint6
     return dispatch_tree(t, [&](auto kt, auto vt, auto, auto lc) -> bpt_status {
         using K = typename decltype(kt)::type;
         using V = typename decltype(vt)::type;
-        uint64_t waves_needed = nq;
-        uint32_t blocks = (uint32_t)std::min<uint64_t>((waves_needed + 3) / 4, (uint64_t)device_cu_count() * 16);
-        k_range_ids<K, V, decltype(lc)::value><<<blocks, 256, 0, s>>>(t->view, d_first, d_count,
-                                                                       d_offsets, nq, (V *)d_ids);
+        constexpr int LF = decltype(lc)::value;
+        const uint64_t groups = (nq + 63) / 64;  // one wave per 64 ranges
+        const uint32_t blocks = (uint32_t)std::min<uint64_t>((groups + 3) / 4, (uint64_t)device_cu_count() * 8);
+        if (t->n <= (1ull << 32))
+            k_range_ids<K, V, LF, kRangeIdsUnroll, true>
+                <<<blocks, 256, 0, s>>>(t->view, d_first, d_count, d_offsets, nq, (V *)d_ids);
+        else
+            k_range_ids<K, V, LF, kRangeIdsUnroll, false>
+                <<<blocks, 256, 0, s>>>(t->view, d_first, d_count, d_offsets, nq, (V *)d_ids);
         BPT_HIP(hipGetLastError());
         return BPT_OK;
     });

@@ -491,6 +491,49 @@ def main():
                                  "speedup_vs_plain": wall_pz / wall_gz,
                                  "mismatches_vs_plain": int((dout.to_host() != plain_z).sum())}
         dqz.free()
+        # 1-D ranges over the same tree: 2^22 [lo, hi] with widths log-uniform in
+        # [1, 2^10] ranks; one step = counts, their scan, and the ids, in each order
+        rng_r = np.random.default_rng(11)
+        nrq = 1 << 22
+        st_ = rng_r.integers(0, n, nrq)
+        wr = np.floor(np.exp2(rng_r.uniform(0, 10, nrq))).astype(np.int64)
+        dlo = bptree.DeviceArray.from_host(keys[st_])
+        dhi = bptree.DeviceArray.from_host(keys[np.minimum(st_ + wr, n - 1)])
+        dfirst = bptree.DeviceArray(nrq, np.uint64)
+        dcnt = bptree.DeviceArray(nrq, np.uint64)
+        doff = bptree.DeviceArray(nrq, np.uint64)
+        L = bptree.lib()
+        base = None
+        for gm, gname in ((bptree.GROUP_NONE, "basic"), (bptree.GROUP_GRID2D, "grid2d"),
+                          (bptree.GROUP_ZORDER, "zorder")):
+            def rstep():
+                tree.range_count_dev(dlo, dhi, dfirst, dcnt, gm)
+                tot = bptree.C.c_uint64(0)
+                bptree.check(L.bpt_exclusive_scan_u64(dcnt.ptr, nrq, doff.ptr, bptree.C.byref(tot), None))
+                dids = bptree.DeviceArray(tot.value, np.uint32)
+                bptree.check(L.bpt_range_ids(tree.h, dfirst.ptr, dcnt.ptr, doff.ptr, nrq, dids.ptr, None))
+                return dids
+            rstep().free()
+            bptree.sync()
+            t0 = time.perf_counter()
+            reps = 3
+            for _ in range(reps):
+                d = rstep()
+                bptree.sync()
+                d.free()
+            dt = (time.perf_counter() - t0) / reps
+            d = rstep()
+            cnt_h = dcnt.to_host()
+            ids_h = d.to_host(count=min(d.n, 1 << 24))
+            d.free()
+            if base is None:
+                base = (cnt_h, ids_h)
+            forms[f"range_{gname}"] = {"ranges_per_s": nrq / dt, "ms": dt * 1e3, "ids": int(cnt_h.sum()),
+                                       "ids_per_s": float(cnt_h.sum()) / dt,
+                                       "identical_to_basic": bool(np.array_equal(cnt_h, base[0])
+                                                                  and np.array_equal(ids_h, base[1]))}
+        for a in (dlo, dhi, dfirst, dcnt, doff):
+            a.free()
         # batched update: 2^20 upserts (half of them existing keys) merged into
         # the tree, which is rebuilt; wall time of the whole call (allocations included)
         rng_u = np.random.default_rng(7)

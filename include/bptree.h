@@ -178,7 +178,9 @@ bpt_status bpt_lower_bound(const bpt_tree *t, const void *d_q, uint64_t nq, uint
 bpt_status bpt_range_count(const bpt_tree *t, const void *d_lo, const void *d_hi, uint64_t nq,
                            uint64_t *d_first, uint64_t *d_count, bpt_stream s);
 /* ids for query i (values of the matching slots, in key order) written at
- * d_ids[offsets[i] ...]; offsets = exclusive scan of counts. */
+ * d_ids[offsets[i] ...]; offsets = exclusive scan of counts (any
+ * non-decreasing offsets with offsets[i] + count[i] <= offsets[i+1] also
+ * work; nothing outside the [offsets[i], offsets[i] + count[i]) is written). */
 bpt_status bpt_range_ids(const bpt_tree *t, const uint64_t *d_first, const uint64_t *d_count,
                          const uint64_t *d_offsets, uint64_t nq, void *d_ids, bpt_stream s);
 /* d_out[i] = sum of d_in[0..i); *h_total (host, may be NULL) = sum of all.
@@ -192,7 +194,9 @@ typedef enum bpt_group_mode {
     BPT_GROUP_GRID2D = 1, /* SearchingBTreeGPURangeWithGrouping (grouping2d.cu bucket) */
     BPT_GROUP_ZORDER = 2  /* SearchingBTreeGPURangeWithGroupingZorder (zOrder2d.cu key) */
 } bpt_group_mode;
-bpt_status bpt_range_count_grouped(const bpt_tree *t, const void *d_lo, const void *d_hi,
+/* The grouped orders use the tree's workspace, like bpt_point_search:
+ * concurrent calls on one tree must share one stream. */
+bpt_status bpt_range_count_grouped(bpt_tree *t, const void *d_lo, const void *d_hi,
                                    uint64_t nq, int group_mode, uint64_t *d_first,
                                    uint64_t *d_count, bpt_stream s);
 

@@ -120,6 +120,46 @@ def test_radix_group_is_stable_partition(bpt, kb, shift, bits):
     assert np.array_equal(starts, np.concatenate([[0], np.cumsum(cnt)]).astype(np.uint64))
 
 
+@pytest.mark.parametrize("kb,vb", [(4, 4), (8, 8), (8, 4)])
+@pytest.mark.parametrize("layout", ["gaps", "empty_runs", "one_wide", "scan"])
+def test_range_ids_offsets(bpt, oracle, kb, vb, layout):
+    """bpt_range_ids with caller-made offsets: gaps between the runs stay
+    untouched, long runs of empty ranges and one range far wider than the
+    rest come back whole."""
+    rng = np.random.default_rng(len(layout) + kb + vb)
+    kdt = np.uint32 if kb == 4 else np.uint64
+    vdt = np.uint32 if vb == 4 else np.uint64
+    n = 300_007
+    keys = np.unique(rng.integers(0, 1 << (8 * kb - 1), size=n + n // 8, dtype=np.uint64))[:n].astype(kdt)
+    n = len(keys)
+    vals = rng.integers(0, (1 << (8 * vb)) - 1, size=n, dtype=np.uint64).astype(vdt)
+    t = bpt.Tree(keys, vals, kb, vb)
+    nq = 20_000
+    cnt = np.floor(np.exp2(rng.uniform(0, 9, nq))).astype(np.uint64)
+    gap = np.zeros(nq, np.uint64)
+    if layout == "gaps":
+        gap = rng.integers(0, 70, nq).astype(np.uint64)
+    elif layout == "empty_runs":
+        cnt[rng.random(nq) < 0.9] = 0
+    elif layout == "one_wide":
+        cnt[:] = rng.integers(0, 4, nq)
+        cnt[nq // 3] = n // 2
+    first = np.array([rng.integers(0, n - int(c) + 1) for c in cnt], dtype=np.uint64)
+    offs = np.cumsum(gap + cnt).astype(np.uint64) - cnt + 17  # gap before each run, base 17
+    total = int(offs[-1] + cnt[-1]) + 5
+    fill = vdt(np.iinfo(vdt).max)
+    want = np.full(total, fill, dtype=vdt)
+    for f, c, o in zip(first, cnt, offs):
+        want[int(o):int(o + c)] = vals[int(f):int(f + c)]
+    dfirst, dcnt, doffs = (bpt.DeviceArray.from_host(a) for a in (first, cnt, offs))
+    dids = bpt.DeviceArray.from_host(np.full(total, fill, dtype=vdt))
+    bpt.check(bpt.lib().bpt_range_ids(t.h, dfirst.ptr, dcnt.ptr, doffs.ptr, nq, dids.ptr, None))
+    got = dids.to_host()
+    for a in (dfirst, dcnt, doffs, dids):
+        a.free()
+    assert np.array_equal(got, want)
+
+
 def test_exclusive_scan(bpt):
     rng = np.random.default_rng(11)
     for n in (1, 7, 2048, 2049, 5_000_003):
