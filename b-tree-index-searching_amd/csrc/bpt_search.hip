@@ -134,7 +134,7 @@ __device__ __forceinline__ uint64_t wave_max_u64(uint64_t x) {
     return x;
 }
 
-constexpr int kRangeIdsUnroll = 4;  // output chunks of 64 ids in flight per wave
+constexpr int kRangeIdsUnroll = 8;  // output chunks of 64 ids in flight per wave
 
 // Ids of a group of ranges whose output runs exceed 32-bit relative
 // positions: one range at a time, lanes copying consecutive value slots.

@@ -504,35 +504,43 @@ def main():
         doff = bptree.DeviceArray(nrq, np.uint64)
         L = bptree.lib()
         base = None
+        # the caller owns the ids buffer: kept across steps, grown when a
+        # step's total exceeds it (the C ABI writes into caller memory)
+        ids_buf = [None]
+
+        def ids_for(total):
+            if ids_buf[0] is None or ids_buf[0].n < total:
+                if ids_buf[0] is not None:
+                    ids_buf[0].free()
+                ids_buf[0] = bptree.DeviceArray(max(total, 1), np.uint32)
+            return ids_buf[0]
         for gm, gname in ((bptree.GROUP_NONE, "basic"), (bptree.GROUP_GRID2D, "grid2d"),
                           (bptree.GROUP_ZORDER, "zorder")):
             def rstep():
                 tree.range_count_dev(dlo, dhi, dfirst, dcnt, gm)
                 tot = bptree.C.c_uint64(0)
                 bptree.check(L.bpt_exclusive_scan_u64(dcnt.ptr, nrq, doff.ptr, bptree.C.byref(tot), None))
-                dids = bptree.DeviceArray(tot.value, np.uint32)
+                dids = ids_for(tot.value)
                 bptree.check(L.bpt_range_ids(tree.h, dfirst.ptr, dcnt.ptr, doff.ptr, nrq, dids.ptr, None))
-                return dids
-            rstep().free()
+                return tot.value
+            rstep()
             bptree.sync()
             t0 = time.perf_counter()
-            reps = 3
+            reps = 5
             for _ in range(reps):
-                d = rstep()
+                rstep()
                 bptree.sync()
-                d.free()
             dt = (time.perf_counter() - t0) / reps
-            d = rstep()
+            tot_ids = rstep()
             cnt_h = dcnt.to_host()
-            ids_h = d.to_host(count=min(d.n, 1 << 24))
-            d.free()
+            ids_h = ids_buf[0].to_host(count=min(tot_ids, 1 << 24))
             if base is None:
                 base = (cnt_h, ids_h)
             forms[f"range_{gname}"] = {"ranges_per_s": nrq / dt, "ms": dt * 1e3, "ids": int(cnt_h.sum()),
                                        "ids_per_s": float(cnt_h.sum()) / dt,
                                        "identical_to_basic": bool(np.array_equal(cnt_h, base[0])
                                                                   and np.array_equal(ids_h, base[1]))}
-        for a in (dlo, dhi, dfirst, dcnt, doff):
+        for a in (dlo, dhi, dfirst, dcnt, doff, ids_buf[0]):
             a.free()
         # batched update: 2^20 upserts (half of them existing keys) merged into
         # the tree, which is rebuilt; wall time of the whole call (allocations included)
