@@ -554,24 +554,16 @@ coop_leaf(const TreeView &t, const K (&qk)[G], const uint32_t (&node)[G], V sent
     return res;
 }
 
-// Phase B alone: the lane's query `myq` sits in node `mynode` of level `top`
-// (top = 0: mynode is its leaf); walks levels top..1 and the leaf.
-template <class K, class V, int F, int LF, bool NT, int OP, int G = 8, bool OFF32 = false>
-__device__ __forceinline__ typename std::conditional<OP == kOpValue, V, uint64_t>::type
-coop_finish64(const TreeView &t, K myq, uint32_t mynode, int top, V sentinel) {
-    static_assert(sizeof(K) == sizeof(V), "cooperative lookup needs equal key/value widths");
+// Inner levels of phase B: group g (lanes Gg..Gg+G-1) walks its G queries
+// qk[] from nodes node[] of level `top` down to their leaves (node[] on
+// return), one load instruction per level fetching 64/G whole nodes.
+template <class K, int F, int G, bool OFF32>
+__device__ __forceinline__ void coop_inner(const TreeView &t, const K (&qk)[G], uint32_t (&node)[G], int top) {
     static_assert(G == 4 || G == 8, "4 or 8 lanes per query");
     constexpr int NB = F * (int)sizeof(K) / G;  // node bytes per lane
     constexpr uint32_t NODE = F * sizeof(K);
     static_assert(NB >= 8, "chunks of at least 8 bytes");
-    const uint32_t lane = threadIdx.x & 63, g = lane / G, sub = lane % G;
-    K qk[G];
-    uint32_t node[G];
-#pragma unroll
-    for (int k = 0; k < G; ++k) {
-        qk[k] = __shfl(myq, (int)(g * G + k), 64);
-        node[k] = (uint32_t)__shfl((int)mynode, (int)(g * G + k), 64);
-    }
+    const uint32_t sub = (threadIdx.x & 63) % G;
     for (int lvl = top; lvl >= 1; --lvl) {
         const uint8_t *lv = t.inner + t.off[lvl];
         const uint32_t last = (uint32_t)t.count[lvl - 1] - 1;
@@ -590,13 +582,30 @@ coop_finish64(const TreeView &t, K myq, uint32_t mynode, int top, V sentinel) {
             node[k] = ch < last ? ch : last;
         }
     }
+}
+
+// Phase B alone: the lane's query `myq` sits in node `mynode` of level `top`
+// (top = 0: mynode is its leaf); walks levels top..1 and the leaf.
+template <class K, class V, int F, int LF, bool NT, int OP, int G = 8, bool OFF32 = false>
+__device__ __forceinline__ typename std::conditional<OP == kOpValue, V, uint64_t>::type
+coop_finish64(const TreeView &t, K myq, uint32_t mynode, int top, V sentinel) {
+    static_assert(sizeof(K) == sizeof(V), "cooperative lookup needs equal key/value widths");
+    const uint32_t lane = threadIdx.x & 63, g = lane / G;
+    K qk[G];
+    uint32_t node[G];
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+        qk[k] = __shfl(myq, (int)(g * G + k), 64);
+        node[k] = (uint32_t)__shfl((int)mynode, (int)(g * G + k), 64);
+    }
+    coop_inner<K, F, G, OFF32>(t, qk, node, top);
     return coop_leaf<K, V, LF, NT, OP, G, OFF32>(t, qk, node, sentinel);
 }
 
-// Phase A (LDS levels, one lane per query) then phase B.
-template <class K, class V, int F, int LF, bool NT, int OP, int G = 8, bool OFF32 = false>
-__device__ __forceinline__ typename std::conditional<OP == kOpValue, V, uint64_t>::type
-coop_lookup64(const TreeView &t, const uint8_t *lds, K myq, V sentinel) {
+// Phase A: the node of the lane's query at the first level walked
+// cooperatively (*top; 0 = a leaf), from the LDS-resident levels.
+template <class K>
+__device__ __forceinline__ uint32_t coop_phase_a(const TreeView &t, const uint8_t *lds, K myq, int *top) {
     const int flat_lvl = t.height - t.lds_levels + 1;
     const bool has_flat = t.lds_levels > 0;
     uint32_t mynode = 0;
@@ -606,8 +615,38 @@ coop_lookup64(const TreeView &t, const uint8_t *lds, K myq, V sentinel) {
         const uint32_t c = lower_bound_flat<K>(flat, flat_n, myq);
         mynode = c < flat_n - 1 ? c : flat_n - 1;
     }
-    const int top = has_flat ? flat_lvl - 1 : t.height;  // first level walked cooperatively
+    *top = has_flat ? flat_lvl - 1 : t.height;
+    return mynode;
+}
+
+// Phase A (LDS levels, one lane per query) then phase B.
+template <class K, class V, int F, int LF, bool NT, int OP, int G = 8, bool OFF32 = false>
+__device__ __forceinline__ typename std::conditional<OP == kOpValue, V, uint64_t>::type
+coop_lookup64(const TreeView &t, const uint8_t *lds, K myq, V sentinel) {
+    int top;
+    const uint32_t mynode = coop_phase_a<K>(t, lds, myq, &top);
     return coop_finish64<K, V, F, LF, NT, OP, G, OFF32>(t, myq, mynode, top, sentinel);
+}
+
+// The leaf of the lane's query: phase A, then the inner levels of phase B
+// with G lanes per node (any leaf format).
+template <class K, int F, int G, bool OFF32>
+__device__ __forceinline__ uint32_t coop_leaf_of(const TreeView &t, const uint8_t *lds, K myq) {
+    int top;
+    const uint32_t mynode = coop_phase_a<K>(t, lds, myq, &top);
+    const uint32_t lane = threadIdx.x & 63, g = lane / G, sub = lane % G;
+    K qk[G];
+    uint32_t node[G];
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+        qk[k] = __shfl(myq, (int)(g * G + k), 64);
+        node[k] = (uint32_t)__shfl((int)mynode, (int)(g * G + k), 64);
+    }
+    coop_inner<K, F, G, OFF32>(t, qk, node, top);
+    uint32_t mine = node[0];
+#pragma unroll
+    for (int k = 1; k < G; ++k) mine = sub == (uint32_t)k ? node[k] : mine;
+    return mine;
 }
 
 // ---------------------------------------------------------------- host helpers

@@ -11,12 +11,13 @@
 //     Tropf & Herzog's BIGMIN (the next code inside the rectangle), staying in
 //     the current leaf when the jump target is near and descending the tree
 //     again otherwise;
-//   * pass 1 counts, the caller scans counts into offsets, pass 2 writes ids,
-//     pass 3 sorts each rectangle's ids ascending.
-// The 2D-grouping and Z-order modes reorder the rectangles by the keys of
-// grouping2d.cu:24-31 / zOrder2d.cu:27-54 applied to (Zmin, Zmax), so that
-// neighbouring lanes walk neighbouring parts of the tree; results are
-// identical in every mode.
+//   * pass 1 counts, with the first descent (to Zmin) run cooperatively;
+//     the caller scans counts into offsets; pass 2 walks the rectangles with
+//     hits again, writes their ids and sorts each run ascending.
+// The 2D-grouping and Z-order modes reorder the rectangles of the count pass
+// by the keys of grouping2d.cu:24-31 / zOrder2d.cu:27-54 applied to
+// (Zmin, Zmax), so that neighbouring lanes walk neighbouring parts of the
+// tree; results are identical in every mode.
 #include <algorithm>
 
 #include "bpt_internal.hpp"
@@ -52,7 +53,9 @@ __device__ __forceinline__ uint32_t compact(uint64_t x) {
 // zmin and zmax, for z outside the box and zmin <= z <= zmax.
 __device__ __forceinline__ uint64_t bigmin(uint64_t z, uint64_t zmin, uint64_t zmax) {
     uint64_t big = 0;
-    for (int i = 63; i >= 0; --i) {
+    // z shares the common prefix of zmin and zmax, where every step is a no-op
+    const uint64_t d = zmin ^ zmax;
+    for (int i = d ? 63 - __clzll(d) : -1; i >= 0; --i) {
         const uint64_t bit = 1ull << i;
         const uint64_t dm = (i & 1) ? 0xAAAAAAAAAAAAAAAAull : 0x5555555555555555ull;
         const uint64_t low = dm & (bit - 1);
@@ -76,14 +79,15 @@ __device__ __forceinline__ uint64_t lb_u64(const TreeView &t, const uint8_t *lds
     return leaf_lower_bound<uint64_t, uint32_t, LF>(t, descend<uint64_t, F>(t, lds, z), z);
 }
 
-// Walks rectangle r; calls emit(slot) for every point inside, in code order.
+// Walks rectangle r from p = lower_bound(zmin); calls emit(slot) for every
+// point inside, in code order.
 template <int F, int LF, class Emit>
-__device__ __forceinline__ void rect_walk(const TreeView &t, const uint8_t *lds, uint4 r, Emit &&emit) {
+__device__ __forceinline__ void rect_walk_from(const TreeView &t, const uint8_t *lds, uint4 r, uint64_t p,
+                                               Emit &&emit) {
     const uint32_t x0 = r.x, y0 = r.y, x1 = r.z, y1 = r.w;
     if (x0 > x1 || y0 > y1) return;
     const uint64_t zmin = spread(x0) | (spread(y0) << 1);
     const uint64_t zmax = spread(x1) | (spread(y1) << 1);
-    uint64_t p = lb_u64<F, LF>(t, lds, zmin);
     while (p < t.n) {
         const uint64_t k = slot_key<uint64_t, uint32_t, LF>(t, p);
         if (k > zmax) break;
@@ -107,6 +111,13 @@ __device__ __forceinline__ void rect_walk(const TreeView &t, const uint8_t *lds,
     }
 }
 
+template <int F, int LF, class Emit>
+__device__ __forceinline__ void rect_walk(const TreeView &t, const uint8_t *lds, uint4 r, Emit &&emit) {
+    if (r.x > r.z || r.y > r.w) return;
+    const uint64_t zmin = spread(r.x) | (spread(r.y) << 1);
+    rect_walk_from<F, LF>(t, lds, r, lb_u64<F, LF>(t, lds, zmin), emit);
+}
+
 template <int F, int LF>
 __global__ __launch_bounds__(256) void k_rect_count(TreeView t, const uint4 *__restrict__ rects,
                                                     const uint32_t *__restrict__ order, uint64_t nr,
@@ -123,39 +134,115 @@ __global__ __launch_bounds__(256) void k_rect_count(TreeView t, const uint4 *__r
     }
 }
 
-// In-place heapsort of a short id run (one thread).
-__device__ void sift(uint32_t *a, uint64_t start, uint64_t end) {
-    uint64_t root = start;
-    while (2 * root + 1 < end) {
-        uint64_t child = 2 * root + 1;
-        if (child + 1 < end && a[child] < a[child + 1]) ++child;
-        if (a[root] < a[child]) {
-            uint32_t tmp = a[root]; a[root] = a[child]; a[child] = tmp;
-            root = child;
-        } else {
-            return;
+// The same counts with the first descent of each rectangle -- to
+// lower_bound(Zmin), nearly all of a rectangle's tree reads when few points
+// fall inside -- run cooperatively: each wave takes 64 rectangles at a time,
+// G lanes per inner node, then one lane per rectangle for its leaf and walk.
+template <int F, int LF, int G, bool OFF32>
+__global__ __launch_bounds__(256) void k_rect_count_coop(TreeView t, const uint4 *__restrict__ rects,
+                                                         const uint32_t *__restrict__ order, uint64_t nr,
+                                                         uint64_t *__restrict__ counts) {
+    extern __shared__ uint4 lds_raw[];
+    uint8_t *lds = reinterpret_cast<uint8_t *>(lds_raw);
+    stage_top_levels(t, lds);
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t stride = ((uint64_t)gridDim.x * blockDim.x) & ~63ull;
+    for (uint64_t base = wave * 64; base < nr; base += stride) {
+        const uint64_t j = base + lane;
+        const bool valid = j < nr;
+        const uint64_t i = valid ? (order ? (uint64_t)order[j] : j) : 0;
+        const uint4 r = valid ? rects[i] : make_uint4(1, 1, 0, 0);  // empty: x0 > x1
+        const uint64_t zmin = spread(r.x) | (spread(r.y) << 1);
+        const uint32_t leaf = coop_leaf_of<uint64_t, F, G, OFF32>(t, lds, zmin);
+        uint64_t c = 0;
+        if (r.x <= r.z && r.y <= r.w)
+            rect_walk_from<F, LF>(t, lds, r, leaf_lower_bound<uint64_t, uint32_t, LF>(t, leaf, zmin),
+                                  [&](uint64_t) { ++c; });
+        if (valid) counts[i] = c;
+    }
+}
+
+constexpr uint32_t kRectSortSmall = 8;    // runs sorted by their own lane, in registers
+constexpr uint32_t kRectSortCap = 1024;   // runs sorted by the whole wave in LDS
+
+// Odd-even transposition network over kRectSortSmall registers.
+__device__ __forceinline__ void sort_small(uint32_t *p, uint32_t c) {
+    uint32_t v[kRectSortSmall];
+#pragma unroll
+    for (uint32_t k = 0; k < kRectSortSmall; ++k) v[k] = k < c ? p[k] : 0xFFFFFFFFu;
+#pragma unroll
+    for (uint32_t r = 0; r < kRectSortSmall; ++r)
+#pragma unroll
+        for (uint32_t k = r & 1; k + 1 < kRectSortSmall; k += 2) {
+            const uint32_t a = v[k] < v[k + 1] ? v[k] : v[k + 1];
+            const uint32_t b = v[k] < v[k + 1] ? v[k + 1] : v[k];
+            v[k] = a;
+            v[k + 1] = b;
+        }
+#pragma unroll
+    for (uint32_t k = 0; k < kRectSortSmall; ++k)
+        if (k < c) p[k] = v[k];
+}
+
+// The whole wave sorts p[0, c), c <= kRectSortCap: a bitonic sort of the run
+// padded to a power of two in the wave's LDS buffer.
+__device__ __forceinline__ void wave_sort_lds(uint32_t *p, uint32_t c, uint32_t *buf) {
+    const uint32_t lane = threadIdx.x & 63;
+    uint32_t m = 64;
+    while (m < c) m <<= 1;
+    for (uint32_t k = lane; k < m; k += 64) buf[k] = k < c ? p[k] : 0xFFFFFFFFu;
+    for (uint32_t size = 2; size <= m; size <<= 1) {
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+            for (uint32_t q = lane; q < m / 2; q += 64) {
+                const uint32_t a = 2 * q - (q & (stride - 1)), b = a + stride;
+                const uint32_t x = buf[a], y = buf[b];
+                if ((x > y) == ((a & size) == 0)) {
+                    buf[a] = y;
+                    buf[b] = x;
+                }
+            }
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    for (uint32_t k = lane; k < c; k += 64) p[k] = buf[k];
+}
+
+// The whole wave sorts p[0, c) of any length in place, in global memory: a
+// bitonic network in its all-ascending form (each merge starts by comparing
+// position i with its mirror in the block, i ^ (size - 1)), so that the
+// positions past c act as +infinity and are never read or written.
+__device__ __forceinline__ void wave_sort_global(uint32_t *p, uint64_t c) {
+    const uint32_t lane = threadIdx.x & 63;
+    uint64_t m = 1;
+    while (m < c) m <<= 1;
+    for (uint64_t size = 2; size <= m; size <<= 1) {
+        for (uint64_t stride = size >> 1; stride > 0; stride >>= 1) {
+            const bool mirror = stride == (size >> 1);
+            for (uint64_t q = lane; q < m / 2; q += 64) {
+                const uint64_t a = 2 * q - (q & (stride - 1));
+                const uint64_t b = mirror ? a ^ (size - 1) : a + stride;
+                if (b < c) {
+                    const uint32_t x = p[a], y = p[b];
+                    if (x > y) {
+                        p[a] = y;
+                        p[b] = x;
+                    }
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
         }
     }
 }
 
-__device__ void sort_run(uint32_t *a, uint64_t n) {
-    if (n < 2) return;
-    if (n <= 16) {
-        for (uint64_t i = 1; i < n; ++i) {
-            uint32_t v = a[i];
-            uint64_t j = i;
-            while (j > 0 && a[j - 1] > v) { a[j] = a[j - 1]; --j; }
-            a[j] = v;
-        }
-        return;
-    }
-    for (uint64_t s = n / 2; s-- > 0;) sift(a, s, n);
-    for (uint64_t e = n - 1; e > 0; --e) {
-        uint32_t tmp = a[0]; a[0] = a[e]; a[e] = tmp;
-        sift(a, 0, e);
-    }
-}
-
+// Ids of the rectangles with hits. A rectangle's count is the gap to the
+// next offset (offsets are the exclusive scan of the counts), so rectangles
+// without points are skipped without a walk; the last one is always walked.
+// Each wave takes 64 rectangles at a time: their lanes walk and write the ids
+// in code order; then runs of up to kRectSortSmall ids are sorted by their
+// own lane in registers, and longer runs by the whole wave, one after
+// another: in LDS up to kRectSortCap ids, in place in global memory beyond.
 template <int F, int LF>
 __global__ __launch_bounds__(256) void k_rect_ids(TreeView t, const uint4 *__restrict__ rects,
                                                   const uint32_t *__restrict__ order, uint64_t nr,
@@ -164,15 +251,36 @@ __global__ __launch_bounds__(256) void k_rect_ids(TreeView t, const uint4 *__res
     extern __shared__ uint4 lds_raw[];
     uint8_t *lds = reinterpret_cast<uint8_t *>(lds_raw);
     stage_top_levels(t, lds);
-    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nr; j += step) {
-        const uint64_t i = order ? order[j] : j;
-        uint32_t *out = ids + offsets[i];
-        uint64_t c = 0;
-        rect_walk<F, LF>(t, lds, rects[i], [&](uint64_t slot) {
-            out[c++] = slot_val<uint64_t, uint32_t, LF>(t, slot);
-        });
-        sort_run(out, c);
+    uint32_t *buf = reinterpret_cast<uint32_t *>(lds + ((t.lds_bytes + 15) & ~15ull)) +
+                    (threadIdx.x >> 6) * kRectSortCap;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t stride = ((uint64_t)gridDim.x * blockDim.x) & ~63ull;
+    for (uint64_t base = wave * 64; base < nr; base += stride) {
+        const uint64_t j = base + lane;
+        uint64_t o = 0, c = 0;
+        if (j < nr) {
+            const uint64_t i = order ? order[j] : j;
+            o = offsets[i];
+            if (!(i + 1 < nr && offsets[i + 1] == o)) {
+                uint32_t *out = ids + o;
+                rect_walk<F, LF>(t, lds, rects[i], [&](uint64_t slot) {
+                    out[c++] = slot_val<uint64_t, uint32_t, LF>(t, slot);
+                });
+            }
+        }
+        if (c > 1 && c <= kRectSortSmall) sort_small(ids + o, (uint32_t)c);
+        // the lanes' writes are visible to the whole wave (one CU, one L1)
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        uint64_t longer = __ballot(c > kRectSortSmall);
+        while (longer) {
+            const int src = __ffsll((unsigned long long)longer) - 1;
+            longer &= longer - 1;
+            const uint64_t so = __shfl(o, src, 64), sc = __shfl(c, src, 64);
+            if (sc <= kRectSortCap) wave_sort_lds(ids + so, (uint32_t)sc, buf);
+            else wave_sort_global(ids + so, sc);
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        }
     }
 }
 
@@ -216,15 +324,33 @@ static bpt_status rect_run(bpt_tree *t, const uint32_t *d_rects, uint64_t nr, in
         if (d_counts) BPT_HIP(hipMemsetAsync(d_counts, 0, nr * 8, s));
         return BPT_OK;
     }
+    // the count pass walks in the requested order; the ids pass walks only the
+    // rectangles with hits, in the caller's order (results are the same)
     uint32_t *order = nullptr;
-    bpt_status st = rect_order(t, d_rects, nr, mode, &order, s);
+    bpt_status st = d_ids ? BPT_OK : rect_order(t, d_rects, nr, mode, &order, s);
     if (st == BPT_OK) {
         st = dispatch_tree(t, [&](auto, auto, auto fc, auto lc) -> bpt_status {
             constexpr int F = decltype(fc)::value, LF = decltype(lc)::value;
             const size_t lds = t->view.lds_bytes ? t->view.lds_bytes : 16;
             if (d_ids) {
-                k_rect_ids<F, LF><<<search_grid(t, nr), t->block, lds, s>>>(
+                const size_t ids_lds = ((t->view.lds_bytes + 15) & ~(size_t)15) +
+                                       (size_t)(t->block / 64) * kRectSortCap * sizeof(uint32_t);
+                k_rect_ids<F, LF><<<search_grid(t, nr), t->block, ids_lds, s>>>(
                     t->view, (const uint4 *)d_rects, order, nr, d_offsets, d_ids);
+            } else if (t->coop && t->view.count[0] < (1ull << 32)) {
+                const bool off32 = t->addr32 && t->inner_bytes <= (1ull << 32);
+                auto go = [&](auto lanes, auto o32) {
+                    k_rect_count_coop<F, LF, decltype(lanes)::value, decltype(o32)::value>
+                        <<<search_grid(t, nr), t->block, lds, s>>>(t->view, (const uint4 *)d_rects, order, nr,
+                                                                  d_counts);
+                };
+                if (t->coop_lanes == 4) {
+                    if (off32) go(IC<4>{}, std::true_type{});
+                    else go(IC<4>{}, std::false_type{});
+                } else {
+                    if (off32) go(IC<8>{}, std::true_type{});
+                    else go(IC<8>{}, std::false_type{});
+                }
             } else {
                 k_rect_count<F, LF><<<search_grid(t, nr), t->block, lds, s>>>(
                     t->view, (const uint4 *)d_rects, order, nr, d_counts);

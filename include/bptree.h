@@ -125,7 +125,8 @@ bpt_status bpt_tree_upsert(bpt_tree *t, const void *d_keys, const void *d_vals, 
 bpt_status bpt_tree_erase(bpt_tree *t, const void *d_keys, uint64_t n, bpt_stream s);
 
 /* Search-kernel options: "coop" (1 = several lanes share each node, the
- * default when key and value widths match; 0 = one lane per query),
+ * default: point and range searches use it when key and value widths
+ * match, the rectangle count for the inner levels; 0 = one lane per query),
  * "lanes" (lanes per query of the cooperative descent: 8, the default, or 4),
  * "leaf_nt" (1 = non-temporal leaf loads, the default),
  * "addr32" (1 = 32-bit byte offsets into levels and leaves below 4 GiB, the
@@ -244,8 +245,11 @@ bpt_status bpt_group2d_keys(const uint64_t *d_a, const uint64_t *d_b, uint64_t n
 /* ------------------------------------------------------------------ */
 /* rects: n x {x0, y0, x1, y1} uint32, inclusive. The tree holds
  * key = Morton(x, y), value = point id. counts[i] = points inside rect i;
- * bpt_rect_ids writes their ids ascending at ids[offsets[i] ...].
- * group_mode orders the queries (bpt_group_mode); results do not change.
+ * bpt_rect_ids writes their ids ascending at ids[offsets[i] ...], with
+ * offsets the exclusive scan of the counts (a rectangle whose offset equals
+ * the next one's has no hits and is skipped).
+ * group_mode orders the queries of the count (bpt_group_mode); results do
+ * not change.
  * The grouped orders use the tree's workspace, like bpt_point_search:
  * concurrent calls on one tree must share one stream. */
 bpt_status bpt_rect_count(bpt_tree *t, const uint32_t *d_rects, uint64_t nr, int group_mode,

@@ -72,6 +72,12 @@ def test_rect_matches_oracle(bpt, oracle, wl, npts, nr, lo_exp, hi_exp):
         c, o, got = t.rect_search(r, group_mode=mode)
         assert np.array_equal(c, want_c), mode
         assert np.array_equal(got, want_ids), mode
+    # the one-lane count and the 4-lane cooperative descent agree too
+    for opt, val in (("lanes", 4), ("coop", 0)):
+        t.set_option(opt, val)
+        c, o, got = t.rect_search(r, group_mode=0)
+        assert np.array_equal(c, want_c), opt
+        assert np.array_equal(got, want_ids), opt
 
 
 def test_rect_duplicate_points(bpt, oracle, wl):
