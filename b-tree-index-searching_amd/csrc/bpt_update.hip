@@ -9,10 +9,13 @@
 //   1. the tree's slots are read out of the leaves in key order;
 //   2. the batch is copied and stably sorted by key; for upserts only the last
 //      pair of each run of equal keys is kept (the last in the caller's order);
-//   3. a slot survives when its key is not in the batch; its place in the
-//      result is its rank among the surviving slots plus the number of batch
-//      keys below it, and an upserted pair's place is its rank in the batch plus
-//      the number of surviving slots below it (a merge with no ties left);
+//   3. each batch key finds its place among the slots by binary search; the
+//      slots holding it are removed, and an upserted key is counted at its
+//      place. A surviving slot's place in the result is its rank among the
+//      surviving slots plus the number of upserted keys below it (a scan of
+//      those counts), and an upserted pair's place is its rank in the batch
+//      plus the number of surviving slots below its place (a merge with no
+//      ties left);
 //   4. the tree is rebuilt from the result with its own options.
 #include <algorithm>
 #include <vector>
@@ -65,41 +68,67 @@ __device__ __forceinline__ uint64_t lower_bound_g(const K *__restrict__ a, uint6
     return lo;
 }
 
-// keep[i] = 1 when slot i's key is not among the m sorted batch keys.
-template <class K>
-__global__ __launch_bounds__(256) void k_survivors(const K *__restrict__ a, uint64_t n, const K *__restrict__ b,
-                                                   uint64_t m, uint32_t *__restrict__ keep) {
+__global__ __launch_bounds__(256) void k_fill_u32(uint32_t *__restrict__ a, uint64_t n, uint32_t v) {
     const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) {
-        const uint64_t p = lower_bound_g<K>(b, m, a[i]);
-        keep[i] = (p < m && b[p] == a[i]) ? 0u : 1u;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) a[i] = v;
+}
+
+// Batch key j finds its place among the tree's sorted slots a: q[j] =
+// lower_bound(a, b[j]) (kept when q is not NULL). The slots holding b[j] are
+// removed (keep = 0).
+template <class K>
+__global__ __launch_bounds__(256) void k_batch_marks(const K *__restrict__ a, uint64_t n, const K *__restrict__ b,
+                                                     uint64_t m, uint32_t *__restrict__ keep, uint32_t *__restrict__ q) {
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m; j += step) {
+        const K key = b[j];
+        const uint64_t p = lower_bound_g<K>(a, n, key);
+        for (uint64_t r = p; r < n && a[r] == key; ++r) keep[r] = 0;
+        if (q) q[j] = (uint32_t)p;
     }
 }
 
+// below[p] = number of upserted keys whose place is p. The batch is sorted, so
+// q is non-decreasing: the last pair of each run of equal places writes the
+// run's length (no atomics, whatever the number of keys at one place).
+__global__ __launch_bounds__(256) void k_batch_runs(const uint32_t *__restrict__ q, uint64_t m,
+                                                    uint32_t *__restrict__ below) {
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m; j += step) {
+        const uint32_t v = q[j];
+        if (j + 1 < m && q[j + 1] == v) continue;
+        below[v] = (uint32_t)(j + 1 - lower_bound_g<uint32_t>(q, j + 1, v));
+    }
+}
+
+// A surviving slot goes after the surviving slots and the upserted keys
+// below it: rank[i] + (excl[i] + below[i]), excl = exclusive scan of below.
 template <class K, class V>
 __global__ __launch_bounds__(256) void k_place_slots(const K *__restrict__ a, const V *__restrict__ av, uint64_t n,
                                                      const uint32_t *__restrict__ keep, const uint32_t *__restrict__ rank,
-                                                     const K *__restrict__ b, uint64_t m, K *__restrict__ ko,
+                                                     const uint32_t *__restrict__ below,
+                                                     const uint32_t *__restrict__ excl, K *__restrict__ ko,
                                                      V *__restrict__ vo) {
     const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step)
         if (keep[i]) {
-            const uint64_t p = (uint64_t)rank[i] + lower_bound_g<K>(b, m, a[i]);
+            const uint64_t p = (uint64_t)rank[i] + (below ? (uint64_t)excl[i] + below[i] : 0);
             ko[p] = a[i];
             vo[p] = av[i];
         }
 }
 
-// Upserted pair j goes after every surviving slot with a smaller key.
+// Upserted pair j goes after every surviving slot with a smaller key: the
+// surviving slots before its place q[j].
 template <class K, class V>
 __global__ __launch_bounds__(256) void k_place_batch(const K *__restrict__ b, const V *__restrict__ bv, uint64_t m,
-                                                     const K *__restrict__ a, uint64_t n,
+                                                     const uint32_t *__restrict__ q, uint64_t n,
                                                      const uint32_t *__restrict__ rank, uint64_t survivors,
                                                      K *__restrict__ ko, V *__restrict__ vo) {
     const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m; j += step) {
-        const uint64_t q = lower_bound_g<K>(a, n, b[j]);
-        const uint64_t p = j + (q < n ? (uint64_t)rank[q] : survivors);
+        const uint64_t qj = q[j];
+        const uint64_t p = j + (qj < n ? (uint64_t)rank[qj] : survivors);
         ko[p] = b[j];
         vo[p] = bv[j];
     }
@@ -184,14 +213,29 @@ bpt_status update_impl(bpt_tree *t, const void *d_keys, const void *d_vals, uint
         bv = cv;
     }
     // 3. survivors and the merge
-    uint32_t *keep = nullptr, *rank = nullptr;
+    uint32_t *keep = nullptr, *rank = nullptr, *below = nullptr, *excl = nullptr, *q = nullptr;
     BPT_TRY(b.get<uint32_t>(n, &keep));
     BPT_TRY(b.get<uint32_t>(n, &rank));
+    const bool counts = upsert && n && mb;
+    if (counts) {
+        BPT_TRY(b.get<uint32_t>(n + 1, &below));
+        BPT_TRY(b.get<uint32_t>(n + 1, &excl));
+        BPT_TRY(b.get<uint32_t>(mb, &q));
+    }
     uint64_t survivors = 0;
     if (n) {
-        k_survivors<K><<<grid_for(n), 256, 0, s>>>(a, n, bk, mb, keep);
+        k_fill_u32<<<grid_for(n), 256, 0, s>>>(keep, n, 1u);
+        if (counts) k_fill_u32<<<grid_for(n + 1), 256, 0, s>>>(below, n + 1, 0u);
+        k_batch_marks<K><<<grid_for(mb), 256, 0, s>>>(a, n, bk, mb, keep, q);
+        if (counts) k_batch_runs<<<grid_for(mb), 256, 0, s>>>(q, mb, below);
         BPT_HIP(hipGetLastError());
         BPT_TRY(scan_flags(keep, n, rank, &survivors, b, s));
+        if (counts) {
+            const uint64_t tb = scan_tmp_bytes(n + 1, 4);
+            uint8_t *tmp = nullptr;
+            BPT_TRY(b.get<uint8_t>(tb, &tmp));
+            BPT_TRY(exclusive_scan<uint32_t>(below, n + 1, excl, tmp, tb, s));
+        }
     }
     const uint64_t total = survivors + (upsert ? mb : 0);
     K *ok = nullptr;
@@ -200,12 +244,18 @@ bpt_status update_impl(bpt_tree *t, const void *d_keys, const void *d_vals, uint
     BPT_TRY(b.get<V>(total, &ov));
     if (n) {
         // erased keys take no places: only upserted keys count below a slot
-        k_place_slots<K, V><<<grid_for(n), 256, 0, s>>>(a, av, n, keep, rank, bk, upsert ? mb : 0, ok, ov);
+        k_place_slots<K, V><<<grid_for(n), 256, 0, s>>>(a, av, n, keep, rank, counts ? below : nullptr, excl, ok,
+                                                        ov);
         BPT_HIP(hipGetLastError());
     }
     if (upsert && mb) {
-        k_place_batch<K, V><<<grid_for(mb), 256, 0, s>>>(bk, bv, mb, a, n, rank, survivors, ok, ov);
-        BPT_HIP(hipGetLastError());
+        if (n) {
+            k_place_batch<K, V><<<grid_for(mb), 256, 0, s>>>(bk, bv, mb, q, n, rank, survivors, ok, ov);
+            BPT_HIP(hipGetLastError());
+        } else {
+            BPT_HIP(hipMemcpyAsync(ok, bk, mb * sizeof(K), hipMemcpyDeviceToDevice, s));
+            BPT_HIP(hipMemcpyAsync(ov, bv, mb * sizeof(V), hipMemcpyDeviceToDevice, s));
+        }
     }
     // 4. rebuild with the tree's options
     return tree_rebuild(t, ok, ov, total, s);

@@ -80,6 +80,31 @@ def test_rect_matches_oracle(bpt, oracle, wl, npts, nr, lo_exp, hi_exp):
         assert np.array_equal(got, want_ids), opt
 
 
+def test_rect_id_runs_at_sort_limits(bpt):
+    """Runs of exactly c ids, c on both sides of each sort tier (lane
+    registers up to 8, wave LDS up to 1024, wave global memory beyond),
+    several in one wave and mixed with empty rectangles."""
+    npts = 70_000
+    rng = np.random.default_rng(5)
+    x = np.arange(npts, dtype=np.uint32)
+    y = np.zeros(npts, dtype=np.uint32)
+    ids = rng.permutation(npts).astype(np.uint32)
+    z = np.zeros(npts, dtype=np.uint64)
+    for b in range(17):  # Morton code of (x, 0): x's bits spread to the even positions
+        z |= ((x.astype(np.uint64) >> np.uint64(b)) & np.uint64(1)) << np.uint64(2 * b)
+    t = bpt.Tree(z, ids, 8, 4)
+    sizes = [0, 1, 2, 7, 8, 9, 63, 64, 65, 1023, 1024, 1025, 2048, 4097, 65_537, 3, 0, 9, 1025]
+    starts = rng.integers(0, npts - 65_537, len(sizes))
+    rects = np.array([[s, 0, s + c - 1, 0] if c else [5, 1, 4, 1] for s, c in zip(starts, sizes)],
+                     dtype=np.uint32)
+    want_c = np.array(sizes, dtype=np.uint64)
+    want_ids = np.concatenate([np.sort(ids[s:s + c]) for s, c in zip(starts, sizes)])
+    for mode in MODES:
+        c, o, got = t.rect_search(rects, group_mode=mode)
+        assert np.array_equal(c, want_c), mode
+        assert np.array_equal(got, want_ids), mode
+
+
 def test_rect_duplicate_points(bpt, oracle, wl):
     rng = np.random.default_rng(9)
     x = rng.integers(0, 64, size=5000, dtype=np.uint64).astype(np.uint32)   # many coincident points

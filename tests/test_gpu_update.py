@@ -92,6 +92,16 @@ def test_upsert_erase_sequence(bpt, oracle, kb, vb, F, LF):
     t.upsert(k2[:1], v2[:1]); m.upsert(k2[:1], v2[:1])  # a batch of one
     _check(bpt, oracle, t, m, rng, kb, vb, "single")
 
+    # many keys at one place: a run inside the widest gap and a run above the maximum
+    gi = int(np.argmax(np.diff(m.k[:20_000].astype(np.float64))))
+    g = int(min(int(m.k[gi + 1]) - int(m.k[gi]) - 1, 3000))
+    inside = (np.uint64(m.k[gi]) + np.arange(1, g + 1, dtype=np.uint64)).astype(kdt)
+    above = (np.uint64(top) - np.arange(3000, dtype=np.uint64)).astype(kdt)
+    k3 = np.concatenate([inside, above])
+    v3 = rng.integers(0, (1 << (8 * vb)) - 1, len(k3), dtype=np.uint64).astype(vdt)
+    t.upsert(k3, v3); m.upsert(k3, v3)
+    _check(bpt, oracle, t, m, rng, kb, vb, "clustered")
+
     allk = m.k.copy()
     t.erase(allk); m.erase(allk)                      # everything: an empty tree
     assert t.info()["n"] == 0
