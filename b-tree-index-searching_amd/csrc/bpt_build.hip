@@ -170,26 +170,38 @@ static bpt_status check_sorted(const void *d_keys, uint64_t n, hipStream_t s, bo
     return BPT_OK;
 }
 
+// Device buffers for the current layout. Buffers that hold the new size and
+// are at most twice it are kept; otherwise they are reallocated, with 1/16
+// slack when `slack` is set so that a run of growing updates rarely
+// reallocates.
+static bpt_status ensure_storage(bpt_tree *t, bool slack) {
+    auto fit = [&](uint8_t *&p, uint64_t &cap, uint64_t need, const char *what) -> bpt_status {
+        if (p && need <= cap && need * 2 >= cap) return BPT_OK;
+        (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        if (need == 0) return BPT_OK;
+        const uint64_t want = slack ? need + need / 16 : need;
+        hipError_t e = hipMalloc(&p, want);
+        if (e != hipSuccess) return hip_fail(e, what, __FILE__, __LINE__);
+        cap = want;
+        return BPT_OK;
+    };
+    bpt_status st = fit(t->d_leaves, t->leaf_alloc, t->leaf_bytes, "hipMalloc(leaves)");
+    if (st == BPT_OK) st = fit(t->d_inner, t->inner_alloc, t->inner_bytes, "hipMalloc(inner)");
+    t->view.inner = t->d_inner;
+    t->view.leaves = t->d_leaves;
+    return st;
+}
+
 bpt_status tree_rebuild(bpt_tree *t, const void *d_keys, const void *d_vals, uint64_t n, hipStream_t s) {
     if (t->learned) {
         learned_free(t->learned);
         t->learned = nullptr;
     }
-    (void)hipFree(t->d_leaves);
-    (void)hipFree(t->d_inner);
-    t->d_leaves = t->d_inner = nullptr;
     t->n = n;
     bpt_status st = compute_layout(t);
-    if (st == BPT_OK && t->leaf_bytes) {
-        hipError_t e = hipMalloc(&t->d_leaves, t->leaf_bytes);
-        if (e != hipSuccess) st = hip_fail(e, "hipMalloc(leaves)", __FILE__, __LINE__);
-    }
-    if (st == BPT_OK && t->inner_bytes) {
-        hipError_t e = hipMalloc(&t->d_inner, t->inner_bytes);
-        if (e != hipSuccess) st = hip_fail(e, "hipMalloc(inner)", __FILE__, __LINE__);
-    }
-    t->view.inner = t->d_inner;
-    t->view.leaves = t->d_leaves;
+    if (st == BPT_OK) st = ensure_storage(t, true);
     if (st == BPT_OK) {
         st = dispatch_tree(t, [&](auto kt, auto vt, auto fc, auto lc) {
             using K = typename decltype(kt)::type;
@@ -205,8 +217,10 @@ bpt_status tree_rebuild(bpt_tree *t, const void *d_keys, const void *d_vals, uin
         (void)hipFree(t->d_leaves);
         (void)hipFree(t->d_inner);
         t->d_leaves = t->d_inner = nullptr;
+        t->leaf_alloc = t->inner_alloc = 0;
         t->n = 0;
         (void)compute_layout(t);
+        t->view.inner = t->view.leaves = nullptr;
     }
     return st;
 }
@@ -250,16 +264,7 @@ bpt_status bpt_tree_build(const bpt_build_desc *desc, const void *d_keys, const 
     (void)hipGetDevice(&t->device);
 
     bpt_status st = compute_layout(t);
-    if (st == BPT_OK && t->leaf_bytes) {
-        hipError_t e = hipMalloc(&t->d_leaves, t->leaf_bytes);
-        if (e != hipSuccess) st = hip_fail(e, "hipMalloc(leaves)", __FILE__, __LINE__);
-    }
-    if (st == BPT_OK && t->inner_bytes) {
-        hipError_t e = hipMalloc(&t->d_inner, t->inner_bytes);
-        if (e != hipSuccess) st = hip_fail(e, "hipMalloc(inner)", __FILE__, __LINE__);
-    }
-    t->view.inner = t->d_inner;
-    t->view.leaves = t->d_leaves;
+    if (st == BPT_OK) st = ensure_storage(t, false);
     if (st == BPT_OK) {
         st = dispatch_tree(t, [&](auto kt, auto vt, auto fc, auto lc) {
             using K = typename decltype(kt)::type;

@@ -75,6 +75,7 @@ struct bpt_tree {
     uint8_t *d_leaves = nullptr;
     uint8_t *d_inner = nullptr;
     uint64_t leaf_bytes = 0, inner_bytes = 0;
+    uint64_t leaf_alloc = 0, inner_alloc = 0;  // allocated bytes (a rebuild reuses what fits)
     int device = 0;
     uint32_t block = 256, blocks_per_cu = 32;
     uint32_t group_bits = 12;
@@ -657,6 +658,12 @@ template <class T>
 bpt_status exclusive_scan(const T *d_in, uint64_t n, T *d_out, void *d_tmp, uint64_t tmp_bytes,
                           hipStream_t s);
 uint64_t scan_tmp_bytes(uint64_t n, uint64_t elem_bytes);
+// LSD radix sort (8-bit stable passes over bits [lo_bit, hi_bit)) of keys with
+// u32 values (original indices when vals_in is NULL); results point into ws.
+uint64_t sort_ws_bytes(uint64_t n, uint32_t key_bytes);
+template <class KT>
+bpt_status sort_pairs(const KT *keys_in, const uint32_t *vals_in, uint64_t n, uint32_t lo_bit, uint32_t hi_bit,
+                      void *ws, uint64_t ws_bytes, KT **keys_sorted, uint32_t **vals_sorted, hipStream_t s);
 bpt_status point_grouped(const bpt_tree *t, const void *d_q, uint64_t nq, void *d_out,
                          void *d_ws, uint64_t ws_bytes, hipStream_t s);
 uint64_t point_grouped_ws(const bpt_tree *t, uint64_t nq);

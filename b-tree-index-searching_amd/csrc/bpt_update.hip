@@ -18,7 +18,6 @@
 //      ties left);
 //   4. the tree is rebuilt from the result with its own options.
 #include <algorithm>
-#include <vector>
 
 #include "bpt_internal.hpp"
 
@@ -136,18 +135,18 @@ __global__ __launch_bounds__(256) void k_place_batch(const K *__restrict__ b, co
 
 namespace {
 
-// Device buffers of one update, freed together.
-struct Bufs {
-    std::vector<void *> p;
-    ~Bufs() {
-        for (void *x : p) (void)hipFree(x);
-    }
+// Bump allocator over one device buffer (the tree's workspace), 256-byte
+// aligned pieces.
+struct Arena {
+    uint8_t *base = nullptr;
+    uint64_t cap = 0, used = 0;
+    static uint64_t bytes(uint64_t n) { return (std::max<uint64_t>(n, 1) + 255) & ~255ull; }
     template <class T>
     bpt_status get(uint64_t count, T **out) {
-        void *x = nullptr;
-        BPT_HIP(hipMalloc(&x, std::max<uint64_t>(count, 1) * sizeof(T)));
-        p.push_back(x);
-        *out = static_cast<T *>(x);
+        const uint64_t b = bytes(count * sizeof(T));
+        BPT_CHECK(used + b <= cap, BPT_ERR_INVALID, "update workspace too small");
+        *out = reinterpret_cast<T *>(base + used);
+        used += b;
         return BPT_OK;
     }
 };
@@ -157,12 +156,12 @@ uint32_t grid_for(uint64_t n) {
 }
 
 // Exclusive scan of n u32 flags into rank; returns the total in *total.
-bpt_status scan_flags(const uint32_t *flag, uint64_t n, uint32_t *rank, uint64_t *total, Bufs &b, hipStream_t s) {
+bpt_status scan_flags(const uint32_t *flag, uint64_t n, uint32_t *rank, uint64_t *total, Arena &ar, hipStream_t s) {
     *total = 0;
     if (n == 0) return BPT_OK;
     const uint64_t tb = scan_tmp_bytes(n, 4);
     uint8_t *tmp = nullptr;
-    BPT_TRY(b.get<uint8_t>(tb, &tmp));
+    BPT_TRY(ar.get<uint8_t>(tb, &tmp));
     BPT_TRY(exclusive_scan<uint32_t>(flag, n, rank, tmp, tb, s));
     uint32_t last_rank = 0, last_flag = 0;
     BPT_HIP(hipMemcpyAsync(&last_rank, rank + n - 1, 4, hipMemcpyDeviceToHost, s));
@@ -172,41 +171,60 @@ bpt_status scan_flags(const uint32_t *flag, uint64_t n, uint32_t *rank, uint64_t
     return BPT_OK;
 }
 
+// Workspace bytes of one update of an n-slot tree by an m-pair batch (an
+// upper bound of what update_impl takes from its arena).
+template <class K, class V>
+uint64_t update_ws_bytes(uint64_t n, uint64_t m) {
+    const auto B = Arena::bytes;
+    return B(n * sizeof(K)) + B(n * sizeof(V))                              // the slots
+           + B(sort_ws_bytes(m, sizeof(K))) + B(m * sizeof(V))              // sorted batch
+           + 2 * B(m * 4) + B(scan_tmp_bytes(m, 4)) + B(m * sizeof(K)) + B(m * sizeof(V))  // dedupe
+           + 2 * B(n * 4) + 2 * B((n + 1) * 4) + B(m * 4)                   // marks, ranks, places
+           + B(scan_tmp_bytes(n, 4)) + B(scan_tmp_bytes(n + 1, 4))
+           + B((n + m) * sizeof(K)) + B((n + m) * sizeof(V));               // the merge
+}
+
 template <class K, class V, int LF>
 bpt_status update_impl(bpt_tree *t, const void *d_keys, const void *d_vals, uint64_t m, bool upsert, hipStream_t s) {
-    Bufs b;
     const uint64_t n = t->n;
+    const uint64_t need = update_ws_bytes<K, V>(n, m);
+    BPT_TRY(t->ws.reserve(need));
+    Arena ar{reinterpret_cast<uint8_t *>(t->ws.ptr), need, 0};
     // 1. the tree's slots in key order
     K *a = nullptr;
     V *av = nullptr;
-    BPT_TRY(b.get<K>(n, &a));
-    BPT_TRY(b.get<V>(n, &av));
+    BPT_TRY(ar.get<K>(n, &a));
+    BPT_TRY(ar.get<V>(n, &av));
     if (n) {
         k_extract_slots<K, V, LF><<<grid_for(n), 256, 0, s>>>(t->view, a, av);
         BPT_HIP(hipGetLastError());
     }
-    // 2. the batch, sorted (stable), deduplicated for upserts
+    // 2. the batch, sorted (stable LSD radix sort), deduplicated for upserts
     K *bk = nullptr;
     V *bv = nullptr;  // erase: keys only
-    BPT_TRY(b.get<K>(m, &bk));
-    BPT_HIP(hipMemcpyAsync(bk, d_keys, m * sizeof(K), hipMemcpyDeviceToDevice, s));
-    if (upsert) {
-        BPT_TRY(b.get<V>(m, &bv));
-        BPT_HIP(hipMemcpyAsync(bv, d_vals, m * sizeof(V), hipMemcpyDeviceToDevice, s));
+    {
+        const uint64_t sb = sort_ws_bytes(m, sizeof(K));
+        uint8_t *sws = nullptr;
+        BPT_TRY(ar.get<uint8_t>(sb, &sws));
+        uint32_t *perm = nullptr;
+        BPT_TRY(sort_pairs<K>((const K *)d_keys, nullptr, m, 0, 8 * sizeof(K), sws, sb, &bk, &perm, s));
+        if (upsert) {
+            BPT_TRY(ar.get<V>(m, &bv));
+            BPT_TRY(bpt_gather(d_vals, sizeof(V), perm, m, bv, reinterpret_cast<bpt_stream>(s)));
+        }
     }
-    BPT_TRY(bpt_sort_pairs(bk, bv, sizeof(K), sizeof(V), m, reinterpret_cast<bpt_stream>(s)));
     uint64_t mb = m;
     if (upsert && m > 1) {
         uint32_t *flag = nullptr, *pos = nullptr;
-        BPT_TRY(b.get<uint32_t>(m, &flag));
-        BPT_TRY(b.get<uint32_t>(m, &pos));
+        BPT_TRY(ar.get<uint32_t>(m, &flag));
+        BPT_TRY(ar.get<uint32_t>(m, &pos));
         k_run_last<K><<<grid_for(m), 256, 0, s>>>(bk, m, flag);
         BPT_HIP(hipGetLastError());
-        BPT_TRY(scan_flags(flag, m, pos, &mb, b, s));
+        BPT_TRY(scan_flags(flag, m, pos, &mb, ar, s));
         K *ck = nullptr;
         V *cv = nullptr;
-        BPT_TRY(b.get<K>(mb, &ck));
-        BPT_TRY(b.get<V>(mb, &cv));
+        BPT_TRY(ar.get<K>(mb, &ck));
+        BPT_TRY(ar.get<V>(mb, &cv));
         k_compact<K, V><<<grid_for(m), 256, 0, s>>>(bk, bv, flag, pos, m, ck, cv);
         BPT_HIP(hipGetLastError());
         bk = ck;
@@ -214,13 +232,13 @@ bpt_status update_impl(bpt_tree *t, const void *d_keys, const void *d_vals, uint
     }
     // 3. survivors and the merge
     uint32_t *keep = nullptr, *rank = nullptr, *below = nullptr, *excl = nullptr, *q = nullptr;
-    BPT_TRY(b.get<uint32_t>(n, &keep));
-    BPT_TRY(b.get<uint32_t>(n, &rank));
+    BPT_TRY(ar.get<uint32_t>(n, &keep));
+    BPT_TRY(ar.get<uint32_t>(n, &rank));
     const bool counts = upsert && n && mb;
     if (counts) {
-        BPT_TRY(b.get<uint32_t>(n + 1, &below));
-        BPT_TRY(b.get<uint32_t>(n + 1, &excl));
-        BPT_TRY(b.get<uint32_t>(mb, &q));
+        BPT_TRY(ar.get<uint32_t>(n + 1, &below));
+        BPT_TRY(ar.get<uint32_t>(n + 1, &excl));
+        BPT_TRY(ar.get<uint32_t>(mb, &q));
     }
     uint64_t survivors = 0;
     if (n) {
@@ -229,19 +247,19 @@ bpt_status update_impl(bpt_tree *t, const void *d_keys, const void *d_vals, uint
         k_batch_marks<K><<<grid_for(mb), 256, 0, s>>>(a, n, bk, mb, keep, q);
         if (counts) k_batch_runs<<<grid_for(mb), 256, 0, s>>>(q, mb, below);
         BPT_HIP(hipGetLastError());
-        BPT_TRY(scan_flags(keep, n, rank, &survivors, b, s));
+        BPT_TRY(scan_flags(keep, n, rank, &survivors, ar, s));
         if (counts) {
             const uint64_t tb = scan_tmp_bytes(n + 1, 4);
             uint8_t *tmp = nullptr;
-            BPT_TRY(b.get<uint8_t>(tb, &tmp));
+            BPT_TRY(ar.get<uint8_t>(tb, &tmp));
             BPT_TRY(exclusive_scan<uint32_t>(below, n + 1, excl, tmp, tb, s));
         }
     }
     const uint64_t total = survivors + (upsert ? mb : 0);
     K *ok = nullptr;
     V *ov = nullptr;
-    BPT_TRY(b.get<K>(total, &ok));
-    BPT_TRY(b.get<V>(total, &ov));
+    BPT_TRY(ar.get<K>(total, &ok));
+    BPT_TRY(ar.get<V>(total, &ov));
     if (n) {
         // erased keys take no places: only upserted keys count below a slot
         k_place_slots<K, V><<<grid_for(n), 256, 0, s>>>(a, av, n, keep, rank, counts ? below : nullptr, excl, ok,
@@ -257,7 +275,7 @@ bpt_status update_impl(bpt_tree *t, const void *d_keys, const void *d_vals, uint
             BPT_HIP(hipMemcpyAsync(ov, bv, mb * sizeof(V), hipMemcpyDeviceToDevice, s));
         }
     }
-    // 4. rebuild with the tree's options
+    // 4. rebuild with the tree's options (the old leaves are no longer read)
     return tree_rebuild(t, ok, ov, total, s);
 }
 

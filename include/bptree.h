@@ -120,7 +120,10 @@ bpt_status bpt_tree_set_launch(bpt_tree *t, uint32_t block, uint32_t blocks_per_
  * whose key is one of d_keys is removed (absent keys are ignored). The batch
  * is merged with the tree's slots on the device and the tree is rebuilt with
  * its own options (O(tree + batch) per call); a trained learned model is
- * dropped. Synchronize s. On failure the tree is left empty. */
+ * dropped. The merge's scratch is the tree's workspace (about twice the
+ * tree's slot bytes plus 16 B per slot), kept for later calls; the leaf and
+ * inner buffers are reused when the new tree fits them. Synchronize s. On
+ * failure the tree is left empty. */
 bpt_status bpt_tree_upsert(bpt_tree *t, const void *d_keys, const void *d_vals, uint64_t n, bpt_stream s);
 bpt_status bpt_tree_erase(bpt_tree *t, const void *d_keys, uint64_t n, bpt_stream s);
 
