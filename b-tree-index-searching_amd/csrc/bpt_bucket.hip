@@ -291,48 +291,60 @@ __global__ __launch_bounds__(kPBlock) void k_bkt_scatter(const K *__restrict__ q
     uint32_t *lcur = delta + d.nb;
     uint32_t *sbl = lcur + d.nb;
     uint16_t *btab = reinterpret_cast<uint16_t *>(sbl + d.tile);
-    const uint32_t tile = xcd_contiguous(blockIdx.x, gridDim.x);
     constexpr int E = per_thread<K>();
-    const uint64_t lo = (uint64_t)tile * d.tile;
-    const uint32_t cnt = (uint32_t)(std::min<uint64_t>(lo + d.tile, nq) - lo);
-    K key[E];  // issued first: their latency overlaps the staging and the scan
+    // persistent, one round of gridDim.x consecutive tiles at a time; in each
+    // round an XCD's blocks take consecutive tiles, whose runs of one bucket
+    // are adjacent in the output (their writes meet in that XCD's L2)
+    const uint32_t t0 = xcd_contiguous(blockIdx.x, gridDim.x), t1 = d.ntiles;
+    auto load_tile = [&](uint32_t tile, K (&key)[E]) {
+        const uint64_t lo = (uint64_t)tile * d.tile;
+        const uint32_t cnt = tile < t1 ? (uint32_t)(std::min<uint64_t>(lo + d.tile, nq) - lo) : 0u;
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-        const uint32_t i = e * kPBlock + threadIdx.x;
-        key[e] = i < cnt ? __builtin_nontemporal_load(&q[lo + i]) : K(0);
-    }
-    for (uint32_t b = threadIdx.x; b < d.nb; b += blockDim.x) {
-        bsep[b] = bsep_g[b];
-        lcur[b] = hist[(uint64_t)tile * d.nb + b];  // this tile's count of bucket b
-    }
+        for (int e = 0; e < E; ++e) {
+            const uint32_t i = e * kPBlock + threadIdx.x;
+            key[e] = i < cnt ? __builtin_nontemporal_load(&q[lo + i]) : K(0);
+        }
+    };
+    K key[E];  // issued first: their latency overlaps the staging and the scan
+    load_tile(t0, key);
+    for (uint32_t b = threadIdx.x; b < d.nb; b += blockDim.x) bsep[b] = bsep_g[b];
     for (uint32_t p = threadIdx.x; p <= (1u << kBucketTabBits); p += blockDim.x) btab[p] = btab_g[p];
     const PrefixIndex<K> pi{(K)hdr[0], (uint32_t)hdr[1]};
-    __syncthreads();
-    block_exclusive_scan(lcur, d.nb, wsum);  // first tile slot of every bucket
-    for (uint32_t b = threadIdx.x; b < d.nb; b += blockDim.x) delta[b] = off[(uint64_t)tile * d.nb + b] - lcur[b];
-    __syncthreads();
-    uint32_t bk[E];
+    for (uint32_t tile = t0; tile < t1; tile += gridDim.x) {
+        const uint64_t lo = (uint64_t)tile * d.tile;
+        const uint32_t cnt = (uint32_t)(std::min<uint64_t>(lo + d.tile, nq) - lo);
+        __syncthreads();  // the previous tile's write-out has read sk, sbl and delta
+        for (uint32_t b = threadIdx.x; b < d.nb; b += blockDim.x)
+            lcur[b] = hist[(uint64_t)tile * d.nb + b];  // this tile's count of bucket b
+        __syncthreads();
+        block_exclusive_scan(lcur, d.nb, wsum);  // first tile slot of every bucket
+        for (uint32_t b = threadIdx.x; b < d.nb; b += blockDim.x)
+            delta[b] = off[(uint64_t)tile * d.nb + b] - lcur[b];
+        __syncthreads();
+        uint32_t bk[E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) bk[e] = bucket_of<K>(bsep, d.nb, btab, pi, key[e]);
+        for (int e = 0; e < E; ++e) bk[e] = bucket_of<K>(bsep, d.nb, btab, pi, key[e]);
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-        const uint32_t i = e * kPBlock + threadIdx.x;
-        if (i < cnt) {
-            const uint32_t slot = atomicAdd(&lcur[bk[e]], 1u);
-            sk[slot] = key[e];
-            sbl[slot] = (bk[e] << 16) | i;
+        for (int e = 0; e < E; ++e) {
+            const uint32_t i = e * kPBlock + threadIdx.x;
+            if (i < cnt) {
+                const uint32_t slot = atomicAdd(&lcur[bk[e]], 1u);
+                sk[slot] = key[e];
+                sbl[slot] = (bk[e] << 16) | i;
+            }
         }
-    }
-    __syncthreads();
-    // tile-sorted element j goes to its bucket run: consecutive j, consecutive slots
+        load_tile(tile + gridDim.x, key);  // the next tile's queries arrive during the write-out
+        __syncthreads();
+        // tile-sorted element j goes to its bucket run: consecutive j, consecutive slots
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-        const uint32_t j = e * kPBlock + threadIdx.x;
-        if (j < cnt) {
-            const uint32_t x = sbl[j];
-            const uint32_t g = delta[x >> 16] + j;
-            qs[g] = sk[j];
-            lidx[g] = (uint16_t)(x & 0xFFFFu);
+        for (int e = 0; e < E; ++e) {
+            const uint32_t j = e * kPBlock + threadIdx.x;
+            if (j < cnt) {
+                const uint32_t x = sbl[j];
+                const uint32_t g = delta[x >> 16] + j;
+                qs[g] = sk[j];
+                lidx[g] = (uint16_t)(x & 0xFFFFu);
+            }
         }
     }
 }
@@ -583,7 +595,8 @@ static bpt_status bucket_chunk(const bpt_tree *t, const K *q, uint64_t nq, V *ou
     BPT_HIP(hipGetLastError());
     k_bkt_colprefix<<<d.ngroups, kBBlock, 0, s>>>(hist, cols, bstart, d, offs);
     BPT_HIP(hipGetLastError());
-    k_bkt_scatter<K><<<d.ntiles, kPBlock, p.lds_scatter, s>>>(q, nq, d, bsep, hdr, btab, hist, offs, qs, lidx);
+    k_bkt_scatter<K><<<std::min<uint32_t>(d.ntiles, (uint32_t)device_cu_count()), kPBlock, p.lds_scatter, s>>>(
+        q, nq, d, bsep, hdr, btab, hist, offs, qs, lidx);
     BPT_HIP(hipGetLastError());
     const V sentinel = ~V(0);
     bool coop = false;
