@@ -81,12 +81,18 @@ def cpu_baseline(keys, vals, q, gpu_out, target_s=10.0):
     res = pt.point(sample, 0xFFFFFFFF, threads)
     dt = time.perf_counter() - t0
     mism = int((res.astype(np.uint32) != gpu_out[:s]).sum())
+    one = q[: 1 << 20].astype(np.uint64)  # the same search on one core, a smaller sample
+    t0 = time.perf_counter()
+    res1 = pt.point(one, 0xFFFFFFFF, 1)
+    dt1 = time.perf_counter() - t0
+    mism += int((res1.astype(np.uint32) != gpu_out[: len(one)]).sum())
     pt.close()
     return {
         "value": s / dt, "unit": "lookups/s", "cores": threads, "kind": "port",
         "sample": f"first {s} of the {len(q)} uniform-hit queries against the full 64M-key page "
                   f"tree (PAGESIZE 5000, reference default), OpenMP static schedule, "
                   f"{dt:.1f} s; page-tree build {build_s:.1f} s excluded",
+        "one_core": {"value": len(one) / dt1, "sample": f"first {len(one)} queries, {dt1:.1f} s"},
         "mismatches_vs_gpu": mism,
     }
 
@@ -457,6 +463,23 @@ def main():
         b.free()
         log("ran the 1 GiB copy kernel 3 times (reads 1 GiB, writes 1 GiB each)")
 
+    # stream-copy bandwidth of this GPU (1 GiB device-to-device, read + write),
+    # the practical ceiling the roofline fraction is also compared with
+    a = bptree.DeviceArray(1 << 28, np.uint32)
+    b = bptree.DeviceArray(1 << 28, np.uint32)
+    copy_ms = []
+    for _ in range(4):
+        ct = bptree.Timer()
+        ct.start()
+        bptree.check(bptree.lib().bpt_copy_d2d(b.ptr, a.ptr, a.nbytes, None))
+        ct.stop()
+        copy_ms.append(ct.ms())
+        ct.close()
+    a.free()
+    b.free()
+    copy_gbps = 2 * (1 << 30) / (float(np.median(copy_ms[1:])) * 1e-3) / 1e9
+    log(f"1 GiB device copy (k_copy_u4, read + write): {copy_gbps:.0f} GB/s")
+
     dq = bptree.DeviceArray.from_host(q)
     dout = bptree.DeviceArray(nq, np.uint32)
     if args.grouped_only:
@@ -475,6 +498,21 @@ def main():
 
     forms = {"plain_uniform": {"lookups_per_s": value, "kernel_ms": kernel_ms}}
     if not args.no_forms:
+        # end to end from host memory: queries H2D, plain search, results D2H
+        # (pageable numpy buffers, synchronous copies), as a host caller sees it
+        host_out = np.empty(nq, np.uint32)
+        e2e = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            bptree.check(bptree.lib().bpt_copy_h2d(dq.ptr, q.ctypes.data, q.nbytes, None))
+            tree.point_search_dev(dq, dout, bptree.PLAIN)
+            bptree.check(bptree.lib().bpt_copy_d2h(host_out.ctypes.data, dout.ptr, host_out.nbytes, None))
+            e2e.append(time.perf_counter() - t0)
+        wall_e = float(np.median(e2e))
+        forms["plain_end_to_end"] = {"lookups_per_s": nq / wall_e, "ms": wall_e * 1e3,
+                                     "identical_to_resident": bool(np.array_equal(host_out, gpu_out)),
+                                     "note": "pageable host buffers; H2D 1 GiB + search + D2H 1 GiB"}
+        del host_out
         wall_g, ev_g = time_form(tree, dq, dout, bptree.GROUPED, max(3, args.steps // 2), 1)
         forms["grouped_uniform"] = {"lookups_per_s": nq / wall_g, "ms": wall_g * 1e3,
                                     "speedup_vs_plain": wall / wall_g,
@@ -595,6 +633,7 @@ def main():
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "kernel": "k_point_coop", "kernel_ms": kernel_ms,
             "bytes_per_lookup": BYTES_PER_LOOKUP,
+            "stream_copy_GBps": copy_gbps, "frac_of_copy": achieved / copy_gbps,
         },
         "forms": forms,
     }
