@@ -173,6 +173,66 @@ def bench_c3(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_c1(args):
+    """The reference's CPU-runnable case: 1M unique uint32 keys, value = rank,
+    1M queries (half from the key set, half uniform uint32). The page tree of
+    the reference's search restated on the CPU (the oracle) at order 32 and at
+    the reference default PAGESIZE 5000, on all host cores and on one (median
+    of 5 after a warm-up, tree build excluded); the GPU plain search on the same
+    inputs is the parity check and is reported beside it."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle  # the checker; this configuration measures it as the CPU baseline
+
+    n = nq = 1 << 20
+    keys, vals = workloads.point_keys(n, workload=1)
+    q = workloads.cpu_queries(keys, nq, workload=1)
+    q64 = q.astype(np.uint64)
+    threads = min(16, len(os.sched_getaffinity(0)))
+    tree = bptree.Tree(keys, vals)
+    dq = bptree.DeviceArray.from_host(q)
+    dout = bptree.DeviceArray(nq, np.uint32)
+    wall_g, ev = time_form(tree, dq, dout, bptree.PLAIN, max(args.steps, 10), args.warmup)
+    gpu = dout.to_host()
+    forms, mism = {}, 0
+    for ps in (32, 5000):
+        pt = oracle.PageTree(keys.astype(np.uint64), vals.astype(np.uint64), ps)
+        for cores in (threads, 1):
+            pt.point(q64, 0xFFFFFFFF, cores)  # warm-up
+            ts = []
+            for _ in range(5):
+                t0 = time.perf_counter()
+                res = pt.point(q64, 0xFFFFFFFF, cores)
+                ts.append(time.perf_counter() - t0)
+            med = float(np.median(ts))
+            forms[f"cpu_pagesize{ps}_{cores}core"] = {"lookups_per_s": nq / med, "ms": med * 1e3, "cores": cores}
+            mism += int((res.astype(np.uint32) != gpu).sum())
+        pt.close()
+    forms["gpu_plain"] = {"lookups_per_s": nq / (float(np.median(ev)) * 1e-3), "kernel_ms": float(np.median(ev))}
+    cpu_model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            cpu_model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
+    head = forms[f"cpu_pagesize32_{threads}core"]
+    line = common_line("CPU point lookups/s @1M uint32 keys, order-32 page tree (reference search semantics)",
+                       head["lookups_per_s"], "lookups/s", args, head["ms"], "u32",
+                       {"workload": "plain CPU point search: 1M unique uint32 keys uniform in [0,2^32), value = "
+                                    "rank, B+tree order 32; 1M queries, half from the key set, half uniform uint32",
+                        "keys": n, "queries": nq},
+                       {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
+                        "traffic": None, "note": "CPU configuration; the GPU plain search of the same inputs is "
+                                                 "in forms.gpu_plain"},
+                       {"forms": forms, "mismatches_cpu_vs_gpu": mism, "hit_fraction": float((gpu != 0xFFFFFFFF).mean()),
+                        "host": {"cores_used": threads, "nproc": os.cpu_count(), "cpu_model": cpu_model},
+                        "cpu_baseline": {"value": head["lookups_per_s"], "unit": "lookups/s", "cores": threads,
+                                         "kind": "port", "sample": f"all {nq} queries, median of 5"}})
+    dq.free()
+    dout.free()
+    tree.close()
+    print(json.dumps(line), flush=True)
+
+
 def bench_c4(args):
     """4096 independent uint64 trees (sizes log-uniform in [1K, 1M]), 4x
     size point queries per tree (80% hits) + size/16 ranges per tree, one
@@ -418,9 +478,9 @@ def main():
                     help="with --config c4: skip the autotuner and use its usual choice (for profiling)")
     ap.add_argument("--learned-only", action="store_true",
                     help="with --config c5: only the plain and learned lookups (for profiling)")
-    ap.add_argument("--config", default="c2", choices=["c2", "c3", "c4", "c5"],
-                    help="c2 (default): 64M-key point search; c3: 2-D rectangles; c4: 4096-tree batch; "
-                         "c5: learned index + serving")
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"],
+                    help="c2 (default): 64M-key point search; c1: 1M-key CPU page tree (the reference's CPU "
+                         "case); c3: 2-D rectangles; c4: 4096-tree batch; c5: learned index + serving")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is "
@@ -428,6 +488,8 @@ def main():
         sys.exit(2)
     if args.config == "c3":
         return bench_c3(args)
+    if args.config == "c1":
+        return bench_c1(args)
     if args.config == "c4":
         return bench_c4(args)
     if args.config == "c5":
