@@ -60,7 +60,7 @@ def time_form(tree, dq, dout, mode, steps, warmup):
     return wall, ev
 
 
-def cpu_baseline(keys, vals, q, gpu_out, target_s=10.0):
+def cpu_baseline(keys, vals, q, gpu_out, target_s=10.0, zipf=None):
     """Oracle page tree (reference semantics, PAGESIZE 5000) on host cores."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle  # the checker; only this leg of the benchmark uses it
@@ -86,6 +86,14 @@ def cpu_baseline(keys, vals, q, gpu_out, target_s=10.0):
     res1 = pt.point(one, 0xFFFFFFFF, 1)
     dt1 = time.perf_counter() - t0
     mism += int((res1.astype(np.uint32) != gpu_out[: len(one)]).sum())
+    zipf_rate = None
+    if zipf is not None:  # the second query set: a sample of the Zipf queries and the GPU's results
+        zq, zgpu = zipf
+        t0 = time.perf_counter()
+        resz = pt.point(zq.astype(np.uint64), 0xFFFFFFFF, threads)
+        dtz = time.perf_counter() - t0
+        mism += int((resz.astype(np.uint32) != zgpu).sum())
+        zipf_rate = {"value": len(zq) / dtz, "sample": f"first {len(zq)} Zipf(0.99) queries, {dtz:.1f} s"}
     pt.close()
     return {
         "value": s / dt, "unit": "lookups/s", "cores": threads, "kind": "port",
@@ -93,6 +101,7 @@ def cpu_baseline(keys, vals, q, gpu_out, target_s=10.0):
                   f"tree (PAGESIZE 5000, reference default), OpenMP static schedule, "
                   f"{dt:.1f} s; page-tree build {build_s:.1f} s excluded",
         "one_core": {"value": len(one) / dt1, "sample": f"first {len(one)} queries, {dt1:.1f} s"},
+        "zipf": zipf_rate,
         "mismatches_vs_gpu": mism,
     }
 
@@ -559,6 +568,7 @@ def main():
     gpu_out = dout.to_host()
 
     forms = {"plain_uniform": {"lookups_per_s": value, "kernel_ms": kernel_ms}}
+    zipf_sample = None
     if not args.no_forms:
         # end to end from host memory: queries H2D, plain search, results D2H
         # (pageable numpy buffers, synchronous copies), as a host caller sees it
@@ -582,15 +592,16 @@ def main():
                                     "mismatches_vs_plain": int((dout.to_host() != gpu_out).sum())}
         qz = workloads.zipf_queries(keys, nq)
         dqz = bptree.DeviceArray.from_host(qz)
-        del qz
         wall_pz, _ = time_form(tree, dqz, dout, bptree.PLAIN, max(3, args.steps // 2), 1)
         plain_z = dout.to_host()
+        zipf_sample = (qz[: 1 << 24].copy(), plain_z[: 1 << 24].copy())
         wall_gz, _ = time_form(tree, dqz, dout, bptree.GROUPED, max(3, args.steps // 2), 1)
         forms["plain_zipf"] = {"lookups_per_s": nq / wall_pz, "ms": wall_pz * 1e3}
         forms["grouped_zipf"] = {"lookups_per_s": nq / wall_gz, "ms": wall_gz * 1e3,
                                  "speedup_vs_plain": wall_pz / wall_gz,
                                  "mismatches_vs_plain": int((dout.to_host() != plain_z).sum())}
         dqz.free()
+        del qz
         # 1-D ranges over the same tree: 2^22 [lo, hi] with widths log-uniform in
         # [1, 2^10] ranks; one step = counts, their scan, and the ids, in each order
         rng_r = np.random.default_rng(11)
@@ -700,7 +711,7 @@ def main():
         "forms": forms,
     }
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(keys, vals, q, gpu_out)
+        line["cpu_baseline"] = cpu_baseline(keys, vals, q, gpu_out, zipf=zipf_sample)
     tree.close()
     print(json.dumps(line), flush=True)
 
