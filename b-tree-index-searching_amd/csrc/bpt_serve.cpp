@@ -3,14 +3,19 @@
 // synchronous batch per process).
 //
 //  * `slots` HIP streams (default priority), each driven by its own host
-//    thread; each in-flight request owns one slot. The threads take requests
-//    from one FIFO queue, so the host work of different requests (staging
-//    copies, HIP calls, waiting) runs in parallel.
+//    thread. The threads take work from one FIFO queue, so the host work of
+//    different launches (staging copies, HIP calls, waiting) runs in
+//    parallel.
+//  * A thread that takes a request at the head of the queue also takes the
+//    requests of the same kind right behind it (up to kFuseQueries queries and
+//    kFuseRequests requests) and answers them with one staged copy-in, one
+//    search and one copy-out; small requests then share the per-launch cost.
 //  * Device memory for requests comes from one pool of `mem_budget` bytes
 //    (first-fit, 256-byte granules). A request is admitted when a slot is
-//    free AND its bytes fit in the pool; otherwise it waits in FIFO order, so
-//    a small budget makes requests queue, never fail (a request larger than
-//    the whole budget is rejected at submit).
+//    free AND its bytes fit in the pool (a fused group that does not fit
+//    falls back to the head request alone); otherwise it waits in FIFO order,
+//    so a small budget makes requests queue, never fail (a request larger
+//    than the whole budget is rejected at submit).
 //  * Admitted work overlaps on the device: copy-in, search kernels and
 //    copy-out of different requests run on different streams.
 //  * Pinned staging buffers (hipHostMalloc) per slot carry the host copies.
@@ -36,10 +41,13 @@ namespace {
 
 using clk = std::chrono::steady_clock;
 
+constexpr uint64_t kFuseQueries = 1 << 20;  // queries of one fused point launch
+constexpr size_t kFuseRequests = 512;        // requests of one fused point launch
+
 struct Request {
     uint64_t ticket = 0;
     bpt_request req{};
-    uint64_t bytes = 0, dev_off = 0;
+    uint64_t bytes = 0;  // device bytes of the request alone
     clk::time_point t_submit, t_done;
     bool done = false;
     bpt_status status = BPT_OK;
@@ -49,7 +57,8 @@ struct Slot {
     hipStream_t stream = nullptr;
     uint8_t *pin = nullptr;  // pinned staging
     uint64_t pin_bytes = 0;
-    Request *cur = nullptr;
+    std::vector<Request *> group;  // the requests of the launch in flight
+    uint64_t off = 0, bytes = 0;   // their pool allocation
     std::thread worker;
 };
 
@@ -130,11 +139,26 @@ uint64_t staging_bytes(const bpt_tree *t, const bpt_request &r) {
     return r.kind == BPT_REQ_POINT ? r.n * (kb + vb) : r.n * (2 * kb + 16);
 }
 
-bpt_status launch(bpt_server *S, Slot &sl, Request *q) {
+// Queries of the slot's group (all of one kind).
+uint64_t group_queries(const Slot &sl) {
+    uint64_t n = 0;
+    for (const Request *q : sl.group) n += q->req.n;
+    return n;
+}
+
+// Stages, launches and queues the copy-out of the slot's group, in group
+// order. Points: staging = [queries][results], device = [queries][results]
+// [search workspace]. Ranges: staging = [lo][hi][first][count], device the
+// same.
+bpt_status launch(bpt_server *S, Slot &sl) {
     const bpt_tree *t = S->tree;
-    const bpt_request &r = q->req;
     const uint64_t kb = t->key_bytes, vb = t->val_bytes;
-    const uint64_t need = staging_bytes(t, r);
+    const bpt_request &r0 = sl.group[0]->req;
+    const bool point = r0.kind == BPT_REQ_POINT;
+    const uint64_t n = group_queries(sl);
+    bpt_request whole = r0;
+    whole.n = n;
+    const uint64_t need = staging_bytes(t, whole);
     if (sl.pin_bytes < need) {
         if (sl.pin) (void)hipHostFree(sl.pin);
         sl.pin = nullptr;
@@ -142,87 +166,130 @@ bpt_status launch(bpt_server *S, Slot &sl, Request *q) {
         BPT_HIP(hipHostMalloc(&sl.pin, std::max<uint64_t>(need, 1 << 20), hipHostMallocDefault));
         sl.pin_bytes = std::max<uint64_t>(need, 1 << 20);
     }
-    uint8_t *dev = S->d_pool + q->dev_off;
+    uint8_t *dev = S->d_pool + sl.off;
     auto al = [](uint64_t b) { return (b + 255) / 256 * 256; };
     hipStream_t s = sl.stream;
-    if (r.kind == BPT_REQ_POINT) {
-        memcpy(sl.pin, r.q, r.n * kb);
-        uint8_t *dq = dev, *dout = dev + al(r.n * kb), *ws = dout + al(r.n * vb);
-        BPT_HIP(hipMemcpyAsync(dq, sl.pin, r.n * kb, hipMemcpyHostToDevice, s));
-        const uint64_t wsb = bpt_point_search_ws_size(t, r.n, S->mode);
-        BPT_TRY(bpt_point_search_ws(t, dq, r.n, dout, S->mode, wsb ? ws : nullptr, wsb, s));
-        BPT_HIP(hipMemcpyAsync(sl.pin + r.n * kb, dout, r.n * vb, hipMemcpyDeviceToHost, s));
+    if (point) {
+        uint64_t at = 0;
+        for (const Request *q : sl.group) {
+            memcpy(sl.pin + at * kb, q->req.q, q->req.n * kb);
+            at += q->req.n;
+        }
+        uint8_t *dq = dev, *dout = dev + al(n * kb), *ws = dout + al(n * vb);
+        BPT_HIP(hipMemcpyAsync(dq, sl.pin, n * kb, hipMemcpyHostToDevice, s));
+        const uint64_t wsb = bpt_point_search_ws_size(t, n, S->mode);
+        BPT_TRY(bpt_point_search_ws(t, dq, n, dout, S->mode, wsb ? ws : nullptr, wsb, s));
+        BPT_HIP(hipMemcpyAsync(sl.pin + n * kb, dout, n * vb, hipMemcpyDeviceToHost, s));
     } else {
-        memcpy(sl.pin, r.lo, r.n * kb);
-        memcpy(sl.pin + r.n * kb, r.hi, r.n * kb);
-        uint8_t *dlo = dev, *dhi = dev + al(r.n * kb), *df = dhi + al(r.n * kb), *dc = df + al(r.n * 8);
-        BPT_HIP(hipMemcpyAsync(dlo, sl.pin, r.n * kb, hipMemcpyHostToDevice, s));
-        BPT_HIP(hipMemcpyAsync(dhi, sl.pin + r.n * kb, r.n * kb, hipMemcpyHostToDevice, s));
-        BPT_TRY(bpt_range_count(t, dlo, dhi, r.n, reinterpret_cast<uint64_t *>(df), reinterpret_cast<uint64_t *>(dc),
+        uint64_t at = 0;
+        for (const Request *q : sl.group) {
+            memcpy(sl.pin + at * kb, q->req.lo, q->req.n * kb);
+            memcpy(sl.pin + (n + at) * kb, q->req.hi, q->req.n * kb);
+            at += q->req.n;
+        }
+        uint8_t *dlo = dev, *dhi = dev + al(n * kb), *df = dhi + al(n * kb), *dc = df + al(n * 8);
+        BPT_HIP(hipMemcpyAsync(dlo, sl.pin, n * kb, hipMemcpyHostToDevice, s));
+        BPT_HIP(hipMemcpyAsync(dhi, sl.pin + n * kb, n * kb, hipMemcpyHostToDevice, s));
+        BPT_TRY(bpt_range_count(t, dlo, dhi, n, reinterpret_cast<uint64_t *>(df), reinterpret_cast<uint64_t *>(dc),
                                 reinterpret_cast<bpt_stream>(s)));
-        BPT_HIP(hipMemcpyAsync(sl.pin + 2 * r.n * kb, df, r.n * 8, hipMemcpyDeviceToHost, s));
-        BPT_HIP(hipMemcpyAsync(sl.pin + 2 * r.n * kb + r.n * 8, dc, r.n * 8, hipMemcpyDeviceToHost, s));
+        BPT_HIP(hipMemcpyAsync(sl.pin + 2 * n * kb, df, n * 8, hipMemcpyDeviceToHost, s));
+        BPT_HIP(hipMemcpyAsync(sl.pin + 2 * n * kb + n * 8, dc, n * 8, hipMemcpyDeviceToHost, s));
     }
     return BPT_OK;
 }
 
-// Copies a finished request's results to the caller's buffers (no lock held).
-void copy_out(const bpt_server *S, const Slot &sl, const Request *q) {
-    const bpt_request &r = q->req;
+// Copies a finished group's results to the callers' buffers (no lock held).
+void copy_out(const bpt_server *S, const Slot &sl, bpt_status st) {
+    if (st != BPT_OK) return;
     const uint64_t kb = S->tree->key_bytes, vb = S->tree->val_bytes;
-    if (q->status != BPT_OK) return;
-    if (r.kind == BPT_REQ_POINT) {
-        memcpy(r.out, sl.pin + r.n * kb, r.n * vb);
-    } else {
-        if (r.first) memcpy(r.first, sl.pin + 2 * r.n * kb, r.n * 8);
-        memcpy(r.count, sl.pin + 2 * r.n * kb + r.n * 8, r.n * 8);
+    const bool point = sl.group[0]->req.kind == BPT_REQ_POINT;
+    const uint64_t n = group_queries(sl);
+    uint64_t at = 0;
+    for (const Request *q : sl.group) {
+        const bpt_request &r = q->req;
+        if (point) {
+            memcpy(r.out, sl.pin + n * kb + at * vb, r.n * vb);
+        } else {
+            if (r.first) memcpy(r.first, sl.pin + 2 * n * kb + at * 8, r.n * 8);
+            memcpy(r.count, sl.pin + 2 * n * kb + (n + at) * 8, r.n * 8);
+        }
+        at += r.n;
     }
 }
 
-// Bookkeeping of a finished request (S->mu held).
-void finish(bpt_server *S, Slot &sl) {
-    Request *q = sl.cur;
-    q->t_done = clk::now();
-    q->done = true;
-    S->pool.release(q->dev_off, q->bytes);
-    S->used -= (q->bytes + 255) / 256 * 256;
-    S->inflight--;
-    S->lat_ms.push_back(std::chrono::duration<double, std::milli>(q->t_done - q->t_submit).count());
-    S->lookups += q->req.n;
-    S->completed++;
-    if (!S->any || q->t_done > S->t_last) S->t_last = q->t_done;
-    sl.cur = nullptr;
+// Takes the next launch off the queue (S->mu held): the head request and
+// the requests of its kind that follow it, as far as the fusion limits
+// allow; the group's device bytes must fit the pool, else the head alone is
+// tried. Returns false when nothing can be admitted now.
+bool take_work(bpt_server *S, Slot &sl) {
+    if (S->queue.empty()) return false;
+    const Request *h = S->queue.front();
+    size_t k = 1;
+    uint64_t n = h->req.n;
+    while (k < S->queue.size() && k < kFuseRequests) {
+        const Request *r = S->queue[k];
+        if (r->req.kind != h->req.kind || n + r->req.n > kFuseQueries) break;
+        n += r->req.n;
+        ++k;
+    }
+    uint64_t off = 0, bytes = 0;
+    if (k > 1) {
+        bpt_request g = h->req;
+        g.n = n;
+        bytes = request_bytes(S->tree, S->mode, g);
+        if (!S->pool.alloc(bytes, &off)) k = 1;
+    }
+    if (k == 1) {
+        bytes = h->bytes;
+        if (!S->pool.alloc(bytes, &off)) return false;
+    }
+    sl.group.assign(S->queue.begin(), S->queue.begin() + k);
+    S->queue.erase(S->queue.begin(), S->queue.begin() + k);
+    sl.off = off;
+    sl.bytes = bytes;
+    return true;
+}
+
+// Bookkeeping of a finished group (S->mu held).
+void finish(bpt_server *S, Slot &sl, bpt_status st) {
+    const clk::time_point now = clk::now();
+    S->pool.release(sl.off, sl.bytes);
+    S->used -= (sl.bytes + 255) / 256 * 256;
+    for (Request *q : sl.group) {
+        q->t_done = now;
+        q->status = st;
+        q->done = true;
+        S->inflight--;
+        S->lat_ms.push_back(std::chrono::duration<double, std::milli>(q->t_done - q->t_submit).count());
+        S->lookups += q->req.n;
+        S->completed++;
+    }
+    if (!S->any || now > S->t_last) S->t_last = now;
+    sl.group.clear();
 }
 
 // One thread per slot: admission is FIFO -- a thread takes the head of the
-// queue only when its bytes fit the pool, otherwise every thread waits for
-// memory to be released -- then stages, launches and waits on its own stream.
+// queue (and the point requests fused with it) only when the bytes fit the
+// pool, otherwise every thread waits for memory to be released -- then
+// stages, launches and waits on its own stream.
 void slot_worker(bpt_server *S, Slot *sl) {
     (void)hipSetDevice(S->tree->device);
     std::unique_lock<std::mutex> lk(S->mu);
     for (;;) {
-        uint64_t off = 0;
-        S->cv_work.wait(lk, [&] {
-            return (S->stop && S->queue.empty()) ||
-                   (!S->queue.empty() && S->pool.alloc(S->queue.front()->bytes, &off));
-        });
-        if (S->queue.empty()) break;  // stopping
-        Request *q = S->queue.front();
-        S->queue.pop_front();
-        q->dev_off = off;
-        S->used += (q->bytes + 255) / 256 * 256;
+        S->cv_work.wait(lk, [&] { return (S->stop && S->queue.empty()) || take_work(S, *sl); });
+        if (sl->group.empty()) break;  // stopping
+        S->used += (sl->bytes + 255) / 256 * 256;
         S->peak = std::max(S->peak, S->used);
-        sl->cur = q;
-        S->inflight++;
+        S->inflight += (uint32_t)sl->group.size();
         S->max_inflight = std::max(S->max_inflight, S->inflight);
         if (!S->queue.empty()) S->cv_work.notify_one();  // the next head may fit too
         lk.unlock();
-        q->status = launch(S, *sl, q);
+        bpt_status st = launch(S, *sl);
         const hipError_t e = hipStreamSynchronize(sl->stream);
-        if (q->status == BPT_OK && e != hipSuccess) q->status = bpt::hip_fail(e, "request", __FILE__, __LINE__);
-        copy_out(S, *sl, q);
+        if (st == BPT_OK && e != hipSuccess) st = bpt::hip_fail(e, "request", __FILE__, __LINE__);
+        copy_out(S, *sl, st);
         lk.lock();
-        finish(S, *sl);
+        finish(S, *sl, st);
         S->cv_done.notify_all();
         S->cv_work.notify_all();  // memory was released
     }

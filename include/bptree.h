@@ -308,7 +308,9 @@ bpt_status bpt_mlp_loss_grad(const float *h_params, uint32_t hidden, const float
  * admitted FIFO when a slot is free and their device bytes fit in a pool of
  * `mem_budget` bytes (they wait otherwise; one larger than the whole budget
  * is rejected with BPT_ERR_CAPACITY). Host buffers stay the caller's until
- * the request completes. Point requests use `mode` (bpt_point_mode). */
+ * the request completes. Point requests use `mode` (bpt_point_mode). A run
+ * of queued requests of one kind (up to 2^20 queries, 512 requests) is
+ * answered in one launch; each caller gets its own results. */
 typedef struct bpt_server bpt_server;
 enum { BPT_REQ_POINT = 0, BPT_REQ_RANGE = 1 };
 typedef struct bpt_request {

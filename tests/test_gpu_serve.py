@@ -111,3 +111,34 @@ def test_requests_overlap_on_the_device(bpt, served_tree):
     srv.close()
     assert st["max_inflight"] > 1
     assert (lat > 0).all()
+
+
+@pytest.mark.parametrize("mode", [0, 2])
+def test_small_point_requests_are_fused(bpt, oracle, served_tree, mode):
+    """Hundreds of small requests arriving together are answered in fused
+    launches of one kind (more requests in flight than slots): runs of point
+    requests, and of range requests, between each other; every answer equals
+    the oracle."""
+    t, keys, vals = served_tree
+    rng = np.random.default_rng(4 + mode)
+    reqs = []
+    for i in range(400):
+        if i % 37 >= 33:  # runs of four range requests between the point requests
+            s = rng.integers(0, len(keys), int(rng.integers(1, 60)))
+            reqs.append(("range", keys[s].astype(np.uint64),
+                         keys[np.minimum(s + 100, len(keys) - 1)].astype(np.uint64)))
+        else:
+            n = int(rng.integers(1, 200))
+            reqs.append(("point", np.where(rng.random(n) < 0.8, keys[rng.integers(0, len(keys), n)],
+                                           rng.integers(0, 1 << 50, n, dtype=np.uint64)).astype(np.uint64)))
+    srv = bpt.Server(t, slots=2, mem_budget=1 << 26, mode=mode)
+    outs = _run(bpt, srv, reqs, np.zeros(len(reqs)))
+    st = srv.stats()
+    srv.close()
+    assert st["completed"] == len(reqs) and st["max_inflight"] > 2
+    for r, m in zip(reqs, outs):
+        if r[0] == "point":
+            assert np.array_equal(m, oracle.point_closed(keys, vals, r[1], SENT))
+        else:
+            f, c = oracle.range_closed(keys, r[1], r[2])
+            assert np.array_equal(m[1], c) and np.array_equal(m[0], f)
