@@ -218,3 +218,33 @@ def test_workload_c3_full_size(bpt, oracle, wl):
     end = int(o0[sample - 1] + c0[sample - 1])
     assert np.array_equal(i0[:end], want_ids)
     t.close()
+
+
+def test_workload_ranges_full_size(bpt, oracle, wl):
+    """The 1-D range workload of the benchmark: 64M uint32 keys (value = rank)
+    and 4M inclusive ranges of 1..1024 ranks. Counts and lower bounds equal
+    the oracle for every range in every order; the ids, consecutive ranks
+    here, are checked exactly in 16 windows of 2^20 ids."""
+    n, nr = 1 << 26, 1 << 22
+    keys, vals = wl.point_keys(n, workload=2)
+    rng = np.random.default_rng(11)
+    st = rng.integers(0, n, nr)
+    w = np.floor(np.exp2(rng.uniform(0, 10, nr))).astype(np.int64)
+    lo, hi = keys[st], keys[np.minimum(st + w, n - 1)]
+    print("ranges generated", flush=True)
+    t = bpt.Tree(keys, vals)
+    want_first, want_cnt = oracle.range_closed(keys, lo, hi)
+    for mode in MODES:
+        first, cnt, offs, ids = t.range_search(lo, hi, group_mode=mode, with_ids=mode == 0)
+        assert np.array_equal(cnt, want_cnt), mode
+        assert np.array_equal(first, want_first), mode
+        if mode == 0:
+            total = int(cnt.sum())
+            assert len(ids) == total
+            for s in np.linspace(0, total - (1 << 20), 16).astype(np.int64):
+                pos = np.arange(s, s + (1 << 20), dtype=np.int64)
+                i = np.searchsorted(offs.astype(np.int64), pos, side="right") - 1
+                want = (first.astype(np.int64)[i] + (pos - offs.astype(np.int64)[i])).astype(np.uint32)
+                assert np.array_equal(ids[pos], want), int(s)
+            del ids
+    t.close()
