@@ -79,6 +79,22 @@ __device__ __forceinline__ uint64_t lb_u64(const TreeView &t, const uint8_t *lds
     return leaf_lower_bound<uint64_t, uint32_t, LF>(t, descend<uint64_t, F>(t, lds, z), z);
 }
 
+// lower_bound(z) when every key of `leaf` is below z: the leaf's level-1
+// node first (its children's maxima; padding entries are the largest key),
+// which holds the answer unless all its children end below z, then a
+// descent from the top.
+template <int F, int LF>
+__device__ __forceinline__ uint64_t lb_after_leaf(const TreeView &t, const uint8_t *lds, uint64_t leaf, uint64_t z) {
+    if (t.height >= 1) {
+        const uint64_t node = leaf / F;
+        const uint64_t *sep = reinterpret_cast<const uint64_t *>(t.inner + t.off[1]) + node * F;
+        const uint64_t child = node * F + node_rank<uint64_t, F>(sep, z);
+        if (child < (node + 1) * F && child < t.count[0])
+            return leaf_lower_bound<uint64_t, uint32_t, LF>(t, child, z);
+    }
+    return lb_u64<F, LF>(t, lds, z);
+}
+
 // Walks rectangle r from p = lower_bound(zmin); calls emit(slot) for every
 // point inside, in code order.
 template <int F, int LF, class Emit>
@@ -106,7 +122,7 @@ __device__ __forceinline__ void rect_walk_from(const TreeView &t, const uint8_t 
         if (q < leaf_end || q >= t.n) {
             p = q;
         } else {
-            p = lb_u64<F, LF>(t, lds, nz);
+            p = lb_after_leaf<F, LF>(t, lds, p / LF, nz);
         }
     }
 }
