@@ -66,6 +66,25 @@ __global__ __launch_bounds__(256) void k_lower_bound(TreeView t, const K *__rest
     }
 }
 
+// Lower bounds through the cooperative descent (equal key and value widths):
+// 64 queries per wave at a time, G lanes per node.
+template <class K, class V, int F, int LF, int G, bool OFF32>
+__global__ __launch_bounds__(256) void k_lower_bound_coop(TreeView t, const K *__restrict__ q,
+                                                          uint64_t *__restrict__ pos, uint64_t nq) {
+    extern __shared__ uint4 lds_raw[];
+    uint8_t *lds = reinterpret_cast<uint8_t *>(lds_raw);
+    stage_top_levels(t, lds);
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t stride = ((uint64_t)gridDim.x * blockDim.x) & ~63ull;
+    for (uint64_t base = wave * 64; base < nq; base += stride) {
+        const uint64_t j = base + lane;
+        const K key = j < nq ? q[j] : K(0);
+        const uint64_t p = coop_lookup64<K, V, F, LF, false, kOpLowerBound, G, OFF32>(t, lds, key, V(0));
+        if (j < nq) pos[j] = p;
+    }
+}
+
 // count = upper_bound(hi) - lower_bound(lo); upper_bound(hi) is
 // lower_bound(hi + 1), or n when hi is the largest key value.
 template <class K, class V, int F, int LF>
@@ -432,8 +451,28 @@ bpt_status bpt_lower_bound(const bpt_tree *t, const void *d_q, uint64_t nq, uint
     return dispatch_tree(t, [&](auto kt, auto vt, auto fc, auto lc) -> bpt_status {
         using K = typename decltype(kt)::type;
         using V = typename decltype(vt)::type;
-        k_lower_bound<K, V, decltype(fc)::value, decltype(lc)::value>
-            <<<search_grid(t, nq), t->block, lds_of(t), s>>>(t->view, (const K *)d_q, d_pos, nq);
+        constexpr int F = decltype(fc)::value, LF = decltype(lc)::value;
+        const uint32_t g = search_grid(t, nq);
+        const size_t l = lds_of(t);
+        if constexpr (sizeof(K) == sizeof(V)) {
+            if (t->coop && t->view.count[0] < (1ull << 32)) {
+                const bool off32 = t->addr32 && t->leaf_bytes <= (1ull << 32) && t->inner_bytes <= (1ull << 32);
+                auto go = [&](auto lanes, auto o32) {
+                    k_lower_bound_coop<K, V, F, LF, decltype(lanes)::value, decltype(o32)::value>
+                        <<<g, t->block, l, s>>>(t->view, (const K *)d_q, d_pos, nq);
+                };
+                if (t->coop_lanes == 4) {
+                    if (off32) go(IC<4>{}, std::true_type{});
+                    else go(IC<4>{}, std::false_type{});
+                } else {
+                    if (off32) go(IC<8>{}, std::true_type{});
+                    else go(IC<8>{}, std::false_type{});
+                }
+                BPT_HIP(hipGetLastError());
+                return BPT_OK;
+            }
+        }
+        k_lower_bound<K, V, F, LF><<<g, t->block, l, s>>>(t->view, (const K *)d_q, d_pos, nq);
         BPT_HIP(hipGetLastError());
         return BPT_OK;
     });

@@ -62,6 +62,9 @@ def test_point_plain_and_grouped(bpt, oracle, cfg, n):
         t.set_option("group_lanes", 4)
     pos = t.lower_bound(q)
     assert np.array_equal(pos, oracle.lower_bound(keys, q))
+    for opt, val in (("lanes", 4), ("coop", 0)):  # the cooperative lower bound with 4 lanes; one lane per query
+        t.set_option(opt, val)
+        assert np.array_equal(t.lower_bound(q), oracle.lower_bound(keys, q)), opt
     t.close()
 
 
