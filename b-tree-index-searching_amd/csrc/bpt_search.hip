@@ -52,6 +52,68 @@ __global__ __launch_bounds__(256) void k_point_coop(TreeView t, const K *__restr
     }
 }
 
+// Trees whose key and value widths differ (a leaf holds its keys, then its
+// values): the inner levels cooperatively, G lanes per node, then the leaf
+// by the query's own lane.
+template <class K, class V, int F, int LF, int G, bool OFF32>
+__global__ __launch_bounds__(256) void k_point_coop_inner(TreeView t, const K *__restrict__ q,
+                                                          V *__restrict__ out, uint64_t nq, V sentinel) {
+    extern __shared__ uint4 lds_raw[];
+    uint8_t *lds = reinterpret_cast<uint8_t *>(lds_raw);
+    stage_top_levels(t, lds);
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t stride = ((uint64_t)gridDim.x * blockDim.x) & ~63ull;
+    for (uint64_t base = wave * 64; base < nq; base += stride) {
+        const uint64_t j = base + lane;
+        const K key = j < nq ? __builtin_nontemporal_load(&q[j]) : K(0);
+        const uint32_t leaf = coop_leaf_of<K, F, G, OFF32>(t, lds, key);
+        const V r = leaf_lookup<K, V, LF>(t, leaf, key, sentinel);
+        if (j < nq) __builtin_nontemporal_store(r, &out[j]);
+    }
+}
+
+template <class K, class V, int F, int LF, int G, bool OFF32>
+__global__ __launch_bounds__(256) void k_range_count_coop_inner(TreeView t, const K *__restrict__ lo,
+                                                                const K *__restrict__ hi,
+                                                                const uint32_t *__restrict__ order,
+                                                                uint64_t *__restrict__ first,
+                                                                uint64_t *__restrict__ count, uint64_t nq) {
+    extern __shared__ uint4 lds_raw[];
+    uint8_t *lds = reinterpret_cast<uint8_t *>(lds_raw);
+    stage_top_levels(t, lds);
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t stride = ((uint64_t)gridDim.x * blockDim.x) & ~63ull;
+    for (uint64_t base = wave * 64; base < nq; base += stride) {
+        const uint64_t j = base + lane;
+        const bool valid = j < nq;
+        const uint64_t i = valid ? (order ? (uint64_t)order[j] : j) : 0;
+        const K a = valid ? lo[i] : K(0), b = valid ? hi[i] : K(0);
+        const bool top = b == key_max<K>();
+        const K b1 = top ? b : K(b + 1);
+        const uint64_t pa = leaf_lower_bound<K, V, LF>(t, coop_leaf_of<K, F, G, OFF32>(t, lds, a), a);
+        const uint64_t pb1 = leaf_lower_bound<K, V, LF>(t, coop_leaf_of<K, F, G, OFF32>(t, lds, b1), b1);
+        const uint64_t pb = top ? t.n : pb1;
+        if (valid) {
+            if (first) first[i] = pa;
+            count[i] = (a > b || pb < pa) ? 0 : pb - pa;
+        }
+    }
+}
+
+// Calls fn(IC<G>, bool_constant<OFF32>) for the tree's cooperative shape.
+template <class Fn>
+static void with_coop_shape(const bpt_tree *t, bool off32, Fn &&fn) {
+    if (t->coop_lanes == 4) {
+        if (off32) fn(IC<4>{}, std::true_type{});
+        else fn(IC<4>{}, std::false_type{});
+    } else {
+        if (off32) fn(IC<8>{}, std::true_type{});
+        else fn(IC<8>{}, std::false_type{});
+    }
+}
+
 template <class K, class V, int F, int LF>
 __global__ __launch_bounds__(256) void k_lower_bound(TreeView t, const K *__restrict__ q,
                                                      uint64_t *__restrict__ pos, uint64_t nq) {
@@ -394,6 +456,15 @@ bpt_status launch_plain(const bpt_tree *t, const void *d_q, uint64_t nq, void *d
                 return BPT_OK;
             }
         }
+        if (t->coop && t->view.count[0] < (1ull << 32)) {  // key and value widths differ
+            with_coop_shape(t, t->addr32 && t->inner_bytes <= (1ull << 32), [&](auto lanes, auto o32) {
+                k_point_coop_inner<K, V, F, LF, decltype(lanes)::value, decltype(o32)::value>
+                    <<<search_grid(t, nq), t->block, lds_of(t), s>>>(t->view, (const K *)d_q, (V *)d_out, nq,
+                                                                     sentinel);
+            });
+            BPT_HIP(hipGetLastError());
+            return BPT_OK;
+        }
         k_point_plain<K, V, F, LF><<<search_grid(t, nq), t->block, lds_of(t), s>>>(
             t->view, (const K *)d_q, (V *)d_out, nq, sentinel);
         BPT_HIP(hipGetLastError());
@@ -515,6 +586,15 @@ bpt_status range_count_ordered(const bpt_tree *t, const void *d_lo, const void *
                 BPT_HIP(hipGetLastError());
                 return BPT_OK;
             }
+        }
+        if (t->coop && t->view.count[0] < (1ull << 32)) {  // key and value widths differ
+            with_coop_shape(t, t->addr32 && t->inner_bytes <= (1ull << 32), [&](auto lanes, auto o32) {
+                k_range_count_coop_inner<K, V, F, LF, decltype(lanes)::value, decltype(o32)::value>
+                    <<<g, t->block, l, s>>>(t->view, (const K *)d_lo, (const K *)d_hi, d_order, d_first, d_count,
+                                            nq);
+            });
+            BPT_HIP(hipGetLastError());
+            return BPT_OK;
         }
         k_range_count<K, V, F, LF><<<g, t->block, l, s>>>(t->view, (const K *)d_lo, (const K *)d_hi, d_order,
                                                           d_first, d_count, nq);

@@ -128,8 +128,9 @@ bpt_status bpt_tree_upsert(bpt_tree *t, const void *d_keys, const void *d_vals, 
 bpt_status bpt_tree_erase(bpt_tree *t, const void *d_keys, uint64_t n, bpt_stream s);
 
 /* Search-kernel options: "coop" (1 = several lanes share each node, the
- * default: point and range searches use it when key and value widths
- * match, the rectangle count for the inner levels; 0 = one lane per query),
+ * default: the whole descent when key and value widths match, the inner
+ * levels otherwise, with the leaf read by the query's lane; 0 = one lane per
+ * query),
  * "lanes" (lanes per query of the cooperative descent: 8, the default, or 4),
  * "leaf_nt" (1 = non-temporal leaf loads, the default),
  * "addr32" (1 = 32-bit byte offsets into levels and leaves below 4 GiB, the
