@@ -102,6 +102,23 @@ __global__ __launch_bounds__(256) void k_range_count_coop_inner(TreeView t, cons
     }
 }
 
+template <class K, class V, int F, int LF, int G, bool OFF32>
+__global__ __launch_bounds__(256) void k_lower_bound_coop_inner(TreeView t, const K *__restrict__ q,
+                                                                uint64_t *__restrict__ pos, uint64_t nq) {
+    extern __shared__ uint4 lds_raw[];
+    uint8_t *lds = reinterpret_cast<uint8_t *>(lds_raw);
+    stage_top_levels(t, lds);
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t stride = ((uint64_t)gridDim.x * blockDim.x) & ~63ull;
+    for (uint64_t base = wave * 64; base < nq; base += stride) {
+        const uint64_t j = base + lane;
+        const K key = j < nq ? q[j] : K(0);
+        const uint64_t p = leaf_lower_bound<K, V, LF>(t, coop_leaf_of<K, F, G, OFF32>(t, lds, key), key);
+        if (j < nq) pos[j] = p;
+    }
+}
+
 // Calls fn(IC<G>, bool_constant<OFF32>) for the tree's cooperative shape.
 template <class Fn>
 static void with_coop_shape(const bpt_tree *t, bool off32, Fn &&fn) {
@@ -542,6 +559,14 @@ bpt_status bpt_lower_bound(const bpt_tree *t, const void *d_q, uint64_t nq, uint
                 BPT_HIP(hipGetLastError());
                 return BPT_OK;
             }
+        }
+        if (t->coop && t->view.count[0] < (1ull << 32)) {  // key and value widths differ
+            with_coop_shape(t, t->addr32 && t->inner_bytes <= (1ull << 32), [&](auto lanes, auto o32) {
+                k_lower_bound_coop_inner<K, V, F, LF, decltype(lanes)::value, decltype(o32)::value>
+                    <<<g, t->block, l, s>>>(t->view, (const K *)d_q, d_pos, nq);
+            });
+            BPT_HIP(hipGetLastError());
+            return BPT_OK;
         }
         k_lower_bound<K, V, F, LF><<<g, t->block, l, s>>>(t->view, (const K *)d_q, d_pos, nq);
         BPT_HIP(hipGetLastError());
