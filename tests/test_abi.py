@@ -79,3 +79,14 @@ def test_errors_without_device_are_reported_not_fatal():
     d = bptree.BuildDesc(4, 4, 48, 16, 0, 0)
     assert L.bpt_tree_build(ctypes.byref(d), None, None, 0, ctypes.byref(h), None) == 1
     assert L.bpt_tree_free(None) == 0
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """The benchmark measures one GPU; asking for more exits with a message
+    (and status 2) before touching a device."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stdout + r.stderr
