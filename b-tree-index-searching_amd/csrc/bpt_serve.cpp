@@ -295,6 +295,63 @@ void slot_worker(bpt_server *S, Slot *sl) {
     }
 }
 
+// Validates r and makes its Request (nullptr, with the error set, when r is
+// rejected).
+Request *make_request(bpt_server *S, const bpt_request *r, bpt_status *st) {
+    *st = BPT_ERR_INVALID;
+    if (!r) {
+        bpt::set_error("request is NULL");
+        return nullptr;
+    }
+    if (r->kind != BPT_REQ_POINT && r->kind != BPT_REQ_RANGE) {
+        bpt::set_error("request kind %d", r->kind);
+        return nullptr;
+    }
+    if (r->n < 1) {
+        bpt::set_error("empty request");
+        return nullptr;
+    }
+    if (r->kind == BPT_REQ_POINT ? !(r->q && r->out) : !(r->lo && r->hi && r->count)) {
+        bpt::set_error("request buffers are NULL");
+        return nullptr;
+    }
+    const uint64_t bytes = request_bytes(S->tree, S->mode, *r);
+    if (bytes > S->budget) {
+        bpt::set_error("request needs %llu bytes, more than the whole budget", (unsigned long long)bytes);
+        *st = BPT_ERR_CAPACITY;
+        return nullptr;
+    }
+    auto *q = new Request;
+    q->req = *r;
+    q->bytes = bytes;
+    *st = BPT_OK;
+    return q;
+}
+
+// Queues made requests under one lock, in order, and wakes the workers. A
+// request's submit time is now unless it already has one (a replayed
+// request's scheduled arrival).
+void enqueue(bpt_server *S, Request *const *qs, size_t n) {
+    if (n == 0) return;
+    const clk::time_point now = clk::now();
+    {
+        std::lock_guard<std::mutex> g(S->mu);
+        for (size_t i = 0; i < n; ++i) {
+            Request *q = qs[i];
+            if (q->t_submit == clk::time_point{}) q->t_submit = now;
+            q->ticket = S->next_ticket++;
+            if (!S->any || q->t_submit < S->t_first) {
+                S->t_first = q->t_submit;
+                S->any = true;
+            }
+            S->reqs[q->ticket] = q;
+            S->queue.push_back(q);
+        }
+    }
+    if (n == 1) S->cv_work.notify_one();
+    else S->cv_work.notify_all();
+}
+
 }  // namespace
 
 using namespace bpt;
@@ -333,30 +390,10 @@ bpt_status bpt_server_create(bpt_tree *t, uint32_t slots, uint64_t mem_budget, i
 
 bpt_status bpt_server_submit(bpt_server *S, const bpt_request *r, uint64_t *ticket) {
     BPT_CHECK(S && r && ticket, BPT_ERR_INVALID, "NULL argument");
-    BPT_CHECK(r->kind == BPT_REQ_POINT || r->kind == BPT_REQ_RANGE, BPT_ERR_INVALID, "request kind %d", r->kind);
-    BPT_CHECK(r->n >= 1, BPT_ERR_INVALID, "empty request");
-    BPT_CHECK(r->kind == BPT_REQ_POINT ? (r->q && r->out) : (r->lo && r->hi && r->count), BPT_ERR_INVALID,
-              "request buffers are NULL");
-    auto *q = new Request;
-    q->req = *r;
-    q->bytes = request_bytes(S->tree, S->mode, *r);
-    if (q->bytes > S->budget) {
-        delete q;
-        set_error("request needs %llu bytes, more than the whole budget", (unsigned long long)request_bytes(S->tree, S->mode, *r));
-        return BPT_ERR_CAPACITY;
-    }
-    q->t_submit = clk::now();
-    {
-        std::lock_guard<std::mutex> g(S->mu);
-        q->ticket = S->next_ticket++;
-        if (!S->any) {
-            S->t_first = q->t_submit;
-            S->any = true;
-        }
-        S->reqs[q->ticket] = q;
-        S->queue.push_back(q);
-    }
-    S->cv_work.notify_one();
+    bpt_status st;
+    Request *q = make_request(S, r, &st);
+    if (!q) return st;
+    enqueue(S, &q, 1);
     *ticket = q->ticket;
     return BPT_OK;
 }
@@ -431,19 +468,40 @@ bpt_status bpt_server_destroy(bpt_server *S) {
 }
 
 // Open-loop replay: request i is submitted at arrival_s[i] seconds after the
-// start (busy-waiting for its time, independent of completions), then every
-// request is waited for; latency_ms[i] = completion - submit.
+// start (busy-waiting for its time, independent of completions; the requests
+// already due are queued together), then every request is waited for;
+// latency_ms[i] = completion - submit.
 bpt_status bpt_server_replay(bpt_server *S, const bpt_request *reqs, uint32_t nreq, const double *arrival_s,
                              double *latency_ms) {
     BPT_CHECK(S && (nreq == 0 || (reqs && arrival_s)), BPT_ERR_INVALID, "NULL argument");
     std::vector<uint64_t> tickets(nreq, 0);
     const clk::time_point t0 = clk::now();
     bpt_status st = BPT_OK;
-    for (uint32_t i = 0; i < nreq; ++i) {
-        const auto due = t0 + std::chrono::duration_cast<clk::duration>(std::chrono::duration<double>(arrival_s[i]));
-        while (clk::now() < due) std::this_thread::yield();
-        bpt_status s = bpt_server_submit(S, &reqs[i], &tickets[i]);
-        if (s != BPT_OK && st == BPT_OK) st = s;
+    auto due = [&](uint32_t i) {
+        return t0 + std::chrono::duration_cast<clk::duration>(std::chrono::duration<double>(arrival_s[i]));
+    };
+    std::vector<Request *> batch;
+    std::vector<uint32_t> index;  // request number of each batch entry
+    for (uint32_t i = 0; i < nreq;) {
+        while (clk::now() < due(i)) std::this_thread::yield();
+        // every request already due is queued at once, under one lock
+        const clk::time_point now = clk::now();
+        const uint32_t first = i;
+        batch.clear();
+        index.clear();
+        for (; i < nreq && (i == first || due(i) <= now); ++i) {
+            bpt_status s;
+            Request *q = make_request(S, &reqs[i], &s);
+            if (q) {
+                q->t_submit = due(i);  // latency counts from the scheduled arrival
+                batch.push_back(q);
+                index.push_back(i);
+            } else if (st == BPT_OK) {
+                st = s;
+            }
+        }
+        enqueue(S, batch.data(), batch.size());
+        for (size_t b = 0; b < batch.size(); ++b) tickets[index[b]] = batch[b]->ticket;
     }
     for (uint32_t i = 0; i < nreq; ++i) {
         if (!tickets[i]) continue;
