@@ -38,6 +38,7 @@ constexpr size_t kMaxLds = 160 * 1024;     // LDS of a workgroup (gfx950)
 constexpr uint64_t kBucketChunk = 1ull << 28;  // queries per partition (u32 slots)
 
 constexpr uint32_t kColGroup = 128;          // tiles per column-sum group
+constexpr uint32_t kColStep = 8;             // column loads in flight per thread
 
 struct BucketDev {
     uint32_t nb;       // buckets
@@ -226,7 +227,13 @@ __global__ __launch_bounds__(kBBlock) void k_bkt_colsum(const uint32_t *__restri
     const uint32_t t0 = g * kColGroup, t1 = std::min(t0 + kColGroup, d.ntiles);
     for (uint32_t b = threadIdx.x; b < d.nb; b += blockDim.x) {
         uint32_t acc = 0;
-        for (uint32_t t = t0; t < t1; ++t) acc += hist[(uint64_t)t * d.nb + b];
+        for (uint32_t t = t0; t < t1; t += kColStep) {  // kColStep independent loads at a time
+            uint32_t v[kColStep];
+#pragma unroll
+            for (uint32_t k = 0; k < kColStep; ++k) v[k] = t + k < t1 ? hist[(uint64_t)(t + k) * d.nb + b] : 0u;
+#pragma unroll
+            for (uint32_t k = 0; k < kColStep; ++k) acc += v[k];
+        }
         S[(uint64_t)g * d.nb + b] = acc;
     }
 }
@@ -240,11 +247,15 @@ __global__ __launch_bounds__(kScanThreads) void k_bkt_colscan(uint32_t *__restri
     __shared__ uint32_t wsum[kScanThreads / 64];
     for (uint32_t b = threadIdx.x; b < d.nb; b += blockDim.x) {
         uint32_t run = 0;
-        for (uint32_t g = 0; g < d.ngroups; ++g) {
-            uint32_t *e = S + (uint64_t)g * d.nb + b;
-            const uint32_t v = *e;
-            *e = run;
-            run += v;
+        for (uint32_t g = 0; g < d.ngroups; g += kColStep) {  // loads first: kColStep in flight
+            uint32_t v[kColStep];
+#pragma unroll
+            for (uint32_t k = 0; k < kColStep; ++k) v[k] = g + k < d.ngroups ? S[(uint64_t)(g + k) * d.nb + b] : 0u;
+#pragma unroll
+            for (uint32_t k = 0; k < kColStep; ++k) {
+                if (g + k < d.ngroups) S[(uint64_t)(g + k) * d.nb + b] = run;
+                run += v[k];
+            }
         }
         tot[b] = run;
     }
@@ -263,10 +274,15 @@ __global__ __launch_bounds__(kBBlock) void k_bkt_colprefix(const uint32_t *__res
     const uint32_t t0 = g * kColGroup, t1 = std::min(t0 + kColGroup, d.ntiles);
     for (uint32_t b = threadIdx.x; b < d.nb; b += blockDim.x) {
         uint32_t run = bstart[b] + S[(uint64_t)g * d.nb + b];
-        for (uint32_t t = t0; t < t1; ++t) {
-            const uint64_t e = (uint64_t)t * d.nb + b;
-            off[e] = run;
-            run += hist[e];
+        for (uint32_t t = t0; t < t1; t += kColStep) {  // kColStep independent loads at a time
+            uint32_t v[kColStep];
+#pragma unroll
+            for (uint32_t k = 0; k < kColStep; ++k) v[k] = t + k < t1 ? hist[(uint64_t)(t + k) * d.nb + b] : 0u;
+#pragma unroll
+            for (uint32_t k = 0; k < kColStep; ++k) {
+                if (t + k < t1) off[(uint64_t)(t + k) * d.nb + b] = run;
+                run += v[k];
+            }
         }
     }
 }
