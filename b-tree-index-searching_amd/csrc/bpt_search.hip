@@ -358,15 +358,63 @@ __device__ __forceinline__ T block_exclusive_scan(T v, T *sh_warp, T *total) {
     return excl;
 }
 
+// A thread's kScanItems consecutive items: 16-byte loads when the array is
+// 16-byte aligned and all the items exist (the items of a thread start on a
+// 32-byte boundary), element loads otherwise.
+template <class T>
+__device__ __forceinline__ void scan_load(const T *__restrict__ in, uint64_t base, uint64_t n, T (&v)[kScanItems]) {
+    if ((reinterpret_cast<uintptr_t>(in) & 15) == 0 && base + kScanItems <= n) {
+        const uint4 *p = reinterpret_cast<const uint4 *>(in + base);
+        constexpr int NV = kScanItems * (int)sizeof(T) / 16;
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            const uint4 x = p[i];
+            if constexpr (sizeof(T) == 4) {
+                v[4 * i] = x.x; v[4 * i + 1] = x.y; v[4 * i + 2] = x.z; v[4 * i + 3] = x.w;
+            } else {
+                v[2 * i] = (uint64_t)x.x | ((uint64_t)x.y << 32);
+                v[2 * i + 1] = (uint64_t)x.z | ((uint64_t)x.w << 32);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < kScanItems; ++j) v[j] = (base + j < n) ? in[base + j] : T(0);
+    }
+}
+
+template <class T>
+__device__ __forceinline__ void scan_store(T *__restrict__ out, uint64_t base, uint64_t n, const T (&v)[kScanItems]) {
+    if ((reinterpret_cast<uintptr_t>(out) & 15) == 0 && base + kScanItems <= n) {
+        uint4 *p = reinterpret_cast<uint4 *>(out + base);
+        constexpr int NV = kScanItems * (int)sizeof(T) / 16;
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            uint4 x;
+            if constexpr (sizeof(T) == 4) {
+                x.x = v[4 * i]; x.y = v[4 * i + 1]; x.z = v[4 * i + 2]; x.w = v[4 * i + 3];
+            } else {
+                x.x = (uint32_t)v[2 * i]; x.y = (uint32_t)(v[2 * i] >> 32);
+                x.z = (uint32_t)v[2 * i + 1]; x.w = (uint32_t)(v[2 * i + 1] >> 32);
+            }
+            p[i] = x;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < kScanItems; ++j)
+            if (base + j < n) out[base + j] = v[j];
+    }
+}
+
 template <class T>
 __global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const T *__restrict__ in, uint64_t n,
                                                             T *__restrict__ block_sums) {
     __shared__ T sh[kScanBlock / 64 + 1];
     const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+    T v[kScanItems];
+    scan_load<T>(in, base, n, v);
     T s = 0;
 #pragma unroll
-    for (int j = 0; j < kScanItems; ++j)
-        if (base + j < n) s += in[base + j];
+    for (int j = 0; j < kScanItems; ++j) s += v[j];
     T total;
     (void)block_exclusive_scan<T>(s, sh, &total);
     if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
@@ -379,19 +427,19 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_down(const T *__restrict__ 
     __shared__ T sh[kScanBlock / 64 + 1];
     const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
     T v[kScanItems];
+    scan_load<T>(in, base, n, v);
     T s = 0;
 #pragma unroll
-    for (int j = 0; j < kScanItems; ++j) {
-        v[j] = (base + j < n) ? in[base + j] : T(0);
-        s += v[j];
-    }
+    for (int j = 0; j < kScanItems; ++j) s += v[j];
     T total;
     T run = block_exclusive_scan<T>(s, sh, &total) + (block_base ? block_base[blockIdx.x] : T(0));
 #pragma unroll
     for (int j = 0; j < kScanItems; ++j) {
-        if (base + j < n) out[base + j] = run;
-        run += v[j];
+        const T x = v[j];
+        v[j] = run;
+        run += x;
     }
+    scan_store<T>(out, base, n, v);
 }
 
 uint64_t scan_tmp_bytes(uint64_t n, uint64_t elem_bytes) {
