@@ -49,29 +49,40 @@ __device__ __forceinline__ uint32_t compact(uint64_t x) {
     return (uint32_t)x;
 }
 
-// Tropf & Herzog (1981): smallest Morton code > z inside the box spanned by
-// zmin and zmax, for z outside the box and zmin <= z <= zmax.
-__device__ __forceinline__ uint64_t bigmin(uint64_t z, uint64_t zmin, uint64_t zmax) {
-    uint64_t big = 0;
-    // z shares the common prefix of zmin and zmax, where every step is a no-op
-    const uint64_t d = zmin ^ zmax;
-    for (int i = d ? 63 - __clzll(d) : -1; i >= 0; --i) {
-        const uint64_t bit = 1ull << i;
-        const uint64_t dm = (i & 1) ? 0xAAAAAAAAAAAAAAAAull : 0x5555555555555555ull;
-        const uint64_t low = dm & (bit - 1);
-        const int v = (z & bit) != 0, lo = (zmin & bit) != 0, hi = (zmax & bit) != 0;
-        if (!v && !lo && hi) {
-            big = (zmin | bit) & ~low;
-            zmax = (zmax & ~bit) | low;
-        } else if (!v && lo && hi) {
-            return zmin;
-        } else if (v && !lo && !hi) {
-            return big;
-        } else if (v && !lo && hi) {
-            zmin = (zmin | bit) & ~low;
-        }
-    }
-    return big;
+// BIGMIN (Tropf & Herzog, 1981): the smallest Morton code above z inside the
+// box [x0, x1] x [y0, y1], for z = Morton(vx, vy) outside the box; 0 (<= z)
+// when there is none. Their algorithm walks the 64 code bits one at a time;
+// this closed form gets the same code from a few bit scans:
+//   * j, the highest code bit where z's prefix leaves the box: x bit b sits at
+//     2b and y bit b at 2b + 1, and a coordinate below (above) its range
+//     leaves at the top bit where it differs from its lower (upper) bound;
+//   * every code above z first differs from z at a bit i where it has 1 and z
+//     has 0; with i >= j the prefix above i stays inside the box, and setting
+//     bit i keeps it there iff that coordinate is below its upper bound and
+//     the bit is no higher than the top bit where the two differ;
+//   * the lowest such i gives the smallest code: keep the prefix above i, set
+//     bit i, and complete each coordinate with the least value in its range,
+//     max(lower bound, prefix with zeros below).
+__device__ __forceinline__ uint64_t bigmin(uint32_t vx, uint32_t vy, uint32_t x0, uint32_t y0, uint32_t x1,
+                                           uint32_t y1) {
+    int j = 0;
+    if (vx < x0) j = 2 * (31 - __clz(vx ^ x0));
+    else if (vx > x1) j = 2 * (31 - __clz(vx ^ x1));
+    if (vy < y0) j = max(j, 2 * (31 - __clz(vy ^ y0)) + 1);
+    else if (vy > y1) j = max(j, 2 * (31 - __clz(vy ^ y1)) + 1);
+    const uint32_t cx = vx < x1 ? ~vx & (uint32_t)((2ull << (31 - __clz(vx ^ x1))) - 1) : 0u;
+    const uint32_t cy = vy < y1 ? ~vy & (uint32_t)((2ull << (31 - __clz(vy ^ y1))) - 1) : 0u;
+    const uint64_t c = (spread(cx) | (spread(cy) << 1)) & (~0ull << j);
+    if (!c) return 0;
+    const int i = __ffsll((unsigned long long)c) - 1;
+    const int kx = (i + 1) >> 1, ky = i >> 1;  // bits of x and of y below code bit i
+    uint64_t px = vx, py = vy;
+    if (i & 1) py |= 1ull << ky;
+    else px |= 1ull << kx;
+    px = (px >> kx) << kx;
+    py = (py >> ky) << ky;
+    const uint32_t mx = px > x0 ? (uint32_t)px : x0, my = py > y0 ? (uint32_t)py : y0;
+    return spread(mx) | (spread(my) << 1);
 }
 
 template <int F, int LF>
@@ -102,7 +113,6 @@ __device__ __forceinline__ void rect_walk_from(const TreeView &t, const uint8_t 
                                                Emit &&emit) {
     const uint32_t x0 = r.x, y0 = r.y, x1 = r.z, y1 = r.w;
     if (x0 > x1 || y0 > y1) return;
-    const uint64_t zmin = spread(x0) | (spread(y0) << 1);
     const uint64_t zmax = spread(x1) | (spread(y1) << 1);
     while (p < t.n) {
         const uint64_t k = slot_key<uint64_t, uint32_t, LF>(t, p);
@@ -113,7 +123,7 @@ __device__ __forceinline__ void rect_walk_from(const TreeView &t, const uint8_t 
             ++p;
             continue;
         }
-        const uint64_t nz = bigmin(k, zmin, zmax);
+        const uint64_t nz = bigmin(x, y, x0, y0, x1, y1);
         if (nz <= k) break;  // no code of the rectangle above k
         // stay inside the current leaf while its keys are below the target
         const uint64_t leaf_end = (p / LF + 1) * LF;

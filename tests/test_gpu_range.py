@@ -119,6 +119,37 @@ def test_rect_duplicate_points(bpt, oracle, wl):
     assert np.array_equal(c, want_c) and np.array_equal(got, want_ids)
 
 
+@pytest.mark.parametrize("shift", [0, 28])
+def test_rect_every_box_on_a_grid(bpt, oracle, wl, shift):
+    """Half the cells of a 16 x 16 grid hold a point; every one of the 18496
+    boxes on the grid is searched, so the walk's BIGMIN jumps run from every
+    out-of-box key of every box. With shift 28 the grid sits in the top
+    coordinate bits and the box bounds are not aligned to it."""
+    rng = np.random.default_rng(11 + shift)
+    cells = rng.permutation(256)[:128]
+    gx, gy = (cells % 16).astype(np.uint64), (cells // 16).astype(np.uint64)
+    low = (1 << shift) - 1
+    x = ((gx << shift) | (rng.integers(0, low + 1, size=128, dtype=np.uint64) if shift else 0)).astype(np.uint32)
+    y = ((gy << shift) | (rng.integers(0, low + 1, size=128, dtype=np.uint64) if shift else 0)).astype(np.uint32)
+    ids = np.arange(128, dtype=np.uint32)
+    z = wl.morton_np(x, y)
+    order = np.argsort(z, kind="stable")
+    b = np.array([(a, c) for a in range(16) for c in range(a, 16)], dtype=np.uint64)
+    bx = np.repeat(b, len(b), axis=0)
+    by = np.tile(b, (len(b), 1))
+    r = np.stack([bx[:, 0] << shift, by[:, 0] << shift, (bx[:, 1] << shift) | low, (by[:, 1] << shift) | low],
+                 axis=1).astype(np.uint32)
+    want_c, _, want_ids = oracle.rect_bruteforce(x, y, ids, r)
+    t = bpt.Tree(z[order], ids[order], 8, 4)
+    for mode in MODES:
+        c, o, got = t.rect_search(r, group_mode=mode)
+        assert np.array_equal(c, want_c), mode
+        assert np.array_equal(got, want_ids), mode
+    t.set_option("coop", 0)
+    c, o, got = t.rect_search(r)
+    assert np.array_equal(c, want_c) and np.array_equal(got, want_ids)
+
+
 def test_grouping_keys_match_reference_functions(bpt, oracle):
     rng = np.random.default_rng(10)
     a = rng.integers(0, 1 << 64, size=3000, dtype=np.uint64)
