@@ -341,6 +341,19 @@ static bpt_status rect_order(bpt_tree *t, const uint32_t *d_rects, uint64_t nr, 
                                  bytes - 2 * zb, order, s);
 }
 
+// The search grid cut to the blocks the GPU holds at once. Each block stages
+// the top levels into LDS; the rectangle kernels hold few blocks per CU, so a
+// larger grid only stages them again.
+template <class Kernel>
+static bpt_status resident_grid(const bpt_tree *t, Kernel kernel, uint64_t nr, size_t lds, uint32_t *grid) {
+    int per_cu = 0;
+    BPT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(kernel),
+                                                         (int)t->block, lds));
+    *grid = search_grid(t, nr);
+    if (per_cu > 0) *grid = (uint32_t)std::min<uint64_t>(*grid, (uint64_t)device_cu_count() * per_cu);
+    return BPT_OK;
+}
+
 static bpt_status rect_run(bpt_tree *t, const uint32_t *d_rects, uint64_t nr, int mode,
                            uint64_t *d_counts, const uint64_t *d_offsets, uint32_t *d_ids, hipStream_t s) {
     BPT_CHECK(t->key_bytes == 8 && t->val_bytes == 4, BPT_ERR_UNSUPPORTED,
@@ -361,21 +374,23 @@ static bpt_status rect_run(bpt_tree *t, const uint32_t *d_rects, uint64_t nr, in
             if (d_ids) {
                 const size_t ids_lds = ((t->view.lds_bytes + 15) & ~(size_t)15) +
                                        (size_t)(t->block / 64) * kRectSortCap * sizeof(uint32_t);
-                k_rect_ids<F, LF><<<search_grid(t, nr), t->block, ids_lds, s>>>(
-                    t->view, (const uint4 *)d_rects, order, nr, d_offsets, d_ids);
+                uint32_t grid = 0;
+                BPT_TRY(resident_grid(t, k_rect_ids<F, LF>, nr, ids_lds, &grid));
+                k_rect_ids<F, LF><<<grid, t->block, ids_lds, s>>>(t->view, (const uint4 *)d_rects, order, nr,
+                                                                d_offsets, d_ids);
             } else if (t->coop && t->view.count[0] < (1ull << 32)) {
                 const bool off32 = t->addr32 && t->inner_bytes <= (1ull << 32);
-                auto go = [&](auto lanes, auto o32) {
-                    k_rect_count_coop<F, LF, decltype(lanes)::value, decltype(o32)::value>
-                        <<<search_grid(t, nr), t->block, lds, s>>>(t->view, (const uint4 *)d_rects, order, nr,
-                                                                  d_counts);
+                auto go = [&](auto lanes, auto o32) -> bpt_status {
+                    constexpr auto kernel = k_rect_count_coop<F, LF, decltype(lanes)::value, decltype(o32)::value>;
+                    uint32_t grid = 0;
+                    BPT_TRY(resident_grid(t, kernel, nr, lds, &grid));
+                    kernel<<<grid, t->block, lds, s>>>(t->view, (const uint4 *)d_rects, order, nr, d_counts);
+                    return BPT_OK;
                 };
                 if (t->coop_lanes == 4) {
-                    if (off32) go(IC<4>{}, std::true_type{});
-                    else go(IC<4>{}, std::false_type{});
+                    BPT_TRY(off32 ? go(IC<4>{}, std::true_type{}) : go(IC<4>{}, std::false_type{}));
                 } else {
-                    if (off32) go(IC<8>{}, std::true_type{});
-                    else go(IC<8>{}, std::false_type{});
+                    BPT_TRY(off32 ? go(IC<8>{}, std::true_type{}) : go(IC<8>{}, std::false_type{}));
                 }
             } else {
                 k_rect_count<F, LF><<<search_grid(t, nr), t->block, lds, s>>>(
