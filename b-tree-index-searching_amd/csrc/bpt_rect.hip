@@ -190,6 +190,7 @@ __global__ __launch_bounds__(256) void k_rect_count_coop(TreeView t, const uint4
 }
 
 constexpr uint32_t kRectSortSmall = 8;    // runs sorted by their own lane, in registers
+constexpr uint64_t kRectSplit = 4;        // longer runs are walked by a whole wave
 constexpr uint32_t kRectSortCap = 1024;   // runs sorted by the whole wave in LDS
 
 // Odd-even transposition network over kRectSortSmall registers.
@@ -266,9 +267,14 @@ __device__ __forceinline__ void wave_sort_global(uint32_t *p, uint64_t c) {
 // next offset (offsets are the exclusive scan of the counts), so rectangles
 // without points are skipped without a walk; the last one is always walked.
 // Each wave takes 64 rectangles at a time: their lanes walk and write the ids
-// in code order; then runs of up to kRectSortSmall ids are sorted by their
-// own lane in registers, and longer runs by the whole wave, one after
-// another: in LDS up to kRectSortCap ids, in place in global memory beyond.
+// in code order. A rectangle with more than kRectSplit ids is walked by the
+// whole wave instead: lane l takes sub-rectangle (l % 8, l / 8) of an 8 x 8
+// split of it (empty where a side is shorter than 8), and the lanes write
+// through a counter in LDS; the sub-rectangles cover the rectangle once, so
+// the run holds the same ids. Then runs of up to kRectSortSmall ids are
+// sorted by their own lane in registers, and longer runs by the whole wave,
+// one after another: in LDS up to kRectSortCap ids, in place in global
+// memory beyond.
 template <int F, int LF>
 __global__ __launch_bounds__(256) void k_rect_ids(TreeView t, const uint4 *__restrict__ rects,
                                                   const uint32_t *__restrict__ order, uint64_t nr,
@@ -284,16 +290,42 @@ __global__ __launch_bounds__(256) void k_rect_ids(TreeView t, const uint4 *__res
     const uint64_t stride = ((uint64_t)gridDim.x * blockDim.x) & ~63ull;
     for (uint64_t base = wave * 64; base < nr; base += stride) {
         const uint64_t j = base + lane;
-        uint64_t o = 0, c = 0;
+        uint64_t o = 0, c = 0, i = 0, known = 0;
         if (j < nr) {
-            const uint64_t i = order ? order[j] : j;
+            i = order ? order[j] : j;
             o = offsets[i];
-            if (!(i + 1 < nr && offsets[i + 1] == o)) {
+            known = i + 1 < nr ? offsets[i + 1] - o : 0;
+            if (i + 1 == nr || (known > 0 && known <= kRectSplit)) {
                 uint32_t *out = ids + o;
                 rect_walk<F, LF>(t, lds, rects[i], [&](uint64_t slot) {
                     out[c++] = slot_val<uint64_t, uint32_t, LF>(t, slot);
                 });
             }
+        }
+        uint64_t wide = __ballot(known > kRectSplit);
+        if (wide) {
+            uint32_t *ctr = buf;  // the sorts below use buf only after every walk
+            const uint32_t kx = lane & 7, ky = lane >> 3;
+            while (wide) {
+                const int src = __ffsll((unsigned long long)wide) - 1;
+                wide &= wide - 1;
+                const uint64_t so = __shfl(o, src, 64), sc = __shfl(known, src, 64);
+                const uint4 r = rects[__shfl(i, src, 64)];
+                if (lane == 0) *ctr = 0;
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+                const uint64_t w = (uint64_t)r.z - r.x + 1, h = (uint64_t)r.w - r.y + 1;
+                const uint64_t xa = r.x + w * kx / 8, xe = r.x + w * (kx + 1) / 8;
+                const uint64_t ya = r.y + h * ky / 8, ye = r.y + h * (ky + 1) / 8;
+                if (xa < xe && ya < ye) {
+                    const uint4 sub = make_uint4((uint32_t)xa, (uint32_t)ya, (uint32_t)(xe - 1), (uint32_t)(ye - 1));
+                    rect_walk<F, LF>(t, lds, sub, [&](uint64_t slot) {
+                        const uint32_t at = atomicAdd(ctr, 1u);
+                        if (at < sc) ids[so + at] = slot_val<uint64_t, uint32_t, LF>(t, slot);
+                    });
+                }
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+            }
+            if (known > kRectSplit) c = known;
         }
         if (c > 1 && c <= kRectSortSmall) sort_small(ids + o, (uint32_t)c);
         // the lanes' writes are visible to the whole wave (one CU, one L1)
