@@ -360,6 +360,9 @@ class Tree:
     def point_search_dev(self, dq, dout, mode=PLAIN, stream=None):
         check(lib().bpt_point_search(self.h, dq.ptr, dq.n, dout.ptr, mode, stream))
 
+    def point_search_ws_size(self, nq, mode):
+        return int(lib().bpt_point_search_ws_size(self.h, nq, mode))
+
     def point_search(self, q, mode=PLAIN, stream=None):
         dq, own = _dev(q, self.kdt)
         out = DeviceArray(dq.n, self.vdt)

@@ -52,14 +52,16 @@ def build(force: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-6000:]}")
     os.replace(tmp, LIB)
-    # the drop-in driver with the reference's command line, linked to the library
-    exe = os.path.join(HERE, "bptree_main")
-    cmd = [HIPCC, *FLAGS, os.path.join(HERE, "examples", "bptree_main.cpp"), "-o", exe + ".tmp",
-           "-L", HERE, "-lbptree_amd", "-Wl,-rpath,$ORIGIN"]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"bptree_main failed to build:\n{r.stderr[-6000:]}")
-    os.replace(exe + ".tmp", exe)
+    # the drop-in driver with the reference's command line, and the driver of
+    # the reference-named grouping functions, linked to the library
+    for name in ("bptree_main", "bptree_groups"):
+        exe = os.path.join(HERE, name)
+        cmd = [HIPCC, *FLAGS, os.path.join(HERE, "examples", name + ".cpp"), "-o", exe + ".tmp",
+               "-L", HERE, "-lbptree_amd", "-Wl,-rpath,$ORIGIN"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"{name} failed to build:\n{r.stderr[-6000:]}")
+        os.replace(exe + ".tmp", exe)
     # seeded generators of the reference's input files (host code only)
     gen = os.path.join(HERE, "bptree_gen")
     cmd = ["g++", "-O3", "-std=c++17", "-Wall", os.path.join(HERE, "examples", "bptree_gen.cpp"), "-o", gen + ".tmp"]

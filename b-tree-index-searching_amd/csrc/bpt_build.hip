@@ -173,47 +173,80 @@ static bpt_status check_sorted(const void *d_keys, uint64_t n, hipStream_t s, bo
 // Device buffers for the current layout. Buffers that hold the new size and
 // are at most twice it are kept; otherwise they are reallocated, with 1/16
 // slack when `slack` is set so that a run of growing updates rarely
-// reallocates.
+// reallocates. New buffers are allocated before any old one is freed, so a
+// failed allocation leaves the tree's buffers and view as they were.
 static bpt_status ensure_storage(bpt_tree *t, bool slack) {
-    auto fit = [&](uint8_t *&p, uint64_t &cap, uint64_t need, const char *what) -> bpt_status {
-        if (p && need <= cap && need * 2 >= cap) return BPT_OK;
-        (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
+    struct Fit {
+        uint8_t *p;
+        uint64_t cap;
+        bool fresh;
+    };
+    auto fit = [&](uint8_t *cur, uint64_t cap, uint64_t need, const char *what, Fit *out) -> bpt_status {
+        *out = {cur, cap, false};
+        if (cur && need <= cap && need * 2 >= cap) return BPT_OK;
+        *out = {nullptr, 0, true};
         if (need == 0) return BPT_OK;
         const uint64_t want = slack ? need + need / 16 : need;
+        BPT_CHECK(t->alloc_limit == 0 || want <= t->alloc_limit, BPT_ERR_NOMEM,
+                  "%s: %llu bytes above the tree's allocation limit of %llu", what, (unsigned long long)want,
+                  (unsigned long long)t->alloc_limit);
+        uint8_t *p = nullptr;
         hipError_t e = hipMalloc(&p, want);
         if (e != hipSuccess) return hip_fail(e, what, __FILE__, __LINE__);
-        cap = want;
+        *out = {p, want, true};
         return BPT_OK;
     };
-    bpt_status st = fit(t->d_leaves, t->leaf_alloc, t->leaf_bytes, "hipMalloc(leaves)");
-    if (st == BPT_OK) st = fit(t->d_inner, t->inner_alloc, t->inner_bytes, "hipMalloc(inner)");
+    Fit leaves, inner;
+    BPT_TRY(fit(t->d_leaves, t->leaf_alloc, t->leaf_bytes, "hipMalloc(leaves)", &leaves));
+    bpt_status st = fit(t->d_inner, t->inner_alloc, t->inner_bytes, "hipMalloc(inner)", &inner);
+    if (st != BPT_OK) {
+        if (leaves.fresh) (void)hipFree(leaves.p);
+        return st;
+    }
+    if (leaves.fresh) {
+        (void)hipFree(t->d_leaves);
+        t->d_leaves = leaves.p;
+        t->leaf_alloc = leaves.cap;
+    }
+    if (inner.fresh) {
+        (void)hipFree(t->d_inner);
+        t->d_inner = inner.p;
+        t->inner_alloc = inner.cap;
+    }
     t->view.inner = t->d_inner;
     t->view.leaves = t->d_leaves;
-    return st;
+    return BPT_OK;
 }
 
 bpt_status tree_rebuild(bpt_tree *t, const void *d_keys, const void *d_vals, uint64_t n, hipStream_t s) {
+    // the new layout and buffers first; if either fails, nothing of the tree
+    // has been written and it is restored as it was
+    const uint64_t old_n = t->n, old_leaf_bytes = t->leaf_bytes, old_inner_bytes = t->inner_bytes;
+    const TreeView old_view = t->view;
+    t->n = n;
+    bpt_status st = compute_layout(t);
+    if (st == BPT_OK) st = ensure_storage(t, true);
+    if (st != BPT_OK) {
+        t->n = old_n;
+        t->leaf_bytes = old_leaf_bytes;
+        t->inner_bytes = old_inner_bytes;
+        t->view = old_view;
+        return st;
+    }
     if (t->learned) {
         learned_free(t->learned);
         t->learned = nullptr;
     }
-    t->n = n;
-    bpt_status st = compute_layout(t);
-    if (st == BPT_OK) st = ensure_storage(t, true);
-    if (st == BPT_OK) {
-        st = dispatch_tree(t, [&](auto kt, auto vt, auto fc, auto lc) {
-            using K = typename decltype(kt)::type;
-            using V = typename decltype(vt)::type;
-            return build_impl<K, V, decltype(fc)::value, decltype(lc)::value>(t, d_keys, d_vals, s);
-        });
-    }
+    st = dispatch_tree(t, [&](auto kt, auto vt, auto fc, auto lc) {
+        using K = typename decltype(kt)::type;
+        using V = typename decltype(vt)::type;
+        return build_impl<K, V, decltype(fc)::value, decltype(lc)::value>(t, d_keys, d_vals, s);
+    });
     if (st == BPT_OK) {
         hipError_t e = hipStreamSynchronize(s);
         if (e != hipSuccess) st = hip_fail(e, "rebuild sync", __FILE__, __LINE__);
     }
-    if (st != BPT_OK) {  // leave an empty, usable tree behind
+    if (st != BPT_OK) {  // a device error while writing: leave an empty, usable tree behind
         (void)hipFree(t->d_leaves);
         (void)hipFree(t->d_inner);
         t->d_leaves = t->d_inner = nullptr;
@@ -366,6 +399,11 @@ bpt_status bpt_tree_set_option(bpt_tree *t, const char *key, int64_t value) {
     if (strcmp(key, "lanes") == 0) {
         BPT_CHECK(value == 4 || value == 8, BPT_ERR_INVALID, "lanes per query must be 4 or 8");
         t->coop_lanes = (uint32_t)value;
+        return BPT_OK;
+    }
+    if (strcmp(key, "alloc_limit") == 0) {
+        BPT_CHECK(value >= 0, BPT_ERR_INVALID, "alloc_limit must be >= 0");
+        t->alloc_limit = (uint64_t)value;
         return BPT_OK;
     }
     set_error("unknown tree option '%s'", key);

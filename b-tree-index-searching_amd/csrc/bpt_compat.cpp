@@ -4,7 +4,8 @@
 // Searches and partitions run on the device through bpt_* calls. What stays
 // on the host is what the reference itself keeps on the host: copying the
 // caller's arrays, and the group-list bookkeeping of its grouping loop (the
-// greedy merge of adjacent buckets below the group cap).
+// greedy merge of adjacent buckets below the group cap, over bucket counts
+// the device computes).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -13,6 +14,7 @@
 #include <cstring>
 #include <vector>
 
+#include "bpt_internal.hpp"
 #include "bptree.h"
 #include "compat/BPTree.h"
 #include "compat/grouping.h"
@@ -80,113 +82,56 @@ ulli *range_counts(ulli *lo, ulli *hi, ulli n, BTPage *tree, int mode) {
     return res;
 }
 
-int bit_width(ulli v) {
-    int b = 0;
-    while (v) { v >>= 1; ++b; }
-    return b;
-}
-
-// The reference's grouping loop over passes [p0, pend) (grouping.cu:1746-1791,
-// grouping2d.cu:635-680), run on the bucket counts of the grouped order:
-// fewer than 10 groups -> per-group split of groups >= cap with the merge of
-// grouping.cu:855-872 (a single group with one non-empty bucket stays whole);
-// otherwise -> groups > cap split, small groups passed through as one entry,
-// and one merge over all groups' buckets (grouping.cu:1333-1352). Stops once
-// no group exceeds cap. bucket(i, pass) is the bucket of grouped element i.
-template <class BucketFn>
-void reference_group_lists(ulli n, ulli cap, int p0, int pend, BucketFn bucket, std::vector<ulli> &list,
-                           std::vector<ulli> &prefix) {
-    list.assign(1, n);
-    prefix.assign(1, 0);
-    for (int pass = p0; pass < pend; ++pass) {
-        std::vector<ulli> nl, np;
-        if (list.size() < 10) {
-            for (size_t i = 0; i < list.size(); ++i) {
-                if (list[i] == 0) continue;
-                if (list[i] < cap) {
-                    nl.push_back(list[i]);
-                    np.push_back(prefix[i]);
-                    continue;
-                }
-                ulli hist[256] = {0}, pre[256];
-                for (ulli k = prefix[i]; k < prefix[i] + list[i]; ++k) hist[bucket(k, pass)]++;
-                int valid = 0;
-                for (int j = 0; j < 256; ++j) valid += hist[j] != 0;
-                if (valid == 1 && list.size() == 1) {
-                    nl.push_back(list[i]);
-                    np.push_back(prefix[i]);
-                    continue;
-                }
-                ulli acc = 0;
-                for (int j = 0; j < 256; ++j) { pre[j] = acc; acc += hist[j]; }
-                for (int j = 0; j < 256; ++j) {
-                    if (!hist[j]) continue;
-                    int k = 0;
-                    ulli cur = hist[j];
-                    while (cur < cap && j + k + 1 < 256) {
-                        if (cur + hist[j + k + 1] < cap) { cur += hist[j + k + 1]; ++k; }
-                        else break;
-                    }
-                    nl.push_back(cur);
-                    np.push_back(prefix[i] + pre[j]);
-                    j += k;
-                }
-            }
-        } else {
-            std::vector<ulli> out(256 * list.size(), 0);
-            for (size_t i = 0; i < list.size(); ++i) {
-                if (list[i] > cap) {
-                    for (ulli k = prefix[i]; k < prefix[i] + list[i]; ++k) out[256 * i + bucket(k, pass)]++;
-                } else {
-                    out[256 * i] = list[i];
-                }
-            }
-            ulli pt = 0;
-            for (size_t i = 0; i < out.size(); ++i) {
-                if (!out[i]) continue;
-                size_t j = 0;
-                ulli cur = out[i];
-                while (cur < cap && i + j + 1 < out.size()) {
-                    if (cur + out[i + j + 1] < cap) { cur += out[i + j + 1]; ++j; }
-                    else break;
-                }
-                nl.push_back(cur);
-                np.push_back(pt);
-                pt += cur;
-                i += j;
-            }
+// Runs the reference's grouping pass loop (msd_group) on device copies of
+// the caller's host arrays. arrays[0] (and arrays[1] for kind 1) are the keys;
+// the others move with them. start_pass < -100 takes the start pass from the
+// device maxima (grouping.cu:1722-1744, grouping2d.cu:588-635). On success
+// the grouped arrays are left on the device in dev[], owned by the caller.
+bool device_grouping(const ulli *const *host, int narrays, int kind, ulli n, ulli cap, int start_pass,
+                     std::vector<ulli> &groupList, std::vector<ulli> &groupPrefix, ulli **dev) {
+    for (int k = 0; k < narrays; ++k) dev[k] = nullptr;
+    bool good = true;
+    for (int k = 0; k < narrays && good; ++k) {
+        void *d = nullptr;
+        good = ok(bpt_device_alloc(n ? n * 8 : 8, &d), "device alloc");
+        dev[k] = static_cast<ulli *>(d);
+        good = good && (n == 0 || ok(bpt_copy_h2d(d, host[k], n * 8, nullptr), "copy to device"));
+    }
+    if (good && start_pass < -100) {
+        uint64_t ma = 0, mb = 0;
+        good = ok(n == 0 ? BPT_OK
+                         : bpt::pair_max<uint64_t>(reinterpret_cast<uint64_t *>(dev[0]),
+                                                   reinterpret_cast<uint64_t *>(dev[kind == 1 ? 1 : 0]), n, &ma,
+                                                   &mb, nullptr),
+                  "grouping maximum");
+        start_pass = kind == 0 ? bpt::group_start_pass_1d(ma) : bpt::group_start_pass_2d(ma, mb);
+    }
+    std::vector<uint64_t> list, prefix;
+    if (good) {
+        uint64_t *arr[4] = {nullptr, nullptr, nullptr, nullptr};
+        for (int k = 0; k < narrays; ++k) arr[k] = reinterpret_cast<uint64_t *>(dev[k]);
+        good = ok(bpt::msd_group(arr, narrays, kind, n, cap, start_pass, list, prefix, nullptr), "grouping");
+    }
+    if (!good) {
+        for (int k = 0; k < narrays; ++k) {
+            bpt_device_free(dev[k]);
+            dev[k] = nullptr;
         }
-        list.swap(nl);
-        prefix.swap(np);
-        ulli mx = 0;
-        for (ulli v : list) mx = std::max(mx, v);
-        if (mx <= cap) break;
+        return false;
     }
+    groupList.assign(list.begin(), list.end());
+    groupPrefix.assign(prefix.begin(), prefix.end());
+    return true;
 }
 
-// Sorts host keys (with up to two payloads) on the device by key bits
-// [lo_bit, 64); results go to new[]'d host arrays.
-bool device_group_by_key(const ulli *keys, const ulli *d1, const ulli *d2, ulli n, int lo_bit, ulli *out_k,
-                         ulli *out_1, ulli *out_2) {
-    ulli *dk = to_device(keys, n);
-    void *perm = nullptr, *tmp = nullptr;
-    bool good = dk && ok(bpt_device_alloc(n * 4, &perm), "device alloc") &&
-                ok(bpt_device_alloc(n * 8, &tmp), "device alloc") &&
-                ok(bpt_sort_perm(dk, 8, n, (uint32_t)lo_bit, 64, static_cast<uint32_t *>(perm), nullptr), "sort") &&
-                ok(bpt_gather(dk, 8, static_cast<uint32_t *>(perm), n, tmp, nullptr), "gather") &&
-                ok(bpt_device_sync(), "sync") && to_host(out_k, static_cast<const ulli *>(tmp), n);
-    const ulli *payload[2] = {d1, d2};
-    ulli *outp[2] = {out_1, out_2};
-    for (int p = 0; p < 2 && good; ++p) {
-        if (!payload[p]) continue;
-        ulli *dp = to_device(payload[p], n);
-        good = dp && ok(bpt_gather(dp, 8, static_cast<uint32_t *>(perm), n, tmp, nullptr), "gather") &&
-               ok(bpt_device_sync(), "sync") && to_host(outp[p], static_cast<const ulli *>(tmp), n);
-        bpt_device_free(dp);
+// The grouped device arrays to new[]'d host arrays, freeing the device copies.
+bool grouped_to_host(ulli **dev, int narrays, ulli n, ulli **host_out) {
+    bool good = true;
+    for (int k = 0; k < narrays; ++k) {
+        good = good && to_host(host_out[k], dev[k], n);
+        bpt_device_free(dev[k]);
+        dev[k] = nullptr;
     }
-    bpt_device_free(dk);
-    bpt_device_free(perm);
-    bpt_device_free(tmp);
     return good;
 }
 
@@ -311,42 +256,35 @@ void *SearchingBTreeGPURangeResults(ulli *input_key_start, ulli *input_key_end, 
 }
 
 // ------------------------------------------------------------------ grouping.h
+// The group lists are those of the reference's pass loop; the caller's
+// vectors are replaced (the reference appends to them, which every one of its
+// call sites passes empty). The reference's small-group copy from offset 0
+// (grouping.cu:789) is not reproduced: every element keeps its key and
+// payloads.
 int bucketIdentify(ulli input, int nPass) {
     const int shift = 8 * (7 - nPass);
     return (shift < 0 || shift >= 64) ? 0 : (int)((input >> shift) & 0xFF);
 }
 
-static void group1d(ulli *input, ulli *d1, ulli *d2, ulli n, int start_pass, ulli cap, ulli *ok_k, ulli *ok_1,
-                    ulli *ok_2, std::vector<ulli> &groupList, std::vector<ulli> &groupPrefix, bool *good) {
-    // grouped order = key order over the bytes from the start pass down
-    const int lo_bit = 0;
-    *good = device_group_by_key(input, d1, d2, n, lo_bit, ok_k, ok_1, ok_2);
-    if (!*good) return;
-    reference_group_lists(n, cap, start_pass, 8, [&](ulli i, int p) { return bucketIdentify(ok_k[i], p); },
-                          groupList, groupPrefix);
-}
-
 void *genGrouping(ulli *input, ulli inputSize, std::vector<ulli> &groupList, std::vector<ulli> &groupPrefix,
                   ulli Local_Group_Size) {
-    ulli mx = 0;
-    for (ulli i = 0; i < inputSize; ++i) mx = std::max(mx, input[i]);
-    const int start = 8 - (int)ceilf((float)bit_width(mx) / 8.0f);  // grouping.cu:1736-1744
-    std::vector<ulli> keys(inputSize ? inputSize : 1);
-    bool good = false;
-    group1d(input, nullptr, nullptr, inputSize, start, Local_Group_Size, keys.data(), nullptr, nullptr, groupList,
-            groupPrefix, &good);
-    if (!good) return nullptr;
-    return to_device(keys.data(), inputSize);  // device pointer, caller frees
+    const ulli *h[1] = {input};
+    ulli *d[1];
+    // start pass from the maximum (grouping.cu:1722-1744)
+    if (!device_grouping(h, 1, 0, inputSize, Local_Group_Size, -1000, groupList, groupPrefix, d)) return nullptr;
+    return d[0];  // device pointer, the caller frees it (BPTreeExactSearch.cu:699)
 }
 
 void genGroupingWithOnedata(ulli *input, ulli *input_data, ulli inputSize, ulli *&output, ulli *&output_data,
                             std::vector<ulli> &groupList, std::vector<ulli> &groupPrefix, ulli Local_Group_Size) {
     output = new ulli[inputSize ? inputSize : 1];
     output_data = new ulli[inputSize ? inputSize : 1];
-    bool good = false;
-    group1d(input, input_data, nullptr, inputSize, 0, Local_Group_Size, output, output_data, nullptr, groupList,
-            groupPrefix, &good);
-    if (!good) fprintf(stderr, "genGroupingWithOnedata failed\n");
+    const ulli *h[2] = {input, input_data};
+    ulli *d[2], *out[2] = {output, output_data};
+    // passes from 0 (grouping.cu:1567)
+    if (!device_grouping(h, 2, 0, inputSize, Local_Group_Size, 0, groupList, groupPrefix, d) ||
+        !grouped_to_host(d, 2, inputSize, out))
+        fprintf(stderr, "genGroupingWithOnedata failed\n");
 }
 
 void genGroupingWithTwodata(ulli *input, ulli *input_data, ulli *input_data2, ulli inputSize, ulli *&output,
@@ -355,11 +293,12 @@ void genGroupingWithTwodata(ulli *input, ulli *input_data, ulli *input_data2, ul
     output = new ulli[inputSize ? inputSize : 1];
     output_data = new ulli[inputSize ? inputSize : 1];
     output_data2 = new ulli[inputSize ? inputSize : 1];
-    bool good = false;
-    // genGroupingWithTwodata starts at pass 0 (grouping.cu:1365-1529)
-    group1d(input, input_data, input_data2, inputSize, 0, Local_Group_Size, output, output_data, output_data2,
-            groupList, groupPrefix, &good);
-    if (!good) fprintf(stderr, "genGroupingWithTwodata failed\n");
+    const ulli *h[3] = {input, input_data, input_data2};
+    ulli *d[3], *out[3] = {output, output_data, output_data2};
+    // passes from 0 (grouping.cu:1413)
+    if (!device_grouping(h, 3, 0, inputSize, Local_Group_Size, 0, groupList, groupPrefix, d) ||
+        !grouped_to_host(d, 3, inputSize, out))
+        fprintf(stderr, "genGroupingWithTwodata failed\n");
 }
 
 // ------------------------------------------------------------------ grouping2d.h
@@ -371,32 +310,14 @@ int bucketIdentify2d(ulli input, ulli input2, int nPass) {
 
 void genGrouping2d(ulli *input, ulli *input2, ulli inputSize, std::vector<ulli> &groupList, ulli *&output,
                    ulli *&output2, std::vector<ulli> &groupPrefix, ulli Local_Group_Size) {
-    const ulli n = inputSize;
-    output = new ulli[n ? n : 1];
-    output2 = new ulli[n ? n : 1];
-    ulli ma = 0, mb = 0;
-    for (ulli i = 0; i < n; ++i) { ma = std::max(ma, input[i]); mb = std::max(mb, input2[i]); }
-    int sa = 16 - (int)ceilf((float)bit_width(ma) / 3.0f), sb = 16 - (int)ceilf((float)bit_width(mb) / 3.0f);
-    const int p0 = std::max(sa, sb);  // grouping2d.cu:615-635
-    const int q0 = std::max(p0, 0);   // passes below 0 put everything in bucket 0
-    // grouped order: by the 2-D buckets of up to 8 successive passes from q0
-    std::vector<ulli> comp(n ? n : 1);
-    for (ulli i = 0; i < n; ++i) {
-        ulli c = 0;
-        for (int k = 0; k < 8; ++k) {
-            const int p = q0 + k;
-            c = (c << 8) | (p < 16 ? (ulli)bucketIdentify2d(input[i], input2[i], p) : 0);
-        }
-        comp[i] = c;
-    }
-    std::vector<ulli> sorted_comp(n ? n : 1);
-    if (!device_group_by_key(comp.data(), input, input2, n, 0, sorted_comp.data(), output, output2)) {
+    output = new ulli[inputSize ? inputSize : 1];
+    output2 = new ulli[inputSize ? inputSize : 1];
+    const ulli *h[2] = {input, input2};
+    ulli *d[2], *out[2] = {output, output2};
+    // start pass from both maxima (grouping2d.cu:588-635)
+    if (!device_grouping(h, 2, 1, inputSize, Local_Group_Size, -1000, groupList, groupPrefix, d) ||
+        !grouped_to_host(d, 2, inputSize, out))
         fprintf(stderr, "genGrouping2d failed\n");
-        return;
-    }
-    reference_group_lists(n, Local_Group_Size, p0, 16,
-                          [&](ulli i, int p) { return bucketIdentify2d(output[i], output2[i], p); }, groupList,
-                          groupPrefix);
 }
 
 // ------------------------------------------------------------------ zOrder2d.h

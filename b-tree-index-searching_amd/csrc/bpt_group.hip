@@ -198,6 +198,10 @@ __device__ __host__ __forceinline__ uint64_t spread32(uint32_t v) {
     return x;
 }
 
+// Quotient of a Z-order coordinate by its interval; a zero interval (an
+// all-zero side, see zorder_interval) gives 0.
+__host__ __device__ __forceinline__ uint64_t zorder_div(uint64_t v, uint64_t d) { return d ? v / d : 0; }
+
 __global__ void k_morton(const uint32_t *__restrict__ x, const uint32_t *__restrict__ y, uint64_t n,
                          uint64_t *__restrict__ z) {
     const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
@@ -242,8 +246,8 @@ __global__ void k_zorder_keys(const KT *__restrict__ a, const KT *__restrict__ b
                               uint64_t ix, uint64_t iy, uint64_t *__restrict__ out) {
     const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) {
-        const uint32_t xg = (uint32_t)((uint64_t)a[i] / ix);
-        const uint32_t yg = (uint32_t)((uint64_t)b[i] / iy);
+        const uint32_t xg = (uint32_t)zorder_div((uint64_t)a[i], ix);
+        const uint32_t yg = (uint32_t)zorder_div((uint64_t)b[i], iy);
         out[i] = spread32(xg) | (spread32(yg) << 1);
     }
 }
@@ -288,15 +292,19 @@ bpt_status pair_max(const KT *a, const KT *b, uint64_t n, uint64_t *ma, uint64_t
     return BPT_OK;
 }
 
-// zOrder2d.cu:129-130: ceil of max / 2^32 in float, as an int. A quotient of
-// 2^31 or more does not fit the int; the reference's host (x86) conversion
-// then yields INT_MIN, which the floor of 1 turns into 1. That outcome is
-// written out here, so host and device (whose conversion saturates) agree.
+// zOrder2d.cu:129-130 computes int interval = ceil((float)max / 2^32); the
+// kernel (:57, :27) carries it as ulli then int, and :29 divides the ulli
+// input by it, converting the int back to ulli (sign-extended). This is that
+// 64-bit divisor. A quotient of 2^31 or more does not fit the int: the
+// reference's host is x86, whose float-to-int conversion gives INT_MIN, so
+// the divisor is 0xFFFFFFFF80000000 (every quotient 0 or 1). That outcome is
+// written out, because the device's own conversion saturates instead.
+// max == 0 gives 0, a division by zero in the reference; zorder_div (above)
+// takes every quotient as 0 then (every coordinate of that side is 0 anyway).
 __host__ __device__ static uint64_t zorder_interval(uint64_t m) {
     const float r = (float)m / 4294967296.0f;
-    if (r >= 2147483648.0f) return 1;
-    const int iv = (int)ceilf(r);
-    return iv < 1 ? 1 : (uint64_t)iv;
+    if (r >= 2147483648.0f) return 0xFFFFFFFF80000000ull;
+    return (uint64_t)(int64_t)(int)ceilf(r);
 }
 
 // grouping2d.cu:610-635
@@ -315,8 +323,8 @@ __global__ void k_zorder_keys_dev(const KT *__restrict__ a, const KT *__restrict
     const uint64_t ix = zorder_interval(mx[0]), iy = zorder_interval(mx[1]);
     const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) {
-        const uint32_t xg = (uint32_t)((uint64_t)a[i] / ix);
-        const uint32_t yg = (uint32_t)((uint64_t)b[i] / iy);
+        const uint32_t xg = (uint32_t)zorder_div((uint64_t)a[i], ix);
+        const uint32_t yg = (uint32_t)zorder_div((uint64_t)b[i], iy);
         out[i] = (uint32_t)((spread32(xg) | (spread32(yg) << 1)) >> 32);
     }
 }
@@ -358,6 +366,11 @@ bpt_status group2d_keys(const KT *a, const KT *b, uint64_t n, uint32_t *out, hip
     return BPT_OK;
 }
 
+template bpt_status pair_max<uint64_t>(const uint64_t *, const uint64_t *, uint64_t, uint64_t *, uint64_t *,
+                                      hipStream_t);
+int group_start_pass_2d(uint64_t maxa, uint64_t maxb) { return group2d_start_pass(maxa, maxb); }
+// grouping.cu:1736-1744: 8 - ceil(bits of the maximum / 8), in float as there
+int group_start_pass_1d(uint64_t maxval) { return 8 - (int)ceilf((float)bitwidth(maxval) / 8.0f); }
 template bpt_status zorder_keys<uint32_t>(const uint32_t *, const uint32_t *, uint64_t, uint64_t *, hipStream_t);
 template bpt_status zorder_keys<uint64_t>(const uint64_t *, const uint64_t *, uint64_t, uint64_t *, hipStream_t);
 template bpt_status group2d_keys<uint32_t>(const uint32_t *, const uint32_t *, uint64_t, uint32_t *, hipStream_t);

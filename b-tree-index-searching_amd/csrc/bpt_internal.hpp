@@ -10,6 +10,7 @@
 #include <string>
 #include <type_traits>
 #include <utility>
+#include <vector>
 
 #include "bptree.h"
 
@@ -88,6 +89,7 @@ struct bpt_tree {
     uint32_t group_xcd = 1;          // grouped search: chunks split per XCD
     uint32_t group_lanes = 4;        // grouped search: lanes per query at the leaf (4 or 8)
     uint32_t group_blocks = 2;       // grouped search: 512-thread blocks per CU
+    uint64_t alloc_limit = 0;        // largest leaf / inner buffer a rebuild may allocate (0: none)
     bpt::TreeView view{};
     bpt::Workspace ws;             // grouped / learned scratch owned by the tree
     bpt::LearnedModel *learned = nullptr;
@@ -664,6 +666,17 @@ uint64_t sort_ws_bytes(uint64_t n, uint32_t key_bytes);
 template <class KT>
 bpt_status sort_pairs(const KT *keys_in, const uint32_t *vals_in, uint64_t n, uint32_t lo_bit, uint32_t hi_bit,
                       void *ws, uint64_t ws_bytes, KT **keys_sorted, uint32_t **vals_sorted, hipStream_t s);
+// Maxima of two arrays (pass a twice for one); synchronizes s.
+template <class KT>
+bpt_status pair_max(const KT *a, const KT *b, uint64_t n, uint64_t *ma, uint64_t *mb, hipStream_t s);
+// The reference's grouping pass loop over device arrays of n uint64
+// (bpt_msd_group.hip): kind 0 buckets arrays[0] by bytes (grouping.cu),
+// kind 1 buckets (arrays[0], arrays[1]) by nibble products (grouping2d.cu).
+// Every array is permuted with the keys; list / prefix receive the groups.
+bpt_status msd_group(uint64_t *const *arrays, int narrays, int kind, uint64_t n, uint64_t cap, int start_pass,
+                     std::vector<uint64_t> &list, std::vector<uint64_t> &prefix, hipStream_t s);
+int group_start_pass_1d(uint64_t maxval);
+int group_start_pass_2d(uint64_t maxa, uint64_t maxb);
 bpt_status point_grouped(const bpt_tree *t, const void *d_q, uint64_t nq, void *d_out,
                          void *d_ws, uint64_t ws_bytes, hipStream_t s);
 uint64_t point_grouped_ws(const bpt_tree *t, uint64_t nq);

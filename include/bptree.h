@@ -122,8 +122,11 @@ bpt_status bpt_tree_set_launch(bpt_tree *t, uint32_t block, uint32_t blocks_per_
  * its own options (O(tree + batch) per call); a trained learned model is
  * dropped. The merge's scratch is the tree's workspace (about twice the
  * tree's slot bytes plus 16 B per slot), kept for later calls; the leaf and
- * inner buffers are reused when the new tree fits them. Synchronize s. On
- * failure the tree is left empty. */
+ * inner buffers are reused when the new tree fits them. Synchronize s.
+ * When a step fails before the new tree is written (argument checks, the
+ * workspace, the merge, allocating new leaf or inner buffers), the tree is
+ * left unchanged, learned model included; only a device error while the new
+ * tree is written leaves it empty. */
 bpt_status bpt_tree_upsert(bpt_tree *t, const void *d_keys, const void *d_vals, uint64_t n, bpt_stream s);
 bpt_status bpt_tree_erase(bpt_tree *t, const void *d_keys, uint64_t n, bpt_stream s);
 
@@ -140,7 +143,10 @@ bpt_status bpt_tree_erase(bpt_tree *t, const void *d_keys, uint64_t n, bpt_strea
  * "group_lanes" (lanes per query at the leaf: 4, the default, or 8),
  * "group_blocks" (search blocks of 512 threads per CU, default 2) and
  * "group_xcd" (1 = each XCD searches its own contiguous part of the
- * bucket-ordered queries, the default). */
+ * bucket-ordered queries, the default). "alloc_limit" (bytes; 0, the
+ * default, means none) caps each leaf or inner buffer that an update's
+ * rebuild allocates: a larger one fails with BPT_ERR_NOMEM, as an exhausted
+ * device would, and the tree stays unchanged. */
 bpt_status bpt_tree_set_option(bpt_tree *t, const char *key, int64_t value);
 
 /* ------------------------------------------------------------------ */

@@ -305,24 +305,36 @@ void ora_demorton2d(u64 z, uint32_t *x, uint32_t *y) {
 }
 
 /* zOrder2d.cu:129-130: int interval = ceil((float)max / (float)2^32).
- * A zero interval (max == 0) would divide by zero there; treated as 1. */
+ * A quotient of 2^31 or more (max >= 2^63 - 2^38 after float rounding) does
+ * not fit the int. The reference's host is x86, whose float-to-int
+ * conversion (cvttss2si) then yields INT_MIN; that result is written out
+ * here rather than left to the overflow. max == 0 gives 0. */
 int ora_zorder_interval(u64 maxval) {
     float r = (float)maxval / 4294967296.0f;
-    /* a quotient of 2^31 or more does not fit the int: x86 gives INT_MIN,
-     * which the floor below turns into 1; written out to avoid the overflow */
-    if (r >= 2147483648.0f) return 1;
-    int iv = (int)ceilf(r);
-    return iv < 1 ? 1 : iv;
+    if (r >= 2147483648.0f) return INT32_MIN;
+    return (int)ceilf(r);
+}
+
+/* The divisor the reference's kernel applies: the int interval travels as
+ * ulli (zOrder2d.cu:57, sign-extended), back to int (:27, low 32 bits), and
+ * :29 divides the ulli input by it, which converts the int to ulli again
+ * (sign-extended). INT_MIN thus divides by 0xFFFFFFFF80000000, so every
+ * quotient is 0 or 1. */
+u64 ora_zorder_divisor(u64 maxval) {
+    return (u64)(int64_t)ora_zorder_interval(maxval);
 }
 
 /* zOrder2d.cu:27-54: quotient truncated to int, then only its low 32 bits
- * enter the interleave (the mask cascade starts with & 0xFFFFFFFF). */
+ * enter the interleave (the mask cascade starts with & 0xFFFFFFFF).
+ * A zero divisor (max == 0, a division by zero in the reference, whose
+ * result CUDA leaves undefined) is taken as quotient 0: every coordinate of
+ * that side is 0 then, so the order of the keys does not depend on it. */
 void ora_zorder_keys(const u64 *a, const u64 *b, u64 n, u64 *out) {
     u64 ma = 0, mb = 0;
     for (u64 i = 0; i < n; i++) { if (a[i] > ma) ma = a[i]; if (b[i] > mb) mb = b[i]; }
-    u64 ix = (u64)ora_zorder_interval(ma), iy = (u64)ora_zorder_interval(mb);
+    u64 ix = ora_zorder_divisor(ma), iy = ora_zorder_divisor(mb);
     for (u64 i = 0; i < n; i++)
-        out[i] = ora_morton2d((uint32_t)(a[i] / ix), (uint32_t)(b[i] / iy));
+        out[i] = ora_morton2d((uint32_t)(ix ? a[i] / ix : 0), (uint32_t)(iy ? b[i] / iy : 0));
 }
 
 /* grouping.cu:25-29 with BIT_PASS 8, BIT_MASK 0xFF (grouping.cu:15-17) */
@@ -373,6 +385,106 @@ u64 ora_merge_groups(const u64 *hist, u64 nb, u64 cap, u64 *sizes, u64 *first_bu
         ng++;
         j += k;
     }
+    return ng;
+}
+
+/* ------------------------------------------------------------------------ */
+/* The grouping pass loop and the group lists it returns                     */
+/* ------------------------------------------------------------------------ */
+/* Bucket of grouped element k at `pass`: 1-D grouping.cu:25-29, 2-D
+ * grouping2d.cu:24-31. */
+static int grp_bucket(const u64 *a, const u64 *b, const u64 *perm, u64 k, int kind, int pass) {
+    u64 i = perm[k];
+    return kind == 0 ? ora_bucket_identify(a[i], pass) : ora_bucket2d(a[i], b[i], pass);
+}
+
+/* Histogram of the range [lo, lo + cnt) of the grouped order, and a stable
+ * counting sort of that range by bucket (the relocate kernels,
+ * grouping.cu:115-478,513-770 and grouping2d.cu:106-343, move each group's
+ * elements to their bucket's run; order inside a bucket is the reference's
+ * atomics' and is not reproduced). */
+static void grp_split(const u64 *a, const u64 *b, u64 *perm, u64 *tmp, u64 lo, u64 cnt, int kind, int pass,
+                      u64 *hist) {
+    u64 pos[256];
+    memset(hist, 0, 256 * sizeof(u64));
+    for (u64 k = lo; k < lo + cnt; k++) hist[grp_bucket(a, b, perm, k, kind, pass)]++;
+    u64 acc = 0;
+    for (int j = 0; j < 256; j++) { pos[j] = acc; acc += hist[j]; }
+    for (u64 k = lo; k < lo + cnt; k++) tmp[lo + pos[grp_bucket(a, b, perm, k, kind, pass)]++] = perm[k];
+    memcpy(perm + lo, tmp + lo, cnt * sizeof(u64));
+}
+
+/* The pass loop of genGrouping (grouping.cu:1746-1791), genGroupingWithOnedata
+ * (:1567-1612), genGroupingWithTwodata (:1413-1460) and genGrouping2d
+ * (grouping2d.cu:635-680). Each pass runs one of two bookkeeping forms:
+ *  - fewer than 10 groups, radixGroupGPU (grouping.cu:776-884; 2-D
+ *    grouping2d.cu:349-461): empty groups are dropped; groups below the cap
+ *    pass through; a larger group is split into its 256 buckets, whose
+ *    non-empty runs merge greedily while the running total stays below the
+ *    cap (:855-872), with prefixes group start + bucket start (:869); a sole
+ *    group whose elements all share one bucket stays as it is (:840-844).
+ *  - 10 groups or more, radixGroupSingleKernelGPU (grouping.cu:1288-1363
+ *    with the kernel :692-770; 2-D grouping2d.cu:464-542): every group above
+ *    the cap is split into 256 bucket counts, every other group is one count
+ *    in its first slot, and one greedy merge runs over all groups' slots in
+ *    order (:1333-1352), so it may join the tail of one group with the head
+ *    of the next; prefixes are running sums.
+ * The loop ends after the pass whose largest group is at most the cap, or
+ * after the last pass (8 for 1-D, 16 for 2-D). An empty list ends it too (the
+ * reference reads groupList[0] of the empty list there, n == 0).
+ * The small groups of the first form are copied from offset 0 in the
+ * reference (grouping.cu:789), which loses data but not counts; the lists do
+ * not depend on it and the data here moves correctly.
+ * perm: in/out, n entries, the grouped order as indices into a (and b).
+ * list, prefix: room for n + 1 entries. Returns the number of groups. */
+u64 ora_group_lists(const u64 *a, const u64 *b, u64 n, u64 cap, int kind, int start_pass, u64 *perm, u64 *list,
+                    u64 *prefix) {
+    const int end_pass = kind == 0 ? 8 : 16;
+    u64 ng = 1;
+    list[0] = n;
+    prefix[0] = 0;
+    u64 *tmp = (u64 *)malloc((n ? n : 1) * sizeof(u64));
+    u64 *nl = (u64 *)malloc((n + 1) * sizeof(u64));
+    u64 *np = (u64 *)malloc((n + 1) * sizeof(u64));
+    u64 hist[256];
+    for (int pass = start_pass; pass < end_pass; pass++) {
+        u64 m = 0;
+        if (ng < 10) {
+            for (u64 i = 0; i < ng; i++) {
+                if (list[i] == 0) continue;
+                if (list[i] < cap) { nl[m] = list[i]; np[m++] = prefix[i]; continue; }
+                u64 h0[256];
+                memset(h0, 0, sizeof h0);
+                for (u64 k = prefix[i]; k < prefix[i] + list[i]; k++) h0[grp_bucket(a, b, perm, k, kind, pass)]++;
+                int valid = 0;
+                for (int j = 0; j < 256; j++) valid += h0[j] != 0;
+                if (valid == 1 && ng == 1) { nl[m] = list[i]; np[m++] = prefix[i]; continue; }
+                grp_split(a, b, perm, tmp, prefix[i], list[i], kind, pass, hist);
+                u64 sizes[256], first[256], pre[256], acc = 0;
+                for (int j = 0; j < 256; j++) { pre[j] = acc; acc += hist[j]; }
+                u64 g = ora_merge_groups(hist, 256, cap, sizes, first);
+                for (u64 t = 0; t < g; t++) { nl[m] = sizes[t]; np[m++] = prefix[i] + pre[first[t]]; }
+            }
+        } else {
+            u64 *slots = (u64 *)calloc(256 * ng, sizeof(u64));
+            for (u64 i = 0; i < ng; i++) {
+                if (list[i] > cap) grp_split(a, b, perm, tmp, prefix[i], list[i], kind, pass, slots + 256 * i);
+                else slots[256 * i] = list[i];
+            }
+            u64 *sizes = (u64 *)malloc(256 * ng * sizeof(u64)), *first = (u64 *)malloc(256 * ng * sizeof(u64));
+            u64 g = ora_merge_groups(slots, 256 * ng, cap, sizes, first), run = 0;
+            for (u64 t = 0; t < g; t++) { nl[m] = sizes[t]; np[m++] = run; run += sizes[t]; }
+            free(slots); free(sizes); free(first);
+        }
+        memcpy(list, nl, m * sizeof(u64));
+        memcpy(prefix, np, m * sizeof(u64));
+        ng = m;
+        if (ng == 0) break;
+        u64 mx = 0;
+        for (u64 i = 0; i < ng; i++) if (list[i] > mx) mx = list[i];
+        if (mx <= cap) break;
+    }
+    free(tmp); free(nl); free(np);
     return ng;
 }
 

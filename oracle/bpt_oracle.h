@@ -78,9 +78,12 @@ void ora_range_closed(const u64 *keys, u64 n, const u64 *lo, const u64 *hi, u64 
 u64  ora_morton2d(uint32_t x, uint32_t y);
 void ora_demorton2d(u64 z, uint32_t *x, uint32_t *y);
 /* Z-order grouping key of a (start, end) pair, zOrder2d.cu:27-54 with the
- * intervals of zOrder2d.cu:125-130 (float ceil of max / 2^32). */
+ * intervals of zOrder2d.cu:125-130 (float ceil of max / 2^32, as an int:
+ * INT_MIN when the quotient does not fit, 0 when max == 0) and the divisor
+ * that int becomes in the kernel (zOrder2d.cu:27-29,57). */
 void ora_zorder_keys(const u64 *a, const u64 *b, u64 n, u64 *out);
 int  ora_zorder_interval(u64 maxval);
+u64  ora_zorder_divisor(u64 maxval);
 /* grouping.cu:25-29 (8-bit MSD bucket of pass p) */
 int  ora_bucket_identify(u64 x, int pass);
 /* grouping2d.cu:24-31 (product of two 4-bit nibbles at a 3-bit stride) */
@@ -93,6 +96,17 @@ int  ora_group2d_start_pass(u64 maxa, u64 maxb);
  * below cap, grouping.cu:855-872. Writes group sizes and bucket starts;
  * returns the number of groups. */
 u64  ora_merge_groups(const u64 *hist, u64 nbuckets, u64 cap, u64 *sizes, u64 *first_bucket);
+
+/* The grouping pass loop of genGrouping / genGroupingWithOnedata /
+ * genGroupingWithTwodata (grouping.cu:1413-1460,1567-1612,1746-1791; kind 0,
+ * 8-bit buckets, passes start_pass..7) and genGrouping2d
+ * (grouping2d.cu:635-680; kind 1, nibble-product buckets, passes
+ * start_pass..15), with both bookkeeping forms (grouping.cu:776-884 and
+ * :1288-1363). perm (n entries, in/out) is the grouped order as indices into
+ * a (and b for kind 1); list and prefix get the group sizes and starts and
+ * need room for n + 1 entries. Returns the number of groups. */
+u64  ora_group_lists(const u64 *a, const u64 *b, u64 n, u64 cap, int kind, int start_pass, u64 *perm,
+                     u64 *list, u64 *prefix);
 
 /* ---------------- rectangles (no reference counterpart) ---------------- */
 /* Brute force: for each rect (x0,y0,x1,y1 inclusive) count points inside and,

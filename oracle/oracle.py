@@ -48,6 +48,10 @@ def lib():
         L.ora_zorder_keys.argtypes = [u64p, u64p, C.c_uint64, u64p]
         L.ora_zorder_interval.argtypes = [C.c_uint64]
         L.ora_zorder_interval.restype = C.c_int
+        L.ora_zorder_divisor.argtypes = [C.c_uint64]
+        L.ora_zorder_divisor.restype = C.c_uint64
+        L.ora_group_lists.argtypes = [u64p, u64p, C.c_uint64, C.c_uint64, C.c_int, C.c_int, u64p, u64p, u64p]
+        L.ora_group_lists.restype = C.c_uint64
         L.ora_bucket_identify.argtypes = [C.c_uint64, C.c_int]
         L.ora_bucket2d.argtypes = [C.c_uint64, C.c_uint64, C.c_int]
         L.ora_group_start_pass.argtypes = [C.c_uint64]
@@ -190,7 +194,33 @@ def zorder_keys(a, b):
 
 
 def zorder_interval(m):
+    """zOrder2d.cu:129-130: the int interval (INT_MIN when it overflows)."""
     return int(lib().ora_zorder_interval(int(m)))
+
+
+def zorder_divisor(m):
+    """The 64-bit divisor that interval becomes in zOrder2d.cu:27-29,57."""
+    return int(lib().ora_zorder_divisor(int(m)))
+
+
+def group_lists(a, b, cap, kind, start_pass):
+    """Group lists of the reference's grouping pass loop (ora_group_lists).
+
+    kind 0: genGrouping / genGroupingWithOnedata / genGroupingWithTwodata
+    (grouping.cu); kind 1: genGrouping2d (grouping2d.cu), b required.
+    Returns (sizes, prefixes, perm) with perm the grouped order of indices."""
+    a, ap = _u64(a)
+    n = len(a)
+    if kind == 1:
+        b, bp = _u64(b)
+    else:
+        b, bp = _u64(np.zeros(1, dtype=np.uint64))
+    perm = np.arange(n, dtype=np.uint64)
+    sizes = np.zeros(n + 1, dtype=np.uint64)
+    pre = np.zeros(n + 1, dtype=np.uint64)
+    g = lib().ora_group_lists(ap, bp, n, int(cap), int(kind), int(start_pass), perm.ctypes.data_as(u64p),
+                              sizes.ctypes.data_as(u64p), pre.ctypes.data_as(u64p))
+    return sizes[:g].copy(), pre[:g].copy(), perm
 
 
 def bucket_identify(x, p):

@@ -130,3 +130,33 @@ def test_updates_keep_options_and_drop_learned(bpt, oracle):
     with pytest.raises(bpt.BptError):                 # the model no longer matches the keys
         t.point_search(q, bpt.LEARNED)
     t.close()
+
+
+def test_failed_rebuild_allocation_leaves_tree_unchanged(bpt, oracle):
+    """An upsert whose rebuild needs a larger leaf buffer than the tree may
+    allocate ("alloc_limit", forced from the host) fails with BPT_ERR_NOMEM
+    and leaves every key, value and the learned model as they were; with the
+    limit lifted the same upsert goes through."""
+    rng = np.random.default_rng(5)
+    keys = np.unique(rng.integers(0, 1 << 62, 200_000, dtype=np.uint64))
+    vals = np.arange(len(keys), dtype=np.uint64)
+    t = bpt.Tree(keys, vals, 8, 8, fanout=16, leaf_cap=8)
+    t.learned_build(leaf_models=1024, train_steps=50, train_samples=1 << 14)
+    t.set_option("alloc_limit", t.info()["leaf_bytes"])   # the current leaf buffer, no more
+    k = rng.integers(0, 1 << 62, 100_000, dtype=np.uint64)   # grows the tree by half
+    v = rng.integers(0, 1 << 62, 100_000, dtype=np.uint64)
+    with pytest.raises(bpt.BptError) as e:
+        t.upsert(k, v)
+    assert e.value.status == 3                        # BPT_ERR_NOMEM
+    q = np.concatenate([keys[rng.integers(0, len(keys), 50_000)], k[:20_000]])
+    want = oracle.point_closed(keys, vals, q, (1 << 64) - 1)
+    assert t.info()["n"] == len(keys)
+    for mode in (bpt.PLAIN, bpt.GROUPED, bpt.LEARNED):
+        assert np.array_equal(t.point_search(q, mode), want), mode
+    t.set_option("alloc_limit", 0)
+    t.upsert(k, v)
+    m = Model(keys, vals)
+    m.upsert(k, v)
+    assert t.info()["n"] == len(m.k)
+    assert np.array_equal(t.point_search(q), oracle.point_closed(m.k, m.v, q, (1 << 64) - 1))
+    t.close()

@@ -147,17 +147,28 @@ def test_morton_numpy_cascade_matches(oracle, wl):
 
 
 def test_zorder_keys_reference_semantics(oracle):
-    # zOrder2d.cu:129-130: interval = ceil((float)max / 2^32); values below
-    # 2^32 give interval 1 so the key is Morton(a, b) exactly.
+    # zOrder2d.cu:129-130: interval = ceil((float)max / 2^32), an int; values
+    # below 2^32 give interval 1, so the key is Morton(a, b) exactly.
     assert oracle.zorder_interval(0xFFFFFFFF) == 1          # float(2^32-1) rounds to 2^32
     assert oracle.zorder_interval(1 << 33) == 2
     assert oracle.zorder_interval((1 << 33) + 1) == 2       # float rounding
     assert oracle.zorder_interval(3 << 40) == 3 << 8
-    # a quotient of 2^31 or more overflows the reference's int: the x86 result
-    # (INT_MIN) falls below the floor of 1
-    assert oracle.zorder_interval((1 << 31) << 32) == 1
-    assert oracle.zorder_interval((1 << 64) - 1) == 1
     assert oracle.zorder_interval((1 << 62)) == 1 << 30
+    # max == 0: 0 / 2^32 = 0, so the interval is 0 (a division by zero follows
+    # in the reference's kernel; the oracle takes every quotient as 0)
+    assert oracle.zorder_interval(0) == 0 and oracle.zorder_divisor(0) == 0
+    # float(max) >= 2^63 makes the quotient >= 2^31, which the int cannot hold:
+    # x86's cvttss2si gives INT_MIN. Floats in [2^62, 2^63) are 2^39 apart, so
+    # 2^63 - 2^38 is the midpoint below 2^63 and rounds up to it (ties to even).
+    INT_MIN = -(1 << 31)
+    assert oracle.zorder_interval(1 << 63) == INT_MIN
+    assert oracle.zorder_interval((1 << 64) - 1) == INT_MIN   # float rounds to 2^64
+    assert oracle.zorder_interval((1 << 63) - (1 << 38)) == INT_MIN
+    assert oracle.zorder_interval((1 << 63) - (1 << 38) - 1) == (1 << 31) - (1 << 7)
+    # the int travels as ulli (zOrder2d.cu:57, sign-extended), back to int
+    # (:27) and divides the ulli input (:29), which sign-extends it again
+    assert oracle.zorder_divisor(1 << 63) == 0xFFFFFFFF80000000
+    assert oracle.zorder_divisor(3 << 40) == 3 << 8
     a = np.array([5, 7, 0xFFFFFFFF], dtype=np.uint64)
     b = np.array([9, 0, 3], dtype=np.uint64)
     got = oracle.zorder_keys(a, b)
@@ -167,6 +178,15 @@ def test_zorder_keys_reference_semantics(oracle):
     iv = oracle.zorder_interval(int(a2.max()))
     assert iv == 6  # max = 3 * 2^33 = 6 * 2^32
     assert [int(v) for v in got2] == [oracle.morton2d(int(x) // iv, int(y)) for x, y in zip(a2, b)]
+    # max >= 2^63: every quotient is 1 at or above 0xFFFFFFFF80000000, else 0
+    a3 = np.array([SENT64, 0xFFFFFFFF80000000, 0xFFFFFFFF7FFFFFFF, 5], dtype=np.uint64)
+    b3 = np.array([9, 0, 3, 1], dtype=np.uint64)
+    got3 = oracle.zorder_keys(a3, b3)
+    assert [int(v) for v in got3] == [oracle.morton2d(1, 9), oracle.morton2d(1, 0),
+                                      oracle.morton2d(0, 3), oracle.morton2d(0, 1)]
+    # a side that is all zeros: coordinate 0 for every pair
+    got4 = oracle.zorder_keys(np.zeros(3, dtype=np.uint64), b)
+    assert [int(v) for v in got4] == [oracle.morton2d(0, int(y)) for y in b]
 
 
 def test_bucket_functions(oracle):
