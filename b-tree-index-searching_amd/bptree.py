@@ -84,6 +84,7 @@ SIGNATURES = [
     ("bpt_point_search", C.c_int, [vp, vp, C.c_uint64, vp, C.c_int, vp]),
     ("bpt_point_search_ws_size", C.c_uint64, [vp, C.c_uint64, C.c_int]),
     ("bpt_point_search_ws", C.c_int, [vp, vp, C.c_uint64, vp, C.c_int, vp, C.c_uint64, vp]),
+    ("bpt_point_search_noalloc", C.c_int, [vp, vp, C.c_uint64, vp, C.c_int, vp]),
     ("bpt_tree_set_group_bits", C.c_int, [vp, C.c_uint32]),
     ("bpt_lower_bound", C.c_int, [vp, vp, C.c_uint64, vp, vp]),
     ("bpt_range_count", C.c_int, [vp, vp, vp, C.c_uint64, vp, vp, vp]),
@@ -362,6 +363,10 @@ class Tree:
 
     def point_search_ws_size(self, nq, mode):
         return int(lib().bpt_point_search_ws_size(self.h, nq, mode))
+
+    def point_search_noalloc_dev(self, dq, dout, mode=PLAIN, stream=None):
+        """bpt_point_search_noalloc: the library's static scratch, no allocation."""
+        check(lib().bpt_point_search_noalloc(self.h, dq.ptr, dq.n, dout.ptr, mode, stream))
 
     def point_search(self, q, mode=PLAIN, stream=None):
         dq, own = _dev(q, self.kdt)

@@ -173,6 +173,19 @@ uint64_t bpt_point_search_ws_size(const bpt_tree *t, uint64_t nq, int mode);
 bpt_status bpt_point_search_ws(const bpt_tree *t, const void *d_q, uint64_t nq, void *d_out,
                                int mode, void *d_ws, uint64_t ws_bytes, bpt_stream s);
 bpt_status bpt_tree_set_group_bits(bpt_tree *t, uint32_t bits);
+/* No counterpart in the reference: the same search for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops). It calls no hipMalloc / hipFree. The grouped and learned modes use
+ * a static device scratch of BPT_NOALLOC_SCRATCH_BYTES per GPU (held until
+ * the process ends) instead of the tree's workspace; a batch whose workspace
+ * (bpt_point_search_ws_size) is larger fails with BPT_ERR_CAPACITY (about 6M
+ * queries on a 4-byte-key tree, 3M on an 8-byte one). All calls on one GPU
+ * share its scratch, so calls that overlap on one GPU (other streams or
+ * threads) race; calls in one stream are safe. The first call on a device
+ * looks the scratch's address up. */
+#define BPT_NOALLOC_SCRATCH_BYTES (64ull << 20)
+bpt_status bpt_point_search_noalloc(const bpt_tree *t, const void *d_q, uint64_t nq, void *d_out,
+                                    int mode, bpt_stream s);
 
 /* pos[i] = lower_bound(keys, q[i]) (first slot with key >= q[i]; n if none).
  * Replaces the position half of kernelSearchingKeyRangeGroupStateTwo
