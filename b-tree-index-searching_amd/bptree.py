@@ -528,7 +528,7 @@ def sort_pairs(keys, vals=None):
 
 class BatchConfig(C.Structure):
     _fields_ = [("fanout", C.c_uint32), ("leaf_cap", C.c_uint32), ("lds_bytes", C.c_uint32),
-                ("chunk", C.c_uint32), ("blocks_per_cu", C.c_uint32)]
+                ("chunk", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("lanes", C.c_uint32)]
 
 
 def _offp(a):
@@ -538,10 +538,10 @@ def _offp(a):
 class Forest:
     """Thousands of 8-byte-key trees in one device arena (bpt_forest_*)."""
 
-    def __init__(self, dkeys, dvals, key_off, fanout=16, lds_bytes=32768, stream=None):
+    def __init__(self, dkeys, dvals, key_off, fanout=16, lds_bytes=32768, lanes=8, stream=None):
         self.key_off = np.ascontiguousarray(key_off, dtype=np.uint64)
         self.ntrees = len(self.key_off) - 1
-        cfg = BatchConfig(fanout, 8, lds_bytes, 0, 0)
+        cfg = BatchConfig(fanout, 8, lds_bytes, 0, 0, lanes)
         h = vp()
         check(lib().bpt_forest_build(C.byref(cfg), self.ntrees, dkeys.ptr, dvals.ptr,
                                      _offp(self.key_off), C.byref(h), stream))

@@ -40,6 +40,7 @@ struct WorkItem {
 
 struct bpt_forest {
     uint32_t ntrees = 0, fanout = 16, leaf_cap = 8, lds_bytes = 32768;
+    uint32_t lanes = 8;  // lanes per query of the cooperative descent (8 or 4)
     uint8_t *d_leaves = nullptr, *d_inner = nullptr;
     bpt::TreeView *d_views = nullptr;
     std::vector<bpt::TreeView> views;
@@ -143,7 +144,7 @@ __device__ __forceinline__ uint32_t next_item(uint32_t *cursor, const uint32_t *
     return 0xFFFFFFFFu;
 }
 
-template <int F>
+template <int F, int G>
 __global__ __launch_bounds__(256) void k_forest_batch(const TreeView *__restrict__ views,
                                                       const WorkItem *__restrict__ items, uint32_t nitems,
                                                       uint32_t *__restrict__ counter, const uint64_t *__restrict__ q,
@@ -179,7 +180,7 @@ __global__ __launch_bounds__(256) void k_forest_batch(const TreeView *__restrict
             for (uint64_t base = wi.begin + (uint64_t)w * 64; base < wi.end; base += (uint64_t)nw * 64) {
                 const uint64_t my = base + lane;
                 const uint64_t key = my < wi.end ? __builtin_nontemporal_load(&q[my]) : 0ull;
-                const uint64_t r = coop_lookup64<uint64_t, uint64_t, F, LF, true, kOpValue>(tv, lds, key, ~0ull);
+                const uint64_t r = coop_lookup64<uint64_t, uint64_t, F, LF, true, kOpValue, G>(tv, lds, key, ~0ull);
                 if (my < wi.end) __builtin_nontemporal_store(r, &out[my]);
             }
         } else {
@@ -187,10 +188,10 @@ __global__ __launch_bounds__(256) void k_forest_batch(const TreeView *__restrict
                 const uint64_t my = base + lane;
                 const bool ok = my < wi.end;
                 const uint64_t a = ok ? lo[my] : 0ull, b = ok ? hi[my] : 0ull;
-                const uint64_t pa = coop_lookup64<uint64_t, uint64_t, F, LF, false, kOpLowerBound>(tv, lds, a, 0ull);
+                const uint64_t pa = coop_lookup64<uint64_t, uint64_t, F, LF, false, kOpLowerBound, G>(tv, lds, a, 0ull);
                 const bool top = b == ~0ull;
                 const uint64_t pb0 =
-                    coop_lookup64<uint64_t, uint64_t, F, LF, false, kOpLowerBound>(tv, lds, top ? b : b + 1, 0ull);
+                    coop_lookup64<uint64_t, uint64_t, F, LF, false, kOpLowerBound, G>(tv, lds, top ? b : b + 1, 0ull);
                 const uint64_t pb = top ? tv.n : pb0;
                 if (ok) {
                     first[my] = pa;
@@ -214,40 +215,67 @@ __host__ __device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
 __host__ __device__ __forceinline__ uint64_t hmix(uint64_t seed, uint64_t stream, uint64_t tree, uint64_t i) {
     return splitmix64(splitmix64(splitmix64(seed ^ splitmix64(stream)) ^ tree) ^ i);
 }
-// Key i of tree t (n keys): stratified uniform over [0, 2^64), one key per
-// stride, so keys are unique and sorted by construction.
-__device__ __forceinline__ uint64_t synth_key(uint64_t seed, uint32_t t, uint64_t n, uint64_t i) {
-    const uint64_t stride = ~0ull / n;
-    return i * stride + hmix(seed, 1, t, i) % stride;
-}
-
-__global__ void k_synth_keys(uint64_t seed, const uint64_t *__restrict__ key_off, uint32_t ntrees, uint64_t total,
-                             uint64_t *__restrict__ keys, uint64_t *__restrict__ vals) {
+// Keys of tree t: n draws uniform over [0, 2^64), h(seed, 1 + 16 * attempt,
+// t, i), sorted; attempt 0 unless a tree's draws collided (a redraw).
+__global__ void k_synth_raw(uint64_t seed, const uint64_t *__restrict__ key_off, uint32_t ntrees, uint64_t total,
+                            uint64_t *__restrict__ keys) {
     const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += step) {
         const uint32_t t = find_tree(key_off, ntrees, g);
-        const uint64_t n = key_off[t + 1] - key_off[t], i = g - key_off[t];
-        keys[g] = synth_key(seed, t, n, i);
-        vals[g] = i;  // value = rank
+        keys[g] = hmix(seed, 1, t, g - key_off[t]);
     }
 }
 
+// tree[j] = tree of the element that the key sort put at j.
+__global__ void k_tree_of(const uint32_t *__restrict__ perm, const uint64_t *__restrict__ key_off, uint32_t ntrees,
+                          uint64_t total, uint32_t *__restrict__ tree) {
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += step)
+        tree[g] = find_tree(key_off, ntrees, perm[g]);
+}
+
+__global__ void k_gather_u64(const uint64_t *__restrict__ src, const uint32_t *__restrict__ idx, uint64_t n,
+                             uint64_t *__restrict__ dst) {
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += step) dst[g] = src[idx[g]];
+}
+
+// Values = rank; dup[t] |= 1 when two keys of tree t are equal.
+__global__ void k_rank_and_dups(const uint64_t *__restrict__ keys, const uint64_t *__restrict__ key_off,
+                                uint32_t ntrees, uint64_t total, uint64_t *__restrict__ vals,
+                                uint32_t *__restrict__ dup) {
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += step) {
+        const uint32_t t = find_tree(key_off, ntrees, g);
+        vals[g] = g - key_off[t];
+        if (g > key_off[t] && keys[g] == keys[g - 1]) atomicOr(&dup[t], 1u);
+    }
+}
+
+// A redraw of tree t's n keys (before sorting).
+__global__ void k_synth_redraw(uint64_t seed, uint32_t t, uint32_t attempt, uint64_t n, uint64_t *__restrict__ keys) {
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step)
+        keys[i] = hmix(seed, 1 + 16ull * attempt, t, i);
+}
+
 // Point queries: qpk per key of every tree; a fraction hit_pct/100 are keys
-// of the tree, the rest uniform 64-bit values.
-__global__ void k_synth_points(uint64_t seed, const uint64_t *__restrict__ key_off, uint32_t ntrees, uint32_t qpk,
-                               uint32_t hit_pct, uint64_t total, uint64_t *__restrict__ q) {
+// of the tree (a uniform rank), the rest uniform 64-bit values.
+__global__ void k_synth_points(uint64_t seed, const uint64_t *__restrict__ keys, const uint64_t *__restrict__ key_off,
+                               uint32_t ntrees, uint32_t qpk, uint32_t hit_pct, uint64_t total,
+                               uint64_t *__restrict__ q) {
     const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += step) {
         const uint32_t t = find_tree(key_off, ntrees, g / qpk);
         const uint64_t n = key_off[t + 1] - key_off[t], j = g - key_off[t] * qpk;
         const uint64_t r = hmix(seed, 2, t, j);
-        q[g] = ((r >> 32) % 100 < hit_pct) ? synth_key(seed, t, n, (r & 0xFFFFFFFFull) % n) : hmix(seed, 3, t, j);
+        q[g] = ((r >> 32) % 100 < hit_pct) ? keys[key_off[t] + (r & 0xFFFFFFFFull) % n] : hmix(seed, 3, t, j);
     }
 }
 
-// Range queries: start rank uniform, width 2^e + (r mod 2^e) ranks with e
-// uniform in [lo_exp, hi_exp), clipped to the tree; [key_s, key_e].
-__global__ void k_synth_ranges(uint64_t seed, const uint64_t *__restrict__ key_off,
+// Range queries: start rank uniform, width floor(2^u) ranks with u uniform
+// in [lo_exp, hi_exp] (log-uniform), clipped to the tree; [key_s, key_e].
+__global__ void k_synth_ranges(uint64_t seed, const uint64_t *__restrict__ keys, const uint64_t *__restrict__ key_off,
                                const uint64_t *__restrict__ range_off, uint32_t ntrees, uint32_t lo_exp,
                                uint32_t hi_exp, uint64_t total, uint64_t *__restrict__ lo, uint64_t *__restrict__ hi) {
     const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
@@ -256,11 +284,12 @@ __global__ void k_synth_ranges(uint64_t seed, const uint64_t *__restrict__ key_o
         const uint64_t n = key_off[t + 1] - key_off[t], j = g - range_off[t];
         const uint64_t r1 = hmix(seed, 4, t, j), r2 = hmix(seed, 5, t, j);
         const uint64_t s = r1 % n;
-        const uint32_t e = lo_exp + (uint32_t)((r2 >> 40) % (hi_exp - lo_exp));
-        const uint64_t wdt = (1ull << e) + (r2 & ((1ull << e) - 1));
+        const double u = (double)lo_exp + (double)(hi_exp - lo_exp) * ((double)(r2 >> 11) * 0x1.0p-53);
+        uint64_t wdt = (uint64_t)floor(exp2(u));
+        wdt = wdt < 1 ? 1 : wdt;
         const uint64_t last = (s + wdt < n ? s + wdt : n) - 1;
-        lo[g] = synth_key(seed, t, n, s);
-        hi[g] = synth_key(seed, t, n, last);
+        lo[g] = keys[key_off[t] + s];
+        hi[g] = keys[key_off[t] + last];
     }
 }
 
@@ -287,11 +316,14 @@ bpt_status bpt_forest_build(const bpt_batch_config *cfg, uint32_t ntrees, const 
     BPT_CHECK(ntrees >= 1, BPT_ERR_INVALID, "no trees");
     const uint32_t F = cfg->fanout ? cfg->fanout : 16;
     BPT_CHECK(F == 16 || F == 32 || F == 64, BPT_ERR_INVALID, "fanout %u not in {16,32,64}", F);
+    BPT_CHECK(cfg->lanes == 0 || cfg->lanes == 4 || cfg->lanes == 8, BPT_ERR_INVALID, "lanes %u not in {4,8}",
+              cfg->lanes);
     hipStream_t s = as_stream(stream);
     auto *f = new bpt_forest;
     f->ntrees = ntrees;
     f->fanout = F;
     f->lds_bytes = cfg->lds_bytes ? std::min<uint32_t>(cfg->lds_bytes, 65536) : 32768;
+    f->lanes = cfg->lanes ? cfg->lanes : 8;
     f->views.resize(ntrees);
     f->desc.resize(ntrees);
     const uint64_t LF = 8, kb = 8;
@@ -454,10 +486,18 @@ static bpt_status batch_enqueue(bpt_batch *b, hipStream_t s) {
     const uint32_t grid = (uint32_t)std::min<uint64_t>((uint64_t)device_cu_count() * b->blocks_per_cu,
                                                        std::max<uint32_t>(b->nitems, 1));
     const size_t lds = std::max<uint32_t>(f->lds_bytes, 16);
+    auto go = [&](auto fc, auto gc) {
+        k_forest_batch<decltype(fc)::value, decltype(gc)::value><<<grid, 256, lds, s>>>(
+            f->d_views, b->d_items, b->nitems, b->d_counter, b->q, b->out, b->lo, b->hi, b->first, b->count);
+    };
+    auto with_lanes = [&](auto fc) {
+        if (f->lanes == 4) go(fc, IC<4>{});
+        else go(fc, IC<8>{});
+    };
     switch (f->fanout) {
-        case 16: k_forest_batch<16><<<grid, 256, lds, s>>>(f->d_views, b->d_items, b->nitems, b->d_counter, b->q, b->out, b->lo, b->hi, b->first, b->count); break;
-        case 32: k_forest_batch<32><<<grid, 256, lds, s>>>(f->d_views, b->d_items, b->nitems, b->d_counter, b->q, b->out, b->lo, b->hi, b->first, b->count); break;
-        case 64: k_forest_batch<64><<<grid, 256, lds, s>>>(f->d_views, b->d_items, b->nitems, b->d_counter, b->q, b->out, b->lo, b->hi, b->first, b->count); break;
+        case 16: with_lanes(IC<16>{}); break;
+        case 32: with_lanes(IC<32>{}); break;
+        case 64: with_lanes(IC<64>{}); break;
     }
     BPT_HIP(hipGetLastError());
     return BPT_OK;
@@ -515,6 +555,7 @@ bpt_status bpt_forest_autotune(uint32_t ntrees, const uint64_t *d_keys, const ui
     const uint32_t fanouts[3] = {16, 32, 64};
     const uint32_t lds_opts[3] = {16384, 32768, 65536};
     const uint32_t chunks[3] = {2048, 8192, 32768};
+    const uint32_t lanes_opts[2] = {8, 4};  // 8 or 16 queries per wavefront
     std::string js = "[";
     double best_ms = 1e30;
     hipEvent_t a, z;
@@ -523,36 +564,42 @@ bpt_status bpt_forest_autotune(uint32_t ntrees, const uint64_t *d_keys, const ui
     bpt_status st = BPT_OK;
     for (uint32_t F : fanouts) {
         for (uint32_t L : lds_opts) {
-            bpt_batch_config c{F, 8, L, 0, 0};
+            bpt_batch_config c{F, 8, L, 0, 0, 8};
             bpt_forest *f = nullptr;
             st = bpt_forest_build(&c, ntrees, d_keys, d_vals, h_key_off, &f, nullptr);
             if (st != BPT_OK) break;
-            for (uint32_t ch : chunks) {
-                bpt_batch *b = nullptr;
-                st = bpt_batch_prepare(f, ch, 8, d_q, h_q_off, d_out, d_lo, d_hi, h_r_off, d_first, d_count, &b);
-                if (st != BPT_OK) break;
-                std::vector<float> ms;
-                for (int r = 0; r <= reps && st == BPT_OK; ++r) {
-                    (void)hipEventRecord(a, nullptr);
-                    st = bpt_batch_run(b, 1, nullptr);
-                    (void)hipEventRecord(z, nullptr);
-                    (void)hipEventSynchronize(z);
-                    float x = 0;
-                    (void)hipEventElapsedTime(&x, a, z);
-                    if (r > 0) ms.push_back(x);  // the first run is a warm-up
+            for (uint32_t G : lanes_opts) {
+                f->lanes = G;
+                for (uint32_t ch : chunks) {
+                    bpt_batch *b = nullptr;
+                    st = bpt_batch_prepare(f, ch, 8, d_q, h_q_off, d_out, d_lo, d_hi, h_r_off, d_first, d_count, &b);
+                    if (st != BPT_OK) break;
+                    std::vector<float> ms;
+                    for (int r = 0; r <= reps && st == BPT_OK; ++r) {
+                        (void)hipEventRecord(a, nullptr);
+                        st = bpt_batch_run(b, 1, nullptr);
+                        (void)hipEventRecord(z, nullptr);
+                        (void)hipEventSynchronize(z);
+                        float x = 0;
+                        (void)hipEventElapsedTime(&x, a, z);
+                        if (r > 0) ms.push_back(x);  // the first run is a warm-up
+                    }
+                    bpt_batch_free(b);
+                    if (st != BPT_OK) break;
+                    std::sort(ms.begin(), ms.end());
+                    const double med = ms.empty() ? 0 : ms[ms.size() / 2];
+                    char buf[160];
+                    snprintf(buf, sizeof buf,
+                             "%s{\"fanout\":%u,\"lds_bytes\":%u,\"lanes\":%u,\"queries_per_wave\":%u,\"chunk\":%u,"
+                             "\"ms\":%.4f}",
+                             js.size() > 1 ? "," : "", F, L, G, 64 / G, ch, med);
+                    js += buf;
+                    if (med < best_ms) {
+                        best_ms = med;
+                        *best = bpt_batch_config{F, 8, L, ch, 8, G};
+                    }
                 }
-                bpt_batch_free(b);
                 if (st != BPT_OK) break;
-                std::sort(ms.begin(), ms.end());
-                const double med = ms.empty() ? 0 : ms[ms.size() / 2];
-                char buf[160];
-                snprintf(buf, sizeof buf, "%s{\"fanout\":%u,\"lds_bytes\":%u,\"chunk\":%u,\"ms\":%.4f}",
-                         js.size() > 1 ? "," : "", F, L, ch, med);
-                js += buf;
-                if (med < best_ms) {
-                    best_ms = med;
-                    *best = bpt_batch_config{F, 8, L, ch, 8};
-                }
             }
             bpt_forest_free(f);
             if (st != BPT_OK) break;
@@ -573,30 +620,94 @@ bpt_status bpt_synth_forest(uint64_t seed, uint32_t ntrees, const uint64_t *h_ke
                             uint32_t hit_pct, const uint64_t *h_r_off, uint32_t w_lo_exp, uint32_t w_hi_exp,
                             uint64_t *d_keys, uint64_t *d_vals, uint64_t *d_q, uint64_t *d_lo, uint64_t *d_hi,
                             bpt_stream stream) {
-    BPT_CHECK(h_key_off && ntrees, BPT_ERR_INVALID, "NULL argument");
-    BPT_CHECK(w_lo_exp < w_hi_exp && w_hi_exp < 63, BPT_ERR_INVALID, "width exponents");
+    BPT_CHECK(h_key_off && ntrees && d_keys && d_vals, BPT_ERR_INVALID, "NULL argument");
+    BPT_CHECK(w_lo_exp <= w_hi_exp && w_hi_exp < 63, BPT_ERR_INVALID, "width exponents");
     hipStream_t s = as_stream(stream);
     std::vector<uint64_t> koff(h_key_off, h_key_off + ntrees + 1);
     BPT_CHECK(koff[0] == 0, BPT_ERR_INVALID, "h_key_off[0] must be 0");
     for (uint32_t t = 0; t < ntrees; ++t) BPT_CHECK(koff[t + 1] > koff[t], BPT_ERR_INVALID, "empty tree %u", t);
+    const uint64_t total = koff[ntrees];
+    BPT_CHECK(total < (1ull << 32), BPT_ERR_CAPACITY, "forest of more than 2^32 keys");
+    uint32_t tbits = 1;
+    while ((1ull << tbits) < ntrees) ++tbits;
+    // scratch: the key sort's workspace, later the tree sort's
+    const uint64_t ws_bytes = std::max(sort_ws_bytes(total, 8), sort_ws_bytes(total, 4));
     uint64_t *dk = nullptr, *dr = nullptr;
+    void *ws = nullptr;
+    uint32_t *dup = nullptr;
     BPT_TRY(upload(koff, &dk));
-    const uint64_t total = koff[ntrees] - koff[0];
-    k_synth_keys<<<grid_for(total), 256, 0, s>>>(seed, dk, ntrees, total, d_keys, d_vals);
-    if (d_q && qpk) k_synth_points<<<grid_for(total * qpk), 256, 0, s>>>(seed, dk, ntrees, qpk, hit_pct, total * qpk, d_q);
     bpt_status st = BPT_OK;
-    if (h_r_off && d_lo && d_hi) {
+    hipError_t e = hipMalloc(&ws, ws_bytes);
+    if (e == hipSuccess) e = hipMalloc(&dup, ntrees * 4ull);
+    if (e != hipSuccess) st = hip_fail(e, "hipMalloc(synthetic forest scratch)", __FILE__, __LINE__);
+    // 1. uniform draws, 2. sorted by key (LSD, stable), 3. stably by tree:
+    // each tree's keys end sorted in its own range
+    if (st == BPT_OK) {
+        k_synth_raw<<<grid_for(total), 256, 0, s>>>(seed, dk, ntrees, total, d_keys);
+        uint64_t *ks = nullptr;
+        uint32_t *perm = nullptr;
+        st = sort_pairs<uint64_t>(d_keys, nullptr, total, 0, 64, ws, ws_bytes, &ks, &perm, s);
+        if (st == BPT_OK) {
+            e = hipMemcpyAsync(d_keys, ks, total * 8, hipMemcpyDeviceToDevice, s);
+            if (e != hipSuccess) st = hip_fail(e, "synthetic keys copy", __FILE__, __LINE__);
+        }
+        // perm lives in the first workspace: tree ids go to d_vals (free so far)
+        uint32_t *tree = reinterpret_cast<uint32_t *>(d_vals);
+        if (st == BPT_OK) k_tree_of<<<grid_for(total), 256, 0, s>>>(perm, dk, ntrees, total, tree);
+        uint32_t *ts = nullptr, *perm2 = nullptr;
+        if (st == BPT_OK) st = sort_pairs<uint32_t>(tree, nullptr, total, 0, tbits, ws, ws_bytes, &ts, &perm2, s);
+        if (st == BPT_OK) {
+            k_gather_u64<<<grid_for(total), 256, 0, s>>>(d_keys, perm2, total, d_vals);
+            e = hipMemcpyAsync(d_keys, d_vals, total * 8, hipMemcpyDeviceToDevice, s);
+            if (e != hipSuccess) st = hip_fail(e, "synthetic keys copy", __FILE__, __LINE__);
+        }
+    }
+    // values = rank; trees whose draws collided are redrawn (a collision
+    // among 2^20 uniform 64-bit draws has probability about 2^-25)
+    std::vector<uint32_t> hdup(ntrees);
+    for (uint32_t attempt = 1; st == BPT_OK; ++attempt) {
+        e = hipMemsetAsync(dup, 0, ntrees * 4ull, s);
+        if (e == hipSuccess) {
+            k_rank_and_dups<<<grid_for(total), 256, 0, s>>>(d_keys, dk, ntrees, total, d_vals, dup);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipMemcpyAsync(hdup.data(), dup, ntrees * 4ull, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) { st = hip_fail(e, "synthetic forest", __FILE__, __LINE__); break; }
+        bool any = false;
+        for (uint32_t t = 0; t < ntrees && st == BPT_OK; ++t) {
+            if (!hdup[t]) continue;
+            any = true;
+            if (attempt > 8) { set_error("tree %u: keys collide after 8 redraws", t); st = BPT_ERR_INVALID; break; }
+            const uint64_t n = koff[t + 1] - koff[t];
+            uint64_t *seg = d_keys + koff[t], *ks = nullptr;
+            uint32_t *perm = nullptr;
+            k_synth_redraw<<<grid_for(n), 256, 0, s>>>(seed, t, attempt, n, seg);
+            st = sort_pairs<uint64_t>(seg, nullptr, n, 0, 64, ws, ws_bytes, &ks, &perm, s);
+            if (st == BPT_OK) {
+                e = hipMemcpyAsync(seg, ks, n * 8, hipMemcpyDeviceToDevice, s);
+                if (e != hipSuccess) st = hip_fail(e, "redraw copy", __FILE__, __LINE__);
+            }
+        }
+        if (!any) break;
+    }
+    if (st == BPT_OK && d_q && qpk)
+        k_synth_points<<<grid_for(total * qpk), 256, 0, s>>>(seed, d_keys, dk, ntrees, qpk, hit_pct, total * qpk, d_q);
+    if (st == BPT_OK && h_r_off && d_lo && d_hi) {
         std::vector<uint64_t> roff(h_r_off, h_r_off + ntrees + 1);
         st = upload(roff, &dr);
         const uint64_t nr = roff[ntrees] - roff[0];
         if (st == BPT_OK && nr)
-            k_synth_ranges<<<grid_for(nr), 256, 0, s>>>(seed, dk, dr, ntrees, w_lo_exp, w_hi_exp, nr, d_lo, d_hi);
+            k_synth_ranges<<<grid_for(nr), 256, 0, s>>>(seed, d_keys, dk, dr, ntrees, w_lo_exp, w_hi_exp, nr, d_lo,
+                                                        d_hi);
     }
-    hipError_t e = hipGetLastError();
+    e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (st == BPT_OK && e != hipSuccess) st = hip_fail(e, "synthetic forest", __FILE__, __LINE__);
     (void)hipFree(dk);
     (void)hipFree(dr);
+    (void)hipFree(ws);
+    (void)hipFree(dup);
     return st;
 }
 

@@ -172,10 +172,14 @@ def hmix(seed, stream, tree, i):
 
 
 def synth_tree_keys(seed, tree, n):
-    stride = np.uint64((2 ** 64 - 1) // n)
+    """Keys of one tree: n draws uniform over [0, 2^64), sorted; a tree whose
+    draws collide is redrawn from stream 1 + 16 * attempt."""
     i = np.arange(n, dtype=np.uint64)
-    with np.errstate(over="ignore"):
-        return i * stride + hmix(seed, 1, tree, i) % stride
+    for attempt in range(9):
+        k = np.sort(hmix(seed, 1 + 16 * attempt, tree, i))
+        if n < 2 or (k[1:] != k[:-1]).all():
+            return k
+    raise RuntimeError("keys collide after 8 redraws")
 
 
 def forest_sizes(ntrees, lo_exp=10, hi_exp=20, seed=0, workload=4):

@@ -375,6 +375,8 @@ typedef struct bpt_batch_config {
     uint32_t lds_bytes;     /* LDS budget for each tree's top levels (0: 32 KiB) */
     uint32_t chunk;         /* queries per work item (0: 8192) */
     uint32_t blocks_per_cu; /* persistent grid (0: 8) */
+    uint32_t lanes;         /* lanes per query of the descent: 8 (a wavefront descends 8
+                               queries at a time) or 4 (16 at a time); 0: 8 */
 } bpt_batch_config;
 bpt_status bpt_forest_build(const bpt_batch_config *cfg, uint32_t ntrees, const uint64_t *d_keys,
                             const uint64_t *d_vals, const uint64_t *h_key_off, bpt_forest **out,
@@ -396,8 +398,8 @@ bpt_status bpt_batch_prepare(bpt_forest *f, uint32_t chunk, uint32_t blocks_per_
 bpt_status bpt_batch_run(bpt_batch *b, int use_graph, bpt_stream s);
 bpt_status bpt_batch_free(bpt_batch *b);
 bpt_status bpt_batch_info(const bpt_batch *b, uint32_t *nitems, uint64_t *npoints, uint64_t *nranges);
-/* Times fanout {16,32,64} x LDS budget {16,32,64} KiB x chunk {2K,8K,32K}
- * on the given batch (reps timed runs after one warm-up each) and returns
+/* Times fanout {16,32,64} x LDS budget {16,32,64} KiB x queries per
+ * wavefront {8,16} (lanes 8 or 4) x chunk {2K,8K,32K} on the given batch (reps timed runs after one warm-up each) and returns
  * the fastest configuration; `report` receives a JSON array of all timings. */
 bpt_status bpt_forest_autotune(uint32_t ntrees, const uint64_t *d_keys, const uint64_t *d_vals,
                                const uint64_t *h_key_off, const uint64_t *d_q, const uint64_t *h_q_off,
@@ -405,11 +407,13 @@ bpt_status bpt_forest_autotune(uint32_t ntrees, const uint64_t *d_keys, const ui
                                const uint64_t *h_r_off, uint64_t *d_first, uint64_t *d_count, int reps,
                                bpt_batch_config *best, char *report, uint64_t report_len);
 /* Synthetic forest workload from a counter-based generator (splitmix64):
- * tree t's key i = i*stride + h(seed,1,t,i) mod stride with
- * stride = (2^64-1) / n_t (sorted, unique, uniform per stride), value = i;
- * qpk point queries per key, hit_pct% of them keys of the tree; range j of
- * tree t starts at a uniform rank and spans 2^e + (r mod 2^e) ranks,
- * e uniform in [w_lo_exp, w_hi_exp). h_key_off[0] must be 0. Synchronizes s. */
+ * tree t's keys are n_t draws h(seed,1,t,i) uniform over [0, 2^64), sorted
+ * on the device (a tree whose draws collide is redrawn from stream
+ * 1 + 16 * attempt), value = rank; qpk point queries per key, hit_pct% of
+ * them keys of the tree at a uniform rank, the rest uniform 64-bit values;
+ * range j of tree t starts at a uniform rank and spans floor(2^u) ranks,
+ * u uniform in [w_lo_exp, w_hi_exp] (log-uniform), clipped to the tree.
+ * h_key_off[0] must be 0. Synchronizes s. */
 bpt_status bpt_synth_forest(uint64_t seed, uint32_t ntrees, const uint64_t *h_key_off, uint32_t qpk,
                             uint32_t hit_pct, const uint64_t *h_r_off, uint32_t w_lo_exp, uint32_t w_hi_exp,
                             uint64_t *d_keys, uint64_t *d_vals, uint64_t *d_q, uint64_t *d_lo,

@@ -1,7 +1,11 @@
 /* Compatibility header: the reference's grouping.h (grouping.h:6-28).
  * The partitions run on the device; group lists follow the reference's
- * host bookkeeping (grouping.cu:855-872, 1333-1352, 1746-1791). Within a
- * group the order is by key (the reference's is arbitrary). */
+ * pass loop and host bookkeeping (grouping.cu:776-884, 1288-1363,
+ * 1746-1791). Within a group the order is a stable sort's (the
+ * reference's is arbitrary). The caller's groupList / groupPrefix are
+ * replaced. Not declared here: radixGroupGPU (its declaration,
+ * grouping.h:6-7, does not match its definition) and the __global__
+ * kernels of grouping.h:9-14. */
 #ifndef __INCLUDED_GROUPING_H__
 #define __INCLUDED_GROUPING_H__
 

@@ -38,12 +38,50 @@ def test_generator_matches_restatement(bpt, wl):
     assert np.array_equal(vals[key_off[5]:key_off[6]], np.arange(int(sizes[5]), dtype=np.uint64))
 
 
-@pytest.mark.parametrize("fanout", [16, 32, 64])
-@pytest.mark.parametrize("lds", [16384, 65536])
-def test_batch_matches_oracle(bpt, oracle, wl, fanout, lds):
+def test_generator_distributions(bpt, wl):
+    """Keys uniform over [0, 2^64), unique and sorted per tree; 80 % of the
+    point queries are keys of their tree; range widths log-uniform in
+    [2^4, 2^16] ranks (uniform log2 of the unclipped counts)."""
+    sizes = np.full(8, 1 << 20, dtype=np.uint64)
+    key_off, r_off = wl.forest_offsets(sizes, ranges_div=4)
+    dk, dv, dq, dlo, dhi = bpt.synth_forest(3, key_off, qpk=4, hit_pct=80, r_off=r_off)
+    keys, q, lo, hi = dk.to_host(), dq.to_host(), dlo.to_host(), dhi.to_host()
+    u = []
+    hits = 0
+    logw = []
+    for t in range(8):
+        k = keys[key_off[t]:key_off[t + 1]]
+        assert (k[1:] > k[:-1]).all()
+        x = k.astype(np.float64) / 2.0 ** 64
+        # sorted uniform draws: the i-th lies near (i + 1) / (n + 1) (max
+        # deviation of the empirical CDF, Kolmogorov-Smirnov: above
+        # 2.5 / sqrt(n) with probability about 2 exp(-12.5))
+        n = len(k)
+        assert np.max(np.abs(x - (np.arange(n) + 1) / (n + 1))) < 2.5 / np.sqrt(n)
+        u.append(x.mean())
+        tq = q[4 * key_off[t]:4 * key_off[t + 1]]
+        hits += int(np.isin(tq, k).sum())
+        a = np.searchsorted(k, lo[r_off[t]:r_off[t + 1]])
+        b = np.searchsorted(k, hi[r_off[t]:r_off[t + 1]])
+        w = (b - a + 1).astype(np.float64)
+        logw.append(np.log2(w[a + w < n]))        # ranges not clipped by the tree's end
+    assert abs(np.mean(u) - 0.5) < 0.01
+    assert 0.79 < hits / (4 * int(key_off[-1])) < 0.81
+    lw = np.concatenate(logw)
+    assert lw.min() >= 4 and lw.max() <= 16
+    # log2 of floor(2^u) lies within 1 of u, so the deciles of a uniform u on
+    # [4, 16] are 5.2, 6.4, ... within about a unit
+    for qt, want in zip(np.quantile(lw, [0.1, 0.5, 0.9]), (5.2, 10.0, 14.8)):
+        assert abs(qt - want) < 0.6, (qt, want)
+
+
+@pytest.mark.parametrize("fanout,lds,lanes", [(16, 16384, 8), (32, 16384, 8), (64, 16384, 8), (16, 65536, 8),
+                                              (32, 65536, 8), (64, 65536, 8), (16, 32768, 4), (32, 16384, 4),
+                                              (64, 65536, 4)])
+def test_batch_matches_oracle(bpt, oracle, wl, fanout, lds, lanes):
     sizes, key_off, r_off, q_off, dk, dv, dq, dlo, dhi = _setup(bpt, wl)
     nq, nr = int(q_off[-1]), int(r_off[-1])
-    f = bpt.Forest(dk, dv, key_off, fanout=fanout, lds_bytes=lds)
+    f = bpt.Forest(dk, dv, key_off, fanout=fanout, lds_bytes=lds, lanes=lanes)
     dout = bpt.DeviceArray(nq, np.uint64)
     dfirst = bpt.DeviceArray(nr, np.uint64)
     dcount = bpt.DeviceArray(nr, np.uint64)
@@ -73,9 +111,10 @@ def test_autotune_returns_fastest_valid_config(bpt, oracle, wl):
     dfirst = bpt.DeviceArray(nr, np.uint64)
     dcount = bpt.DeviceArray(nr, np.uint64)
     best, report = bpt.forest_autotune(dk, dv, key_off, dq, q_off, dout, dlo, dhi, r_off, dfirst, dcount, reps=1)
-    assert len(report) == 27
-    assert best["fanout"] in (16, 32, 64) and min(r["ms"] for r in report) > 0
-    f = bpt.Forest(dk, dv, key_off, fanout=best["fanout"], lds_bytes=best["lds_bytes"])
+    assert len(report) == 54
+    assert {r["queries_per_wave"] for r in report} == {8, 16}
+    assert best["fanout"] in (16, 32, 64) and best["lanes"] in (4, 8) and min(r["ms"] for r in report) > 0
+    f = bpt.Forest(dk, dv, key_off, fanout=best["fanout"], lds_bytes=best["lds_bytes"], lanes=best["lanes"])
     b = bpt.Batch(f, dq, q_off, dout, dlo, dhi, r_off, dfirst, dcount, chunk=best["chunk"])
     b.run(graph=True)
     bpt.sync()
@@ -89,9 +128,10 @@ def test_autotune_returns_fastest_valid_config(bpt, oracle, wl):
 
 def test_workload_c4_full_size(bpt, oracle, wl):
     """The benchmarked forest: 4096 uint64 trees of 2^10..2^20 keys (632M in
-    all), 4 point queries per key and size/16 ranges per tree, one
-    graph-captured launch; 32 trees (the smallest, the largest and a spread)
-    against the oracle on their slices of the device inputs."""
+    all), keys uniform over [0, 2^64), 4 point queries per key (80 % hits)
+    and size/16 ranges of log-uniform width per tree, one graph-captured
+    launch; every tree's points and ranges against the oracle, its slices of
+    the device inputs streamed to the host a group of trees at a time."""
     sizes = wl.forest_sizes(4096, 10, 20)
     key_off, r_off = wl.forest_offsets(sizes)
     dk, dv, dq, dlo, dhi = bpt.synth_forest(0, key_off, qpk=4, hit_pct=80, r_off=r_off)
@@ -107,20 +147,33 @@ def test_workload_c4_full_size(bpt, oracle, wl):
     b = bpt.Batch(f, dq, q_off, dout, dlo, dhi, r_off, dfirst, dcount, chunk=2048)
     b.run(graph=True)
     bpt.sync()
-    rng = np.random.default_rng(11)
-    trees = sorted({int(np.argmin(sizes)), int(np.argmax(sizes)), *rng.choice(len(sizes), 30, replace=False).tolist()})
-    hits = []
-    for t in trees:
-        k0, k1 = int(key_off[t]), int(key_off[t + 1])
-        q0, q1 = int(q_off[t]), int(q_off[t + 1])
-        r0, r1 = int(r_off[t]), int(r_off[t + 1])
-        k, v = dk.to_host(offset=k0, count=k1 - k0), dv.to_host(offset=k0, count=k1 - k0)
-        q = dq.to_host(offset=q0, count=q1 - q0)
-        want = oracle.point_closed(k, v, q, SENT)
-        assert np.array_equal(dout.to_host(offset=q0, count=q1 - q0), want), t
-        fe, ce = oracle.range_closed(k, dlo.to_host(offset=r0, count=r1 - r0), dhi.to_host(offset=r0, count=r1 - r0))
-        assert np.array_equal(dfirst.to_host(offset=r0, count=r1 - r0), fe), t
-        assert np.array_equal(dcount.to_host(offset=r0, count=r1 - r0), ce), t
-        hits.append(np.mean(want != SENT))
-    assert 0.75 < float(np.mean(hits)) < 0.85
     b.close()
+    f.close()
+    hits = 0
+    t0 = 0
+    ko, qo, ro = key_off.astype(np.int64), q_off.astype(np.int64), r_off.astype(np.int64)
+    while t0 < 4096:
+        t1 = t0 + 1
+        while t1 < 4096 and ko[t1 + 1] - ko[t0] <= (1 << 24):
+            t1 += 1
+        k = dk.to_host(offset=int(ko[t0]), count=int(ko[t1] - ko[t0]))
+        v = dv.to_host(offset=int(ko[t0]), count=int(ko[t1] - ko[t0]))
+        q = dq.to_host(offset=int(qo[t0]), count=int(qo[t1] - qo[t0]))
+        out = dout.to_host(offset=int(qo[t0]), count=int(qo[t1] - qo[t0]))
+        lo = dlo.to_host(offset=int(ro[t0]), count=int(ro[t1] - ro[t0]))
+        hi = dhi.to_host(offset=int(ro[t0]), count=int(ro[t1] - ro[t0]))
+        first = dfirst.to_host(offset=int(ro[t0]), count=int(ro[t1] - ro[t0]))
+        count = dcount.to_host(offset=int(ro[t0]), count=int(ro[t1] - ro[t0]))
+        for t in range(t0, t1):
+            ks = slice(ko[t] - ko[t0], ko[t + 1] - ko[t0])
+            qs = slice(qo[t] - qo[t0], qo[t + 1] - qo[t0])
+            rs = slice(ro[t] - ro[t0], ro[t + 1] - ro[t0])
+            want = oracle.point_closed(k[ks], v[ks], q[qs], SENT)
+            assert np.array_equal(out[qs], want), t
+            fe, ce = oracle.range_closed(k[ks], lo[rs], hi[rs])
+            assert np.array_equal(first[rs], fe), t
+            assert np.array_equal(count[rs], ce), t
+            hits += int((want != SENT).sum())
+        t0 = t1
+    print(f"c4 checked: all 4096 trees, {nq} points, {nr} ranges", flush=True)
+    assert 0.79 < hits / nq < 0.81

@@ -92,7 +92,7 @@ def test_learned_requires_model(bpt):
 def test_workload_c5_full_size(bpt, oracle, wl):
     """256M unique lognormal uint64 keys, the learned model trained as in the
     benchmark, 2^27 queries (90 % hits): learned equals plain on every query,
-    and plain equals the oracle on a 2^20-query sample."""
+    and both equal the oracle on every query."""
     n, nq = 1 << 28, 1 << 27
     keys = wl.lognormal_keys(n, sort=lambda a: bpt.sort_pairs(a)[0])
     vals = np.arange(n, dtype=np.uint64)
@@ -110,8 +110,8 @@ def test_workload_c5_full_size(bpt, oracle, wl):
     t.point_search_dev(dq, out, bpt.LEARNED)
     learned = out.to_host()
     assert int((plain != learned).sum()) == 0
-    s = q[: 1 << 20]
-    assert np.array_equal(plain[: 1 << 20], oracle.point_closed(keys, vals, s, (1 << 64) - 1))
+    assert np.array_equal(plain, oracle.point_closed(keys, vals, q, (1 << 64) - 1))
+    print(f"c5 checked: all {nq} queries", flush=True)
     dq.free()
     out.free()
     t.close()

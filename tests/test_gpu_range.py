@@ -154,7 +154,11 @@ def test_grouping_keys_match_reference_functions(bpt, oracle):
     rng = np.random.default_rng(10)
     a = rng.integers(0, 1 << 64, size=3000, dtype=np.uint64)
     b = rng.integers(0, 1 << 40, size=3000, dtype=np.uint64)
+    # max(a) >= 2^63: the reference's INT_MIN interval; max(b) < 2^63: interval 256
     assert np.array_equal(bpt.zorder2d(a, b), oracle.zorder_keys(a, b))
+    assert np.array_equal(bpt.zorder2d(b, a), oracle.zorder_keys(b, a))
+    a2 = rng.integers(0, 1 << 62, size=3000, dtype=np.uint64)    # both below 2^63
+    assert np.array_equal(bpt.zorder2d(a2, b), oracle.zorder_keys(a2, b))
     g = bpt.group2d_keys(a, b)
     p0 = max(0, oracle.group2d_start_pass(int(a.max()), int(b.max())))
     for i in range(0, 3000, 97):
@@ -233,7 +237,8 @@ def test_exclusive_scan(bpt):
 
 def test_workload_c3_full_size(bpt, oracle, wl):
     """32M Morton-keyed points, 4M rectangles: the three query orders return
-    identical counts and ids, and both equal the oracle on a sample."""
+    identical counts and ids, and the counts and ids of every rectangle equal
+    the oracle's (Morton-range scan with a rectangle filter)."""
     npts, nr = 1 << 25, 1 << 22
     x, y, ids, keys, kid = wl.points_2d(npts)
     r = wl.rects(nr)
@@ -243,19 +248,19 @@ def test_workload_c3_full_size(bpt, oracle, wl):
     for mode in MODES[1:]:
         c, o, got = t.rect_search(r, group_mode=mode)
         assert np.array_equal(c, c0) and np.array_equal(got, i0), mode
-    sample = 2048
-    want_c, want_o, want_ids = oracle.rect_morton(keys, kid, r[:sample], threads=8)
-    assert np.array_equal(c0[:sample], want_c)
-    end = int(o0[sample - 1] + c0[sample - 1])
-    assert np.array_equal(i0[:end], want_ids)
     t.close()
+    want_c, want_o, want_ids = oracle.rect_morton(keys, kid, r)
+    assert np.array_equal(c0, want_c)
+    assert np.array_equal(o0, want_o)
+    assert np.array_equal(i0, want_ids)
+    print(f"c3 checked: all {nr} rectangles, {len(want_ids)} ids", flush=True)
 
 
 def test_workload_ranges_full_size(bpt, oracle, wl):
     """The 1-D range workload of the benchmark: 64M uint32 keys (value = rank)
     and 4M inclusive ranges of 1..1024 ranks. Counts and lower bounds equal
-    the oracle for every range in every order; the ids, consecutive ranks
-    here, are checked exactly in 16 windows of 2^20 ids."""
+    the oracle for every range in every order; every id equals the oracle's
+    (values are ranks, so range i's ids are first[i], first[i] + 1, ...)."""
     n, nr = 1 << 26, 1 << 22
     keys, vals = wl.point_keys(n, workload=2)
     rng = np.random.default_rng(11)
@@ -272,10 +277,15 @@ def test_workload_ranges_full_size(bpt, oracle, wl):
         if mode == 0:
             total = int(cnt.sum())
             assert len(ids) == total
-            for s in np.linspace(0, total - (1 << 20), 16).astype(np.int64):
-                pos = np.arange(s, s + (1 << 20), dtype=np.int64)
-                i = np.searchsorted(offs.astype(np.int64), pos, side="right") - 1
-                want = (first.astype(np.int64)[i] + (pos - offs.astype(np.int64)[i])).astype(np.uint32)
-                assert np.array_equal(ids[pos], want), int(s)
+            assert np.array_equal(offs, np.concatenate([[0], np.cumsum(want_cnt)[:-1]]).astype(np.uint64))
+            wv = vals.astype(np.int64)
+            for r0 in range(0, nr, 1 << 18):          # all ranges, a block of 2^18 at a time
+                r1 = min(nr, r0 + (1 << 18))
+                c = want_cnt[r0:r1].astype(np.int64)
+                o = offs[r0:r1].astype(np.int64)
+                pos = np.arange(int(o[0]), int(o[-1] + c[-1]), dtype=np.int64)
+                start = np.repeat(want_first[r0:r1].astype(np.int64), c)
+                want = wv[start + (pos - np.repeat(o, c))].astype(np.uint32)
+                assert np.array_equal(ids[pos], want), r0
             del ids
     t.close()
