@@ -5,8 +5,13 @@ Workload: 64M (2^26) unique uint32 keys uniform over [0, 2^32), value = rank,
 bulk-built into the flat gfx950 tree (fanout 32, 16-slot leaves); one step =
 one plain point-search pass over 256M (2^28) queries drawn uniformly from the
 key set, with queries, tree and results resident in HBM. Also measured
-beside it (not the headline value): the radix-grouped form and the Zipf(0.99)
-query set with 10% misses.
+beside it (not the headline value), on the same tree: the radix-grouped and
+learned forms on the uniform and the Zipf(0.99) (10% misses) query sets, 1-D
+ranges in the three query orders, the serving loop's p50/p99 latency at 0.5x
+and 0.8x of its saturation, and batched upserts; then, at their full sizes,
+the 2-D rectangle (32M points, 4M rectangles), forest (4096 trees) and
+learned-index + serving (256M lognormal keys) configurations, summarised
+under "configs" (each also runs alone with --config c3 / c4 / c5).
 
 Prints ONE JSON line. `roofline.achieved` uses the algorithmic bytes of a
 lookup (4 B query + 4 B result + one 128 B leaf line = 136 B) over the plain
@@ -60,6 +65,17 @@ def time_form(tree, dq, dout, mode, steps, warmup):
     return wall, ev
 
 
+def host_info(threads):
+    cpu_model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            cpu_model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
+    return {"threads_used": threads, "nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)),
+            "cpu_model": cpu_model}
+
+
 def cpu_baseline(keys, vals, q, gpu_out, target_s=10.0, zipf=None):
     """Oracle page tree (reference semantics, PAGESIZE 5000) on host cores."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -97,6 +113,7 @@ def cpu_baseline(keys, vals, q, gpu_out, target_s=10.0, zipf=None):
     pt.close()
     return {
         "value": s / dt, "unit": "lookups/s", "cores": threads, "kind": "port",
+        "host": host_info(threads),
         "sample": f"first {s} of the {len(q)} uniform-hit queries against the full 64M-key page "
                   f"tree (PAGESIZE 5000, reference default), OpenMP static schedule, "
                   f"{dt:.1f} s; page-tree build {build_s:.1f} s excluded",
@@ -179,7 +196,10 @@ def bench_c3(args):
                        {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
                         "traffic": None, "note": "reported, not gated: bytes depend on the decomposition"},
                        extra)
-    print(json.dumps(line), flush=True)
+    tree.close()
+    for a in (dr, dcount, doff):
+        a.free()
+    return line
 
 
 def bench_c1(args):
@@ -239,7 +259,7 @@ def bench_c1(args):
     dq.free()
     dout.free()
     tree.close()
-    print(json.dumps(line), flush=True)
+    return line
 
 
 def bench_c4(args):
@@ -260,13 +280,13 @@ def bench_c4(args):
     dcount = bptree.DeviceArray(nr, np.uint64)
     t0 = time.perf_counter()
     if args.no_autotune:  # the configuration the autotuner picks on MI355X
-        best = {"fanout": 16, "leaf_cap": 8, "lds_bytes": 32768, "chunk": 8192, "blocks_per_cu": 8}
+        best = {"fanout": 16, "leaf_cap": 8, "lds_bytes": 32768, "chunk": 8192, "blocks_per_cu": 8, "lanes": 8}
         report = []
     else:
         best, report = bptree.forest_autotune(dk, dv, key_off, dq, q_off, dout, dlo, dhi, r_off, dfirst, dcount,
                                               reps=1)
     log(f"autotune {time.perf_counter() - t0:.1f} s -> {best}")
-    forest = bptree.Forest(dk, dv, key_off, fanout=best["fanout"], lds_bytes=best["lds_bytes"])
+    forest = bptree.Forest(dk, dv, key_off, fanout=best["fanout"], lds_bytes=best["lds_bytes"], lanes=best["lanes"])
     batch = bptree.Batch(forest, dq, q_off, dout, dlo, dhi, r_off, dfirst, dcount, chunk=best["chunk"])
     for _ in range(args.warmup):
         batch.run(graph=True)
@@ -343,15 +363,77 @@ def bench_c4(args):
     bytes_pts = 144 * nq
     line = common_line("batched point lookups/s @4096 uint64 trees (one persistent hipGraph launch)", nq / wall,
                        "lookups/s", args, wall * 1e3, "u64",
-                       {"workload": "4096 uint64 trees, sizes log-uniform in [1K, 1M] (stratified-uniform keys, "
-                                    "value = rank); 4x size point queries per tree (80% hits) + size/16 range "
-                                    "queries per tree (widths 2^4..2^16 ranks) in the same launch",
+                       {"workload": "4096 uint64 trees, sizes log-uniform in [1K, 1M] (keys uniform in [0, 2^64), "
+                                    "unique, value = rank); 4x size point queries per tree (80% hits) + size/16 "
+                                    "range queries per tree (widths log-uniform in [2^4, 2^16] ranks) in the same "
+                                    "launch",
                         "trees": ntrees, "keys": int(key_off[-1])},
                        {"bound": "hbm", "achieved": bytes_pts / (pms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": bytes_pts / (pms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
                         "kernel": "k_forest_batch (points only)", "kernel_ms": pms, "bytes_per_lookup": 144},
                        extra)
-    print(json.dumps(line), flush=True)
+    batch.close()
+    batch_p.close()
+    forest.close()
+    for a in (dk, dv, dq, dlo, dhi, dout, dfirst, dcount):
+        a.free()
+    return line
+
+
+def serving_run(tree, keys, n, miss_space, nreq, modes, workload):
+    """The serving loop over a resident tree: an open-loop stream of nreq mixed
+    batches (sizes floor(2^u), u uniform in [0, 16]; 90% point batches with
+    10% misses uniform in [0, miss_space), 10% range batches of widths
+    log-uniform in [2^4, 2^12] keys). For each mode: saturation (all requests
+    at once), then Poisson arrivals at 0.5x and 0.8x of it, p50/p99 latency.
+    Every answer is checked against the tree's own search of the same keys
+    in one batch (the serving tests check them against the oracle)."""
+    kdt = keys.dtype.type
+    vdt = np.uint32 if tree.info()["val_bytes"] == 4 else np.uint64
+    rr = workloads.rng_for(0, workload, 2)
+    reqs, keep = [], []
+    for _ in range(nreq):
+        m = int(np.floor(np.exp2(rr.uniform(0, 16))))
+        if rr.random() < 0.9:
+            qq = np.where(rr.random(m) < 0.9, keys[rr.integers(0, n, m)],
+                          rr.integers(0, miss_space, m, dtype=np.uint64).astype(kdt)).astype(kdt)
+            out = np.zeros(m, dtype=vdt)
+            reqs.append(bptree.Server.point_request(qq, out))
+            keep.append(("p", qq, out))
+        else:
+            st_ = rr.integers(0, n, m)
+            w = np.floor(np.exp2(rr.uniform(4, 12, m))).astype(np.int64)
+            lo, hi = keys[st_], keys[np.minimum(st_ + w, n - 1)]
+            first, cnt = np.zeros(m, dtype=np.uint64), np.zeros(m, dtype=np.uint64)
+            reqs.append(bptree.Server.range_request(lo, hi, first, cnt))
+            keep.append(("r", lo, hi, first, cnt))
+    total_lookups = sum(int(r.n) for r in reqs)
+    pq = np.concatenate([k[1] for k in keep if k[0] == "p"])
+    want_p = tree.point_search(pq, bptree.PLAIN)
+    serving = {}
+    for mode in modes:
+        name = {bptree.PLAIN: "plain", bptree.GROUPED: "grouped", bptree.LEARNED: "learned"}[mode]
+        srv = bptree.Server(tree, slots=8, mem_budget=8 << 30, mode=mode)
+        srv.replay(reqs[:200], np.zeros(200))  # warm-up
+        srv.reset_stats()
+        t0 = time.perf_counter()
+        srv.replay(reqs, np.zeros(nreq))
+        sat = nreq / (time.perf_counter() - t0)
+        res = {"saturation_requests_per_s": sat, "saturation": srv.stats()}
+        for frac in (0.5, 0.8):
+            srv.reset_stats()
+            arr = np.cumsum(rr.exponential(1.0 / (frac * sat), size=nreq))
+            srv.replay(reqs, arr)
+            stt = srv.stats()
+            res[f"load_{frac}"] = {"rate_requests_per_s": frac * sat, "p50_ms": stt["p50_ms"], "p99_ms": stt["p99_ms"],
+                                   "lookups_per_s": stt["lookups"] / stt["elapsed_s"] if stt["elapsed_s"] else None,
+                                   "max_inflight": stt["max_inflight"]}
+        got_p = np.concatenate([k[2] for k in keep if k[0] == "p"])
+        res["point_mismatches_vs_one_batch"] = int((got_p != want_p).sum())
+        srv.close()
+        serving[name] = res
+        log(f"serving {name}: " + json.dumps(res))
+    return serving, nreq, total_lookups
 
 
 def bench_c5(args):
@@ -399,46 +481,7 @@ def bench_c5(args):
     dq.free()
     dout.free()
 
-    # serving: mixed batches, sizes log-uniform in [1, 64K], 90% point / 10% range of 2^4..2^12 keys
-    nreq = 3000
-    rr = workloads.rng_for(0, 5, 2)
-    reqs, keep = [], []
-    for i in range(nreq):
-        m = int(np.floor(np.exp2(rr.uniform(0, 16))))
-        if rr.random() < 0.9:
-            qq = np.where(rr.random(m) < 0.9, keys[rr.integers(0, n, m)],
-                          rr.integers(0, 1 << 50, m, dtype=np.uint64)).astype(np.uint64)
-            out = np.zeros(m, dtype=np.uint64)
-            reqs.append(bptree.Server.point_request(qq, out))
-            keep.append((qq, out))
-        else:
-            st_ = rr.integers(0, n, m)
-            w = np.floor(np.exp2(rr.uniform(4, 12, m))).astype(np.int64)
-            lo, hi = keys[st_], keys[np.minimum(st_ + w, n - 1)]
-            first, cnt = np.zeros(m, dtype=np.uint64), np.zeros(m, dtype=np.uint64)
-            reqs.append(bptree.Server.range_request(lo, hi, first, cnt))
-            keep.append((lo, hi, first, cnt))
-    total_lookups = sum(int(r.n) for r in reqs)
-    serving = {}
-    for mode, name in ((bptree.LEARNED, "learned"), (bptree.PLAIN, "plain")):
-        srv = bptree.Server(tree, slots=8, mem_budget=8 << 30, mode=mode)
-        srv.replay(reqs[:200], np.zeros(200))  # warm-up
-        srv.reset_stats()
-        t0 = time.perf_counter()
-        srv.replay(reqs, np.zeros(nreq))
-        sat = nreq / (time.perf_counter() - t0)
-        res = {"saturation_requests_per_s": sat, "saturation": srv.stats()}
-        for frac in (0.5, 0.8):
-            srv.reset_stats()
-            arr = np.cumsum(rr.exponential(1.0 / (frac * sat), size=nreq))
-            srv.replay(reqs, arr)
-            stt = srv.stats()
-            res[f"load_{frac}"] = {"rate_requests_per_s": frac * sat, "p50_ms": stt["p50_ms"], "p99_ms": stt["p99_ms"],
-                                   "lookups_per_s": stt["lookups"] / stt["elapsed_s"] if stt["elapsed_s"] else None,
-                                   "max_inflight": stt["max_inflight"]}
-        srv.close()
-        serving[name] = res
-        log(f"serving {name}: " + json.dumps(res))
+    serving, nreq, total_lookups = serving_run(tree, keys, n, 1 << 50, 3000, (bptree.LEARNED, bptree.PLAIN), 5)
     extra = {"forms": forms, "learned_model": lst, "serving": serving, "serving_requests": nreq,
              "serving_lookups": total_lookups}
     if not args.no_cpu_baseline:
@@ -467,7 +510,8 @@ def bench_c5(args):
                         "traffic": None, "kernel": "k_point_learned", "kernel_ms": lk["kernel_ms"],
                         "bytes_per_lookup": 144},
                        extra)
-    print(json.dumps(line), flush=True)
+    tree.close()
+    return line
 
 
 def main():
@@ -479,6 +523,8 @@ def main():
     ap.add_argument("--queries-log2", type=int, default=28)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-forms", action="store_true", help="skip grouped / Zipf side measurements")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the 2-D rectangle, forest and learned-index configurations after the headline")
     ap.add_argument("--grouped-only", action="store_true",
                     help="time only the grouped form on the uniform set (for profiling its kernels)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -495,14 +541,12 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is "
                                    "designed (range-shard or replicate) but not built"}))
         sys.exit(2)
-    if args.config == "c3":
-        return bench_c3(args)
+    if args.config in ("c3", "c4", "c5"):
+        line = {"c3": bench_c3, "c4": bench_c4, "c5": bench_c5}[args.config](args)
+        print(json.dumps(line), flush=True)
+        return
     if args.config == "c1":
         return bench_c1(args)
-    if args.config == "c4":
-        return bench_c4(args)
-    if args.config == "c5":
-        return bench_c5(args)
 
     n, nq = 1 << args.keys_log2, 1 << args.queries_log2
     dev = bptree.device_info(0)
@@ -590,6 +634,19 @@ def main():
                                     "speedup_vs_plain": wall / wall_g,
                                     "algorithmic_GBps": BYTES_PER_LOOKUP * nq / wall_g / 1e9,
                                     "mismatches_vs_plain": int((dout.to_host() != gpu_out).sum())}
+        # the learned position model on the same tree: MLP 1-32-32-1 root over
+        # 65536 linear leaf models, windows of at most 256 slots, trained on
+        # the GPU; the plain descent answers the queries whose window is wider
+        t0 = time.perf_counter()
+        lst = tree.learned_build(leaf_models=65536, max_window=256, train_samples=1 << 20, train_steps=2000,
+                                 root=0)
+        train_s = time.perf_counter() - t0
+        wall_l, ev_l = time_form(tree, dq, dout, bptree.LEARNED, max(3, args.steps // 2), 1)
+        forms["learned_uniform"] = {"lookups_per_s": nq / wall_l, "ms": wall_l * 1e3,
+                                    "kernel_ms": float(np.mean(ev_l)), "speedup_vs_plain": wall / wall_l,
+                                    "mismatches_vs_plain": int((dout.to_host() != gpu_out).sum()),
+                                    "build_s": train_s, "model": lst}
+        log(f"learned: {wall_l * 1e3:.3f} ms ({wall / wall_l:.2f}x plain), model {lst}")
         qz = workloads.zipf_queries(keys, nq)
         dqz = bptree.DeviceArray.from_host(qz)
         wall_pz, _ = time_form(tree, dqz, dout, bptree.PLAIN, max(3, args.steps // 2), 1)
@@ -599,6 +656,10 @@ def main():
         forms["plain_zipf"] = {"lookups_per_s": nq / wall_pz, "ms": wall_pz * 1e3}
         forms["grouped_zipf"] = {"lookups_per_s": nq / wall_gz, "ms": wall_gz * 1e3,
                                  "speedup_vs_plain": wall_pz / wall_gz,
+                                 "mismatches_vs_plain": int((dout.to_host() != plain_z).sum())}
+        wall_lz, _ = time_form(tree, dqz, dout, bptree.LEARNED, max(3, args.steps // 2), 1)
+        forms["learned_zipf"] = {"lookups_per_s": nq / wall_lz, "ms": wall_lz * 1e3,
+                                 "speedup_vs_plain": wall_pz / wall_lz,
                                  "mismatches_vs_plain": int((dout.to_host() != plain_z).sum())}
         dqz.free()
         del qz
@@ -653,6 +714,9 @@ def main():
                                                                   and np.array_equal(ids_h, base[1]))}
         for a in (dlo, dhi, dfirst, dcnt, doff, ids_buf[0]):
             a.free()
+        # the serving loop over this tree (p50 / p99 latency)
+        serving, nreq, sl = serving_run(tree, keys, n, 1 << 32, 3000, (bptree.PLAIN, bptree.LEARNED), 2)
+        forms["serving"] = {"requests": nreq, "lookups": sl, **serving}
         # batched updates: batches of 2^20 upserts (half of them existing keys)
         # merged into the tree, which is rebuilt; wall time of each whole call.
         # The first call also allocates the merge's workspace and regrows the
@@ -680,7 +744,7 @@ def main():
     tfile = os.path.join(ROOT, "profiles", "point_plain_64M_traffic.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            traffic = json.load(f).get("l2_egress_bytes_per_launch")
 
     line = {
         "metric": METRIC,
@@ -704,6 +768,9 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "traffic_counts": "bytes leaving the L2 (TCC FETCH_SIZE + WRITE_SIZE per launch, scaled by the "
+                              "same counters on a copy kernel of known size); gfx950 has no counter behind the "
+                              "Infinity Cache, so Infinity Cache hits are included: an upper bound on HBM bytes",
             "kernel": "k_point_coop", "kernel_ms": kernel_ms,
             "bytes_per_lookup": BYTES_PER_LOOKUP,
             "stream_copy_GBps": copy_gbps, "frac_of_copy": achieved / copy_gbps,
@@ -713,7 +780,45 @@ def main():
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(keys, vals, q, gpu_out, zipf=zipf_sample)
     tree.close()
+    dq.free()
+    dout.free()
+    if not args.no_configs:
+        # the other configurations at their full sizes, summarised beside the
+        # headline (each also runs alone with --config)
+        del keys, vals, q, gpu_out, zipf_sample
+        line["configs"] = {}
+        for name, fn in (("c3", bench_c3), ("c4", bench_c4), ("c5", bench_c5)):
+            t0 = time.perf_counter()
+            sub = fn(args)
+            line["configs"][name] = summarize(name, sub)
+            line["configs"][name]["bench_s"] = time.perf_counter() - t0
+            log(f"{name}: " + json.dumps(line["configs"][name]))
     print(json.dumps(line), flush=True)
+
+
+def summarize(name, sub):
+    """The headline figures of one configuration's line."""
+    keep = {"metric": sub["metric"], "value": sub["value"], "unit": sub["unit"], "ms_per_step": sub["ms_per_step"],
+            "workload": sub["config"]["workload"]}
+    rf = sub.get("roofline") or {}
+    if rf.get("frac") is not None:
+        keep["roofline"] = {k: rf.get(k) for k in ("achieved", "peak", "unit", "frac", "kernel", "kernel_ms",
+                                                   "bytes_per_lookup")}
+    cb = sub.get("cpu_baseline")
+    if cb:
+        keep["cpu_baseline"] = {k: cb.get(k) for k in ("value", "unit", "cores", "kind", "sample",
+                                                       "mismatches_vs_gpu")}
+    if name == "c3":
+        keep["forms"] = sub["forms"]
+        keep["hits_total"] = sub["hits_total"]
+    elif name == "c4":
+        keep.update({k: sub[k] for k in ("autotune", "speedup_vs_per_tree_launches", "points", "ranges",
+                                        "batch_ms", "points_only_ms")})
+    elif name == "c5":
+        keep["forms"] = {k: {kk: vv for kk, vv in v.items() if kk != "model"} for k, v in sub["forms"].items()}
+        keep["learned_model"] = sub["learned_model"]
+        keep["serving"] = {m: {k: v for k, v in r.items() if k != "saturation"} for m, r in sub["serving"].items()}
+    return keep
 
 
 if __name__ == "__main__":
