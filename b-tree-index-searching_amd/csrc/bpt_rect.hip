@@ -263,9 +263,34 @@ __device__ __forceinline__ void wave_sort_global(uint32_t *p, uint64_t c) {
     }
 }
 
+// The hit count of one rectangle by one wave: lane l walks sub-rectangle
+// (l % 8, l / 8) of an 8 x 8 split (the split of the ids pass), and the
+// lanes' counts are summed. For the last rectangle of an ids pass, which has
+// no next offset to give its count.
+template <int F, int LF>
+__global__ __launch_bounds__(64) void k_rect_count_split(TreeView t, const uint4 *__restrict__ rect,
+                                                         uint64_t *__restrict__ count) {
+    extern __shared__ uint4 lds_raw[];
+    uint8_t *lds = reinterpret_cast<uint8_t *>(lds_raw);
+    stage_top_levels(t, lds);
+    const uint32_t lane = threadIdx.x & 63, kx = lane & 7, ky = lane >> 3;
+    const uint4 r = *rect;
+    const uint64_t w = (uint64_t)r.z - r.x + 1, h = (uint64_t)r.w - r.y + 1;
+    const uint64_t xa = r.x + w * kx / 8, xe = r.x + w * (kx + 1) / 8;
+    const uint64_t ya = r.y + h * ky / 8, ye = r.y + h * (ky + 1) / 8;
+    uint64_t c = 0;
+    if (r.x <= r.z && r.y <= r.w && xa < xe && ya < ye) {
+        const uint4 sub = make_uint4((uint32_t)xa, (uint32_t)ya, (uint32_t)(xe - 1), (uint32_t)(ye - 1));
+        rect_walk<F, LF>(t, lds, sub, [&](uint64_t) { ++c; });
+    }
+    for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, 64);
+    if (lane == 0) *count = c;
+}
+
 // Ids of the rectangles with hits. A rectangle's count is the gap to the
-// next offset (offsets are the exclusive scan of the counts), so rectangles
-// without points are skipped without a walk; the last one is always walked.
+// next offset (offsets are the exclusive scan of the counts), and the last
+// rectangle's, which has no next offset, is *last_count (counted just
+// before), so rectangles without points are skipped without a walk.
 // Each wave takes 64 rectangles at a time: their lanes walk and write the ids
 // in code order. A rectangle with more than kRectSplit ids is walked by the
 // whole wave instead: lane l takes sub-rectangle (l % 8, l / 8) of an 8 x 8
@@ -279,6 +304,7 @@ template <int F, int LF>
 __global__ __launch_bounds__(256) void k_rect_ids(TreeView t, const uint4 *__restrict__ rects,
                                                   const uint32_t *__restrict__ order, uint64_t nr,
                                                   const uint64_t *__restrict__ offsets,
+                                                  const uint64_t *__restrict__ last_count,
                                                   uint32_t *__restrict__ ids) {
     extern __shared__ uint4 lds_raw[];
     uint8_t *lds = reinterpret_cast<uint8_t *>(lds_raw);
@@ -294,8 +320,8 @@ __global__ __launch_bounds__(256) void k_rect_ids(TreeView t, const uint4 *__res
         if (j < nr) {
             i = order ? order[j] : j;
             o = offsets[i];
-            known = i + 1 < nr ? offsets[i + 1] - o : 0;
-            if (i + 1 == nr || (known > 0 && known <= kRectSplit)) {
+            known = i + 1 < nr ? offsets[i + 1] - o : *last_count;
+            if (known > 0 && known <= kRectSplit) {
                 uint32_t *out = ids + o;
                 rect_walk<F, LF>(t, lds, rects[i], [&](uint64_t slot) {
                     out[c++] = slot_val<uint64_t, uint32_t, LF>(t, slot);
@@ -404,12 +430,18 @@ static bpt_status rect_run(bpt_tree *t, const uint32_t *d_rects, uint64_t nr, in
             constexpr int F = decltype(fc)::value, LF = decltype(lc)::value;
             const size_t lds = t->view.lds_bytes ? t->view.lds_bytes : 16;
             if (d_ids) {
+                // the last rectangle's count (it has no next offset), so that a
+                // large one is walked by a whole wave like the others
+                BPT_TRY(t->ws.reserve(256));
+                uint64_t *last = static_cast<uint64_t *>(t->ws.ptr);
+                k_rect_count_split<F, LF><<<1, 64, lds, s>>>(t->view, (const uint4 *)d_rects + (nr - 1), last);
+                BPT_HIP(hipGetLastError());
                 const size_t ids_lds = ((t->view.lds_bytes + 15) & ~(size_t)15) +
                                        (size_t)(t->block / 64) * kRectSortCap * sizeof(uint32_t);
                 uint32_t grid = 0;
                 BPT_TRY(resident_grid(t, k_rect_ids<F, LF>, nr, ids_lds, &grid));
                 k_rect_ids<F, LF><<<grid, t->block, ids_lds, s>>>(t->view, (const uint4 *)d_rects, order, nr,
-                                                                d_offsets, d_ids);
+                                                                d_offsets, last, d_ids);
             } else if (t->coop && t->view.count[0] < (1ull << 32)) {
                 const bool off32 = t->addr32 && t->inner_bytes <= (1ull << 32);
                 auto go = [&](auto lanes, auto o32) -> bpt_status {

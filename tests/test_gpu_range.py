@@ -105,6 +105,25 @@ def test_rect_id_runs_at_sort_limits(bpt):
         assert np.array_equal(got, want_ids), mode
 
 
+def test_rect_last_rectangle_large(bpt, oracle, wl):
+    """The ids pass learns the last rectangle's count separately (it has no
+    next offset): a last rectangle covering the whole domain (every point)
+    or half of it is walked by a whole wave and returns every id, sorted."""
+    x, y, ids, keys, kid = wl.points_2d(1 << 21, seed=4)
+    t = bpt.Tree(keys, kid, 8, 4)
+    r = wl.rects(1000, seed=4, lo_exp=10, hi_exp=24)
+    for last in ([0, 0, 0xFFFFFFFF, 0xFFFFFFFF], [0, 0, 0x7FFFFFFF, 0xFFFFFFFF], [7, 7, 7, 7]):
+        rr = np.concatenate([r, np.array([last], dtype=np.uint32)])
+        want_c, want_o, want_ids = oracle.rect_morton(keys, kid, rr)
+        for mode in MODES:
+            c, o, got = t.rect_search(rr, group_mode=mode)
+            assert np.array_equal(c, want_c), (last, mode)
+            assert np.array_equal(got, want_ids), (last, mode)
+    c, o, got = t.rect_search(np.array([[0, 0, 0xFFFFFFFF, 0xFFFFFFFF]], dtype=np.uint32))
+    assert int(c[0]) == len(keys) and np.array_equal(got, np.sort(kid))
+    t.close()
+
+
 def test_rect_duplicate_points(bpt, oracle, wl):
     rng = np.random.default_rng(9)
     x = rng.integers(0, 64, size=5000, dtype=np.uint64).astype(np.uint32)   # many coincident points

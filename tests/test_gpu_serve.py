@@ -142,3 +142,60 @@ def test_small_point_requests_are_fused(bpt, oracle, served_tree, mode):
         else:
             f, c = oracle.range_closed(keys, r[1], r[2])
             assert np.array_equal(m[1], c) and np.array_equal(m[0], f)
+
+
+@pytest.fixture(scope="module")
+def c5_tree(bpt, wl):
+    """The learned-index configuration's tree: 256M unique lognormal uint64
+    keys, value = rank, with the learned model trained as in the benchmark."""
+    n = 1 << 28
+    keys = wl.lognormal_keys(n, sort=lambda a: bpt.sort_pairs(a)[0])
+    vals = np.arange(n, dtype=np.uint64)
+    print("c5 serving: keys generated", flush=True)
+    t = bpt.Tree(keys, vals, 8, 8)
+    t.learned_build(leaf_models=65536, max_window=256, train_samples=1 << 20, train_steps=2000, root=0)
+    yield t, keys, vals
+    t.close()
+
+
+@pytest.mark.parametrize("mode", [0, 2])
+def test_config_scale_serving_same_alone_and_in_the_mix(bpt, oracle, c5_tree, mode):
+    """At the configuration's scale: batches of floor(2^u) queries, u uniform
+    in [0, 16] (1 to 64K), 90% point (10% misses) / 10% range of 2^4..2^12
+    keys, over the 256M-key tree. Every answer is the same run alone (one
+    slot, one request at a time) and in an open-loop Poisson mix on 8 slots,
+    and equals the oracle."""
+    t, keys, vals = c5_tree
+    rng = np.random.default_rng(30 + mode)
+    reqs = []
+    for _ in range(300):
+        m = int(np.floor(np.exp2(rng.uniform(0, 16))))
+        if rng.random() < 0.9:
+            q = np.where(rng.random(m) < 0.9, keys[rng.integers(0, len(keys), m)],
+                         rng.integers(0, 1 << 50, m, dtype=np.uint64)).astype(np.uint64)
+            reqs.append(("point", q))
+        else:
+            s = rng.integers(0, len(keys), m)
+            w = np.floor(np.exp2(rng.uniform(4, 12, m))).astype(np.int64)
+            reqs.append(("range", keys[s].astype(np.uint64), keys[np.minimum(s + w, len(keys) - 1)].astype(np.uint64)))
+    alone_srv = bpt.Server(t, slots=1, mem_budget=1 << 30, mode=mode)
+    alone = _run(bpt, alone_srv, reqs)
+    alone_srv.close()
+    srv = bpt.Server(t, slots=8, mem_budget=8 << 30, mode=mode)
+    arrival = np.cumsum(rng.exponential(10e-6, size=len(reqs)))   # Poisson, 100K requests/s
+    mixed = _run(bpt, srv, reqs, arrival)
+    st = srv.stats()
+    srv.close()
+    print(f"c5 serving mode {mode}: {st}", flush=True)
+    assert st["completed"] == len(reqs) and st["max_inflight"] > 1
+    pq = [r[1] for r in reqs if r[0] == "point"]
+    want_p = np.split(oracle.point_closed(keys, vals, np.concatenate(pq), SENT), np.cumsum([len(x) for x in pq])[:-1])
+    k = 0
+    for r, a, m in zip(reqs, alone, mixed):
+        if r[0] == "point":
+            assert np.array_equal(a, m) and np.array_equal(m, want_p[k])
+            k += 1
+        else:
+            assert np.array_equal(a[0], m[0]) and np.array_equal(a[1], m[1])
+            f, c = oracle.range_closed(keys, r[1], r[2])
+            assert np.array_equal(m[1], c) and np.array_equal(m[0], f)
