@@ -507,7 +507,8 @@ __global__ __launch_bounds__(256) void k_point_learned_coop(TreeView t, LearnedV
 #pragma unroll
                 for (uint32_t j = 0; j < EPL; ++j) {
                     const uint32_t e = ja + sub * EPL + j;
-                    ev[k][j] = (a[k] <= b[k] && e <= jb && e < nn) ? nmax[e] : key_max<K>();
+                    // a window inside one level-1 node needs no maxima
+                    ev[k][j] = (a[k] <= b[k] && ja < jb && e <= jb && e < nn) ? nmax[e] : key_max<K>();
                 }
             }
             uint32_t node[8];
@@ -529,7 +530,7 @@ __global__ __launch_bounds__(256) void k_point_learned_coop(TreeView t, LearnedV
                     r = group8_sum(c);
                 }
                 const uint32_t nd = jj + r;
-                node[k] = a[k] > b[k] ? 0u : (nd > jb ? jb : nd);
+                node[k] = a[k] > b[k] ? 0u : (ja == jb ? ja : (nd > jb ? jb : nd));
             }
             const uint8_t *lv = t.inner + t.off[1];
             Chunk<NB> c1[8];

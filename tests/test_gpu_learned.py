@@ -115,3 +115,28 @@ def test_workload_c5_full_size(bpt, oracle, wl):
     dq.free()
     out.free()
     t.close()
+
+
+def test_learned_clustered_models_with_far_keys(bpt, oracle):
+    """Models (routing linear in the key, 4096 of them) that each hold a
+    cluster of 2000 keys and one key 2^51 further: windows of about 2000
+    slots spanning many level-1 nodes, predictions flat over the cluster;
+    then 16 models, whose windows span most of the tree. Every answer equals
+    the oracle."""
+    rng = np.random.default_rng(12)
+    parts = []
+    for j in range(0, 4096, 41):
+        base = j << 52
+        parts.append(base + np.arange(2000, dtype=np.uint64) * np.uint64(3))
+        parts.append(np.array([base + (1 << 51)], dtype=np.uint64))
+    keys = np.concatenate(parts).astype(np.uint64)
+    vals = np.arange(len(keys), dtype=np.uint64)
+    t = bpt.Tree(keys, vals, 8, 8)
+    q = np.concatenate([keys, keys + np.uint64(1), keys[rng.integers(0, len(keys), 50_000)],
+                        rng.integers(0, 1 << 64, 50_000, dtype=np.uint64)])
+    want = oracle.point_closed(keys, vals, q, SENT)
+    for models in (4096, 16):
+        st = t.learned_build(leaf_models=models, max_window=1 << 20, train_steps=50, root=3)
+        assert st["fallback_models"] == 0, st
+        assert np.array_equal(t.point_search(q, bpt.LEARNED), want), models
+    t.close()
