@@ -8,8 +8,12 @@
 // bucket is a range of leaves chosen from the tree itself, so buckets are
 // balanced in leaves whatever the key distribution:
 //   bucket b = leaves [b*lpb, (b+1)*lpb); its separator = its largest key;
-//   slice    = the bucket's entries of level s (s = 1: the leaf maxima),
-//              at most 32 KiB, staged in LDS by the search kernel.
+//   slice    = the bucket's sub-leaf maxima (the largest key of every
+//              64-byte half or quarter of a leaf, kept by the tree), or its
+//              entries of level s (s = 1: the leaf maxima), at most the
+//              tree's group_slice bytes (16 to 128 KiB), staged in LDS by the
+//              search kernel. With sub-leaf maxima a query reads the 64 bytes
+//              of its leaf that hold its key, not the whole leaf.
 // Passes over the queries (tiles of T queries):
 //   k_bkt_hist       bucket counts of each tile, one row per tile
 //   column scan      first slot of every (tile, bucket) run in bucket-major
@@ -20,7 +24,8 @@
 //   k_bkt_search     chunks of the bucket-ordered queries, grid-strided so
 //                    that the blocks in flight share a window of buckets
 //                    (their leaves stay in cache); lower bound in the LDS
-//                    slice, then levels s-1..1 and the leaf from global memory
+//                    slice, then levels s-1..1 and the leaf (or the sub-leaf)
+//                    from global memory
 //   k_bkt_unpermute  each tile gathers its runs back through LDS into the
 //                    caller's order
 // Every step is deterministic in its results (the order inside a run is not,
@@ -32,7 +37,7 @@
 namespace bpt {
 
 constexpr int kBBlock = 256;
-constexpr uint32_t kSliceBytes = 32768;    // LDS slice of the search kernel
+constexpr uint32_t kMaxSliceBytes = 131072; // LDS slice of the search kernel (at most)
 constexpr uint32_t kMaxBuckets = 1024;     // runs per partition tile
 constexpr size_t kMaxLds = 160 * 1024;     // LDS of a workgroup (gfx950)
 constexpr uint64_t kBucketChunk = 1ull << 28;  // queries per partition (u32 slots)
@@ -50,6 +55,8 @@ struct BucketDev {
     uint32_t ngroups;  // column-sum groups of kColGroup tiles
     uint32_t chunk;    // queries per search work chunk
     uint32_t xcd;      // search chunks split per XCD
+    uint32_t sbits;    // prefix-table bits of a search slice
+    uint32_t spl;      // sub-leaves per leaf when the slice holds sub-leaf maxima (0: level `level`)
 };
 
 static inline uint64_t align256(uint64_t b) { return (b + 255) / 256 * 256; }
@@ -109,7 +116,7 @@ __device__ __forceinline__ uint32_t lower_bound_window(const K *a, uint32_t lo, 
 }
 
 constexpr uint32_t kBucketTabBits = 10;  // prefix table of the bucket separators
-constexpr uint32_t kSliceTabBits = 10;   // prefix table of a search slice
+constexpr uint32_t kMaxSliceTabBits = 13;  // prefix table of a search slice (at most)
 
 template <class K>
 __device__ __forceinline__ uint32_t bucket_of(const K *bsep, uint32_t nb, const uint16_t *btab,
@@ -365,15 +372,93 @@ __global__ __launch_bounds__(kPBlock) void k_bkt_scatter(const K *__restrict__ q
     }
 }
 
-// LDS: bstart[nb + 1] | slice[epb] | stab[2^10 + 1]
+// Leaf step on a sub-leaf: the lane's query `myq` lies in sub-leaf `mysub`
+// (64 bytes of a leaf: slots [mysub*SUB, (mysub+1)*SUB) of the tree, SUB =
+// 64 / (2 * key bytes)). Group g (lanes 4g..4g+3) takes the 4 queries of its
+// lanes; lane s of the group loads 16-byte piece s of each query's sub-leaf,
+// so one load instruction fetches 16 sub-leaves and a query costs 64 bytes
+// instead of the whole leaf. The rank r of the query in its sub-leaf is the
+// DPP sum of the lanes' counts; the lane holding slot r checks the key and
+// hands the value (or the sentinel) to the query's lane.
+// subleaf_issue broadcasts the group's queries and issues the loads;
+// subleaf_finish ranks and answers. (Issuing the next batch's loads before
+// finishing the current one measured slower: the wave's loads are not what
+// bounds this step.)
+constexpr int kSubLanes = 4;
+
+template <class K>
+struct SubLeafBatch {
+    K qk[kSubLanes];
+    uint32_t sb[kSubLanes];
+    Chunk<16> c[kSubLanes];
+};
+
+template <class K, class V, int LF, bool OFF32>
+__device__ __forceinline__ void subleaf_issue(const TreeView &t, K myq, uint32_t mysub, SubLeafBatch<K> &b) {
+    constexpr int G = kSubLanes;
+    constexpr uint32_t SUB = 64 / (2 * sizeof(K));  // slots per sub-leaf
+    constexpr uint32_t SPL = LF / SUB;              // sub-leaves per leaf
+    constexpr uint32_t STRIDE = 2 * LF * sizeof(K);
+    static_assert(sizeof(K) == sizeof(V) && LF % SUB == 0 && SPL >= 2, "interleaved leaves of several sub-leaves");
+    const uint32_t lane = threadIdx.x & 63, g = lane / G, sl = lane % G;
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+        b.qk[k] = __shfl(myq, (int)(g * G + k), 64);
+        b.sb[k] = (uint32_t)__shfl((int)mysub, (int)(g * G + k), 64);
+    }
+    if constexpr (OFF32) {
+        const __amdgpu_buffer_rsrc_t rs = raw_rsrc(t.leaves);
+#pragma unroll
+        for (int k = 0; k < G; ++k)
+            load_chunk_buf<16, false>(rs, (b.sb[k] / SPL) * STRIDE + (b.sb[k] % SPL) * 64 + sl * 16, b.c[k]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < G; ++k)
+            load_chunk<16>(t.leaves + (uint64_t)(b.sb[k] / SPL) * STRIDE + (b.sb[k] % SPL) * 64 + sl * 16, b.c[k]);
+    }
+}
+
+template <class K, class V>
+__device__ __forceinline__ V subleaf_finish(const TreeView &t, const SubLeafBatch<K> &b, V sentinel) {
+    constexpr int G = kSubLanes;
+    constexpr uint32_t SUB = 64 / (2 * sizeof(K));
+    constexpr uint32_t PPL = 16 / (2 * sizeof(K));  // slots per lane
+    const uint32_t lane = threadIdx.x & 63, g = lane / G, sl = lane % G;
+    V res = sentinel;
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+        uint32_t cnt = 0;
+#pragma unroll
+        for (uint32_t i = 0; i < PPL; ++i) cnt += (piece_key<K>(b.c[k].w, i) < b.qk[k]);
+        const uint32_t r = group_sum<G>(cnt);  // lower bound inside the sub-leaf, 0..SUB
+        const uint32_t local = r - sl * PPL;   // r's slot in this lane, if < PPL
+        K key = piece_key<K>(b.c[k].w, 0);
+        V val = piece_val<V>(b.c[k].w, 0);
+#pragma unroll
+        for (uint32_t i = 1; i < PPL; ++i) {
+            key = local == i ? piece_key<K>(b.c[k].w, i) : key;
+            val = local == i ? piece_val<V>(b.c[k].w, i) : val;
+        }
+        // slot sb*SUB + r of the tree holds a key iff it is below n
+        const bool hit = (local < PPL) & (key == b.qk[k]) & ((uint64_t)b.sb[k] * SUB + r < t.n);
+        const V cand = hit ? val : sentinel;
+        const uint32_t owner = r / PPL < (uint32_t)G ? r / PPL : (uint32_t)G - 1;  // r = SUB: sentinel
+        const V got = __shfl(cand, (int)(g * G + owner), 64);
+        res = sl == (uint32_t)k ? got : res;
+    }
+    return res;
+}
+
+// LDS: bstart[nb + 1] | slice[epb] | stab[2^sbits + 1]
 // Work: the bucket-ordered queries in chunks of d.chunk. With d.xcd the
 // chunks are split into 8 contiguous parts, one per XCD, and the blocks of an
 // XCD stride through their part together, so an XCD works on about one bucket
 // at a time and that bucket's leaves stay in its L2.
-constexpr int kSearchBlock = 512;  // 8 waves share one staged slice; two blocks per CU
+constexpr int kMaxSearchBlock = 1024;  // waves of a block share one staged slice
+constexpr int kPieceBatches = 8;   // batches of 64 queries a wave loads at once
 
-template <class K, class V, int F, int LF, bool COOP, int G, bool OFF32>
-__global__ __launch_bounds__(kSearchBlock) void k_bkt_search(TreeView t, BucketDev d, const K *__restrict__ qs,
+template <class K, class V, int F, int LF, bool COOP, int G, bool OFF32, bool SUBL>
+__global__ __launch_bounds__(kMaxSearchBlock) void k_bkt_search(TreeView t, BucketDev d, const K *__restrict__ qs,
                                                         const uint32_t *__restrict__ bstart_g, uint64_t nq,
                                                         V *__restrict__ res, V sentinel) {
     extern __shared__ uint4 lds_raw[];
@@ -382,8 +467,9 @@ __global__ __launch_bounds__(kSearchBlock) void k_bkt_search(TreeView t, BucketD
     uint16_t *stab = reinterpret_cast<uint16_t *>(slice + d.epb);
     for (uint32_t b = threadIdx.x; b <= d.nb; b += blockDim.x) bstart[b] = bstart_g[b];
     __syncthreads();
-    const K *level = reinterpret_cast<const K *>(t.inner + t.off[d.level]);
-    const uint64_t level_count = t.count[d.level - 1];  // entries of level s = nodes of level s-1
+    // slice source: the sub-leaf maxima, or level s (its entries = nodes of level s-1)
+    const K *level = reinterpret_cast<const K *>(t.inner + (SUBL ? t.sub_off : t.off[d.level]));
+    const uint64_t level_count = SUBL ? t.count[0] * d.spl : t.count[d.level - 1];
     const int top = (int)d.level - 1;
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
     const uint64_t nchunks = (nq + d.chunk - 1) / d.chunk;
@@ -423,31 +509,66 @@ __global__ __launch_bounds__(kSearchBlock) void k_bkt_search(TreeView t, BucketD
                 for (uint32_t e = threadIdx.x; e < nvec; e += blockDim.x) reinterpret_cast<uint4 *>(slice)[e] = src4[e];
                 for (uint32_t e = nvec * per16 + threadIdx.x; e < eb; e += blockDim.x) slice[e] = level[first + e];
                 __syncthreads();
-                pi = prefix_index<K>(slice[0], slice[eb - 1], kSliceTabBits);
-                for (uint32_t p = threadIdx.x; p <= (1u << kSliceTabBits); p += blockDim.x)
-                    stab[p] = (uint16_t)(p == (1u << kSliceTabBits)
+                pi = prefix_index<K>(slice[0], slice[eb - 1], d.sbits);
+                for (uint32_t p = threadIdx.x; p <= (1u << d.sbits); p += blockDim.x)
+                    stab[p] = (uint16_t)(p == (1u << d.sbits)
                                              ? eb : lower_bound_flat<K>(slice, eb, prefix_start<K>(pi, p)));
                 __syncthreads();
                 cur = b;
             }
-            // the next batch's queries are loaded while this batch is searched
-            const uint64_t stride = (uint64_t)nw * 64;
-            uint64_t base = pos + (uint64_t)w * 64;
-            K next = base + lane < seg_end ? __builtin_nontemporal_load(&qs[base + lane]) : K(0);
-            for (; base < seg_end; base += stride) {
-                const uint64_t my = base + lane;
-                const bool ok = my < seg_end;
-                const K key = next;
-                next = my + stride < seg_end ? __builtin_nontemporal_load(&qs[my + stride]) : K(0);
-                const uint32_t p = prefix_of<K>(pi, key, kSliceTabBits);
-                const uint32_t at = lower_bound_window<K>(slice, stab[p], stab[p + 1], key);
-                const uint32_t node = (uint32_t)first + (at < eb ? at : eb - 1);
-                V r;
-                // leaves are reused by every query of the bucket: regular (temporal) loads
-                if constexpr (COOP)
-                    r = coop_finish64<K, V, F, LF, false, kOpValue, G, OFF32>(t, key, node, top, sentinel);
-                else r = thread_finish<K, V, F, LF>(t, key, node, top, sentinel);
-                if (ok) __builtin_nontemporal_store(r, &res[my]);
+            // A wave takes pieces of kPieceBatches batches of 64 consecutive
+            // queries and loads a piece's queries with back-to-back loads
+            // before its first search. A wave's loads complete in issue order,
+            // so a query load issued before a batch's leaf loads is waited for
+            // with them: one wait on the query stream per piece, not per batch.
+            // Loads past the segment's end re-read its last query (in bounds;
+            // their results are dropped).
+            const uint64_t piece = (uint64_t)kPieceBatches * 64;
+            for (uint64_t p0 = pos + (uint64_t)w * piece; p0 < seg_end; p0 += (uint64_t)nw * piece) {
+                K qv[kPieceBatches];
+#pragma unroll
+                for (int j = 0; j < kPieceBatches; ++j) {
+                    const uint64_t i = p0 + (uint64_t)j * 64 + lane;
+                    qv[j] = __builtin_nontemporal_load(&qs[i < seg_end ? i : seg_end - 1]);
+                }
+                // the node (or sub-leaf) of a query: lower bound in the LDS slice
+                auto locate = [&](K key) -> uint32_t {
+                    const uint32_t p = prefix_of<K>(pi, key, d.sbits);
+                    const uint32_t at = lower_bound_window<K>(slice, stab[p], stab[p + 1], key);
+                    return (uint32_t)first + (at < eb ? at : eb - 1);
+                };
+                V rv[kPieceBatches];
+                if constexpr (SUBL) {
+                    // Every batch of the piece runs (past the segment's end on its
+                    // last query, in bounds); only results inside it are stored.
+                    // Leaves are reused by every query of the bucket: regular
+                    // (temporal) loads.
+#pragma unroll
+                    for (int j = 0; j < kPieceBatches; ++j) {
+                        SubLeafBatch<K> bt;
+                        subleaf_issue<K, V, LF, OFF32>(t, qv[j], locate(qv[j]), bt);
+                        rv[j] = subleaf_finish<K, V>(t, bt, sentinel);
+                    }
+                } else {
+#pragma unroll
+                    for (int j = 0; j < kPieceBatches; ++j) {
+                        rv[j] = sentinel;
+                        if (p0 + (uint64_t)j * 64 < seg_end) {  // the same in every lane of the wave
+                            const K key = qv[j];
+                            const uint32_t node = locate(key);
+                            // leaves are reused by every query of the bucket: regular (temporal) loads
+                            if constexpr (COOP)
+                                rv[j] = coop_finish64<K, V, F, LF, false, kOpValue, G, OFF32>(t, key, node, top,
+                                                                                              sentinel);
+                            else rv[j] = thread_finish<K, V, F, LF>(t, key, node, top, sentinel);
+                        }
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < kPieceBatches; ++j) {
+                    const uint64_t i = p0 + (uint64_t)j * 64 + lane;
+                    if (i < seg_end) __builtin_nontemporal_store(rv[j], &res[i]);
+                }
             }
             pos = seg_end;
         }
@@ -524,25 +645,43 @@ static BucketPlan bucket_plan(const bpt_tree *t, uint64_t nq) {
     BucketPlan p{};
     const TreeView &v = t->view;
     const uint64_t L = v.count[0], F = t->fanout;
-    const uint64_t slice_cap = kSliceBytes / t->key_bytes;
+    const uint64_t slice_cap = std::min<uint32_t>(t->group_slice, kMaxSliceBytes) / t->key_bytes;
     const uint64_t nb_cap = std::min<uint64_t>(1ull << t->group_bits, kMaxBuckets);
-    uint64_t span = slice_cap;  // leaves per bucket at level s = slice_cap * F^(s-1)
-    uint32_t s = 1;
-    for (;; ++s) {
-        if (span >= L || (L + span - 1) / span <= nb_cap || s == (uint32_t)v.height) break;
-        span *= F;
-    }
     BucketDev &d = p.d;
-    d.level = s;
-    d.lpb = std::min(span, L);
-    d.nb = (uint32_t)((L + d.lpb - 1) / d.lpb);
-    uint64_t per = 1;  // leaves per entry of level s
-    for (uint32_t k = 1; k < s; ++k) per *= F;
-    d.epb = (uint32_t)((d.lpb + per - 1) / per);
+    const uint64_t spl = v.sub_slots ? t->leaf_cap / v.sub_slots : 0;  // sub-leaves per leaf
+    if (spl && t->group_sub && t->coop && L * spl < (1ull << 32) &&
+        (L + slice_cap / spl - 1) / (slice_cap / spl) <= nb_cap) {
+        // slice of sub-leaf maxima: the search reads 64 bytes of a leaf
+        d.level = 1;
+        d.spl = (uint32_t)spl;
+        d.lpb = std::min(slice_cap / spl, L);
+        d.nb = (uint32_t)((L + d.lpb - 1) / d.lpb);
+        d.epb = (uint32_t)(d.lpb * spl);
+    } else {
+        uint64_t span = slice_cap;  // leaves per bucket at level s = slice_cap * F^(s-1)
+        uint32_t s = 1;
+        for (;; ++s) {
+            if (span >= L || (L + span - 1) / span <= nb_cap || s == (uint32_t)v.height) break;
+            span *= F;
+        }
+        d.level = s;
+        d.spl = 0;
+        d.lpb = std::min(span, L);
+        d.nb = (uint32_t)((L + d.lpb - 1) / d.lpb);
+        uint64_t per = 1;  // leaves per entry of level s
+        for (uint32_t k = 1; k < s; ++k) per *= F;
+        d.epb = (uint32_t)((d.lpb + per - 1) / per);
+    }
+    // prefix table of about epb / 4 entries: windows of ~4 slice entries
+    d.sbits = 4;
+    while (d.sbits < kMaxSliceTabBits && (4ull << d.sbits) < d.epb) ++d.sbits;
     d.tile = t->key_bytes == 4 ? tile_of<uint32_t>() : tile_of<uint64_t>();
     d.ntiles = (uint32_t)std::max<uint64_t>(1, (nq + d.tile - 1) / d.tile);
     d.ngroups = (d.ntiles + kColGroup - 1) / kColGroup;
-    const uint64_t cap = (uint64_t)device_cu_count() * t->group_blocks;
+    // search blocks per CU: as asked, at most as many as the CU's LDS holds
+    const size_t slice_lds = ((d.nb + 1 + 3) / 4) * 16 + (size_t)d.epb * t->key_bytes + ((1u << d.sbits) + 1) * 2;
+    const uint64_t per_cu = std::max<uint64_t>(1, std::min<uint64_t>(t->group_blocks, kMaxLds / slice_lds));
+    const uint64_t cap = (uint64_t)device_cu_count() * per_cu;
     if (t->group_chunk) {
         d.chunk = t->group_chunk;
     } else {
@@ -570,10 +709,10 @@ static BucketPlan bucket_plan(const bpt_tree *t, uint64_t nq) {
     p.off_btab = off;   off += align256(((1ull << kBucketTabBits) + 1) * 2);
     p.total = off;
     const size_t kb = t->key_bytes, vb = t->val_bytes;
-    const size_t btab_bytes = ((1u << kBucketTabBits) + 1) * 2, stab_bytes = ((1u << kSliceTabBits) + 1) * 2;
+    const size_t btab_bytes = ((1u << kBucketTabBits) + 1) * 2;
     p.lds_hist = d.nb * (kb + 4) + btab_bytes;
     p.lds_scatter = d.nb * (kb + 8) + (size_t)d.tile * (kb + 4) + btab_bytes;
-    p.lds_search = ((d.nb + 1 + 3) / 4) * 16 + (size_t)d.epb * kb + stab_bytes;
+    p.lds_search = slice_lds;
     p.lds_unpermute = ((2 * d.nb + 1 + 3) / 4) * 16 + (size_t)d.tile * vb;
     return p;
 }
@@ -618,24 +757,37 @@ static bpt_status bucket_chunk(const bpt_tree *t, const K *q, uint64_t nq, V *ou
     bool coop = false;
     if constexpr (sizeof(K) == sizeof(V)) coop = t->coop && t->view.count[0] < (1ull << 32);
     const bool off32 = t->addr32 && t->leaf_bytes <= (1ull << 32) && t->inner_bytes <= (1ull << 32);
-    auto go = [&](auto cp, auto lanes, auto o32) {
-        k_bkt_search<K, V, F, LF, decltype(cp)::value, decltype(lanes)::value, decltype(o32)::value>
-            <<<p.grid, kSearchBlock, p.lds_search, s>>>(t->view, d, qs, bstart, nq, res, sentinel);
+    auto go = [&](auto cp, auto lanes, auto o32, auto subl) {
+        k_bkt_search<K, V, F, LF, decltype(cp)::value, decltype(lanes)::value, decltype(o32)::value,
+                     decltype(subl)::value>
+            <<<p.grid, t->group_threads, p.lds_search, s>>>(t->view, d, qs, bstart, nq, res, sentinel);
     };
+    auto go_leaf = [&](auto cp, auto lanes, auto o32) { go(cp, lanes, o32, std::false_type{}); };
+    if constexpr (sizeof(K) == sizeof(V) && LF * sizeof(K) > 32) {  // leaves of several sub-leaves
+        if (d.spl) {
+            if (off32) go(std::true_type{}, IC<4>{}, std::true_type{}, std::true_type{});
+            else go(std::true_type{}, IC<4>{}, std::false_type{}, std::true_type{});
+            BPT_HIP(hipGetLastError());
+            k_bkt_unpermute<K, V><<<d.ntiles, kPBlock, p.lds_unpermute, s>>>(hist, offs, d, lidx, res, nq, out);
+            BPT_HIP(hipGetLastError());
+            return BPT_OK;
+        }
+    }
+    BPT_CHECK(d.spl == 0, BPT_ERR_INVALID, "sub-leaf slice planned for a tree without sub-leaves");
     if constexpr (sizeof(K) == sizeof(V)) {
         if (coop) {
             if (t->group_lanes == 4) {
-                if (off32) go(std::true_type{}, IC<4>{}, std::true_type{});
-                else go(std::true_type{}, IC<4>{}, std::false_type{});
+                if (off32) go_leaf(std::true_type{}, IC<4>{}, std::true_type{});
+                else go_leaf(std::true_type{}, IC<4>{}, std::false_type{});
             } else {
-                if (off32) go(std::true_type{}, IC<8>{}, std::true_type{});
-                else go(std::true_type{}, IC<8>{}, std::false_type{});
+                if (off32) go_leaf(std::true_type{}, IC<8>{}, std::true_type{});
+                else go_leaf(std::true_type{}, IC<8>{}, std::false_type{});
             }
         } else {
-            go(std::false_type{}, IC<8>{}, std::false_type{});
+            go_leaf(std::false_type{}, IC<8>{}, std::false_type{});
         }
     } else {
-        go(std::false_type{}, IC<8>{}, std::false_type{});
+        go_leaf(std::false_type{}, IC<8>{}, std::false_type{});
     }
     BPT_HIP(hipGetLastError());
     k_bkt_unpermute<K, V><<<d.ntiles, kPBlock, p.lds_unpermute, s>>>(hist, offs, d, lidx, res, nq, out);

@@ -63,6 +63,18 @@ __global__ __launch_bounds__(256) void k_build_inner(const K *__restrict__ keys,
     }
 }
 
+// Sub-leaf maxima: entry s is the last key of slots [s*sub, (s+1)*sub), the
+// largest key there (the key maximum when the run holds no key).
+template <class K>
+__global__ __launch_bounds__(256) void k_build_sub(const K *__restrict__ keys, uint64_t n, uint64_t nsub,
+                                                   uint32_t sub, K *__restrict__ out) {
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < nsub; s += step) {
+        const uint64_t first = s * sub, last = first + sub - 1;
+        out[s] = first < n ? keys[last < n ? last : n - 1] : key_max<K>();
+    }
+}
+
 template <class K>
 __global__ __launch_bounds__(256) void k_check_sorted(const K *__restrict__ keys, uint64_t n,
                                                       uint32_t *bad) {
@@ -100,6 +112,17 @@ static bpt_status compute_layout(bpt_tree *t) {
     for (int k = h; k >= 1; --k) {
         v.off[k] = off;
         off += v.count[k] * F * kb;
+    }
+    // Sub-leaf maxima, after the levels: the largest key of every run of
+    // `sub` slots of a leaf (one 64-byte run of (key, value) pairs in the
+    // interleaved format), for leaves that hold several such runs. The
+    // grouped search stages a bucket's slice of them in LDS and then reads
+    // 64 bytes of a leaf per query instead of the whole leaf.
+    const uint64_t sub = 64 / (2 * kb);
+    if (t->key_bytes == t->val_bytes && h >= 1 && LF > sub && LF % sub == 0) {
+        v.sub_slots = (uint32_t)sub;
+        v.sub_off = (off + 255) / 256 * 256;
+        off = v.sub_off + v.count[0] * (LF / sub) * kb;
     }
     t->inner_bytes = off;
     t->leaf_bytes = v.count[0] * t->leaf_stride;
@@ -140,10 +163,18 @@ static bpt_status build_impl(bpt_tree *t, const void *d_keys, const void *d_vals
             span = (span > (~0ull) / F) ? ~0ull : span * F;
             if (k >= 1) d.first_slot[k] = v.off[k] / sizeof(K);
         }
-        const uint64_t total = t->inner_bytes / sizeof(K);
+        // separator slots of every level (level 1, the leaf maxima, is stored last)
+        const uint64_t total = (v.off[1] + v.count[1] * F * sizeof(K)) / sizeof(K);
         uint64_t blocks = std::min<uint64_t>((total + 255) / 256, (uint64_t)device_cu_count() * 16);
         k_build_inner<K, F><<<(uint32_t)blocks, 256, 0, s>>>((const K *)d_keys, n, d, total,
                                                              (K *)t->d_inner);
+        BPT_HIP(hipGetLastError());
+    }
+    if (v.sub_slots) {
+        const uint64_t nsub = v.count[0] * (LF / v.sub_slots);
+        uint64_t blocks = std::min<uint64_t>((nsub + 255) / 256, (uint64_t)device_cu_count() * 16);
+        k_build_sub<K><<<(uint32_t)blocks, 256, 0, s>>>((const K *)d_keys, n, nsub, v.sub_slots,
+                                                        (K *)(t->d_inner + v.sub_off));
         BPT_HIP(hipGetLastError());
     }
     return BPT_OK;
@@ -390,6 +421,22 @@ bpt_status bpt_tree_set_option(bpt_tree *t, const char *key, int64_t value) {
     if (strcmp(key, "group_blocks") == 0) {
         BPT_CHECK(value >= 1 && value <= 8, BPT_ERR_INVALID, "group_blocks must be in [1, 8]");
         t->group_blocks = (uint32_t)value;
+        return BPT_OK;
+    }
+    if (strcmp(key, "group_slice") == 0) {
+        BPT_CHECK(value == 16384 || value == 32768 || value == 65536 || value == 131072, BPT_ERR_INVALID,
+                  "group_slice must be 16384, 32768, 65536 or 131072 bytes");
+        t->group_slice = (uint32_t)value;
+        return BPT_OK;
+    }
+    if (strcmp(key, "group_threads") == 0) {
+        BPT_CHECK(value == 512 || value == 768 || value == 1024, BPT_ERR_INVALID,
+                  "group_threads must be 512, 768 or 1024");
+        t->group_threads = (uint32_t)value;
+        return BPT_OK;
+    }
+    if (strcmp(key, "group_sub") == 0) {
+        t->group_sub = value ? 1 : 0;
         return BPT_OK;
     }
     if (strcmp(key, "group_xcd") == 0) {

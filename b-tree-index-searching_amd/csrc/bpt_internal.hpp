@@ -53,9 +53,10 @@ struct TreeView {
     int32_t height;         // internal levels (0: the single leaf is the root)
     int32_t lds_levels;     // top internal levels copied into LDS
     uint32_t lds_bytes;     // bytes of `inner` copied into LDS (a prefix)
-    uint32_t pad_;
+    uint32_t sub_slots;     // slots per sub-leaf of the sub-leaf maxima (0: none)
     uint64_t count[kMaxLevels + 1];  // nodes per level, [0] = leaves
     uint64_t off[kMaxLevels + 1];    // byte offset of level k inside `inner`
+    uint64_t sub_off;                // byte offset of the sub-leaf maxima inside `inner`
 };
 
 struct Workspace {
@@ -88,7 +89,10 @@ struct bpt_tree {
     uint32_t group_chunk = 0;        // grouped search: queries per work chunk (0: by batch size)
     uint32_t group_xcd = 1;          // grouped search: chunks split per XCD
     uint32_t group_lanes = 4;        // grouped search: lanes per query at the leaf (4 or 8)
-    uint32_t group_blocks = 2;       // grouped search: 512-thread blocks per CU
+    uint32_t group_blocks = 1;       // grouped search: search blocks per CU (at most what LDS holds)
+    uint32_t group_threads = 1024;   // grouped search: threads per search block (512, 768 or 1024)
+    uint32_t group_slice = 131072;   // grouped search: LDS slice bytes per bucket (16 to 128 KiB)
+    uint32_t group_sub = 1;          // grouped search: slices of sub-leaf maxima when the tree has them
     uint64_t alloc_limit = 0;        // largest leaf / inner buffer a rebuild may allocate (0: none)
     bpt::TreeView view{};
     bpt::Workspace ws;             // grouped / learned scratch owned by the tree

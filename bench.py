@@ -380,6 +380,28 @@ def bench_c4(args):
     return line
 
 
+def compat_grouping_run(n):
+    """Runs bptree_groups genGrouping on n uniform uint64 keys and returns its
+    wall time (the call: copy in, the device pass loop, the group lists)."""
+    import subprocess
+    import tempfile
+    exe = os.path.join(PKG, "bptree_groups")
+    keys = np.random.default_rng(12).integers(0, 1 << 64, size=n, dtype=np.uint64)
+    with tempfile.TemporaryDirectory(dir="/tmp") as d:
+        keys.tofile(os.path.join(d, "k"))
+        del keys
+        r = subprocess.run([exe, "genGrouping", str(n), "33000", os.path.join(d, "o"), os.path.join(d, "k")],
+                           capture_output=True, text=True, timeout=300)
+        if r.returncode != 0:
+            return {"error": r.stderr[-300:]}
+        sizes = np.fromfile(os.path.join(d, "o.list"), dtype=np.uint64)
+    fields = dict(kv.split("=") for kv in r.stdout.split() if "=" in kv)
+    return {"ms": float(fields["ms"]), "groups": int(fields["groups"]), "keys": n,
+            "sizes_sum_to_n": bool(int(sizes.sum()) == n), "max_group": int(sizes.max()),
+            "note": "wall time of the call from host memory: H2D 512 MB, device histograms and segmented "
+                    "reorders per pass, the merge on the host; grouped keys stay on the device"}
+
+
 def serving_run(tree, keys, n, miss_space, nreq, modes, workload):
     """The serving loop over a resident tree: an open-loop stream of nreq mixed
     batches (sizes floor(2^u), u uniform in [0, 16]; 90% point batches with
@@ -630,9 +652,18 @@ def main():
                                      "note": "pageable host buffers; H2D 1 GiB + search + D2H 1 GiB"}
         del host_out
         wall_g, ev_g = time_form(tree, dq, dout, bptree.GROUPED, max(3, args.steps // 2), 1)
+        # the grouped form's own algorithmic bytes per lookup: the histogram
+        # reads the query (4), the scatter reads it and writes it with its
+        # 2-byte tile index (10), the search reads it and writes the result
+        # (8) and reads the leaves once (leaf bytes / queries), the unpermute
+        # reads the result and the index and writes the result (10)
+        own = 4 + 10 + 8 + info["leaf_bytes"] / nq + 10
         forms["grouped_uniform"] = {"lookups_per_s": nq / wall_g, "ms": wall_g * 1e3,
                                     "speedup_vs_plain": wall / wall_g,
                                     "algorithmic_GBps": BYTES_PER_LOOKUP * nq / wall_g / 1e9,
+                                    "own_bytes_per_lookup": own,
+                                    "own_roofline": {"achieved": own * nq / wall_g / 1e9, "peak": HBM_PEAK_GBS,
+                                                     "unit": "GB/s", "frac": own * nq / wall_g / 1e9 / HBM_PEAK_GBS},
                                     "mismatches_vs_plain": int((dout.to_host() != gpu_out).sum())}
         # the learned position model on the same tree: MLP 1-32-32-1 root over
         # 65536 linear leaf models, windows of at most 256 slots, trained on
@@ -714,6 +745,11 @@ def main():
                                                                   and np.array_equal(ids_h, base[1]))}
         for a in (dlo, dhi, dfirst, dcnt, doff, ids_buf[0]):
             a.free()
+        # the reference-named genGrouping (host keys in, grouped keys on the
+        # device out) over 2^26 uniform uint64 keys with the reference's group
+        # cap of 33000, through the compiled drop-in driver
+        forms["compat_genGrouping_2^26"] = compat_grouping_run(1 << 26)
+        log("genGrouping: " + json.dumps(forms["compat_genGrouping_2^26"]))
         # the serving loop over this tree (p50 / p99 latency)
         serving, nreq, sl = serving_run(tree, keys, n, 1 << 32, 3000, (bptree.PLAIN, bptree.LEARNED), 2)
         forms["serving"] = {"requests": nreq, "lookups": sl, **serving}

@@ -141,7 +141,16 @@ bpt_status bpt_tree_erase(bpt_tree *t, const void *d_keys, uint64_t n, bpt_strea
  * "group_chunk" (queries per search work chunk; 0, the default, picks about
  * 16 chunks per search block, a power of two in [1024, 32768]),
  * "group_lanes" (lanes per query at the leaf: 4, the default, or 8),
- * "group_blocks" (search blocks of 512 threads per CU, default 2) and
+ * "group_blocks" (search blocks per CU, default 1; at most as many as the
+ * CU's LDS holds),
+ * "group_threads" (threads per search block: 512, 768 or 1024, the default),
+ * "group_slice" (LDS bytes of one bucket's slice of the tree: 16384, 32768,
+ * 65536 or 131072, the default; a larger slice means fewer, wider buckets
+ * and fewer search blocks per CU),
+ * "group_sub" (1 = slices of sub-leaf maxima, so that a query reads 64
+ * bytes of its leaf, the default where the tree's leaves span several
+ * 64-byte runs and its key and value widths match; 0 = leaf maxima and
+ * whole leaves) and
  * "group_xcd" (1 = each XCD searches its own contiguous part of the
  * bucket-ordered queries, the default). "alloc_limit" (bytes; 0, the
  * default, means none) caps each leaf or inner buffer that an update's

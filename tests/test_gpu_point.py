@@ -57,9 +57,12 @@ def test_point_plain_and_grouped(bpt, oracle, cfg, n):
     t.set_option("coop", 1)
     if kb * 8 >= 12:
         assert np.array_equal(t.point_search(q, bpt.GROUPED), want)
+        t.set_option("group_sub", 0)  # leaf-maxima slices, whole leaves
+        assert np.array_equal(t.point_search(q, bpt.GROUPED), want)
         t.set_option("group_lanes", 8)
         assert np.array_equal(t.point_search(q, bpt.GROUPED), want)
         t.set_option("group_lanes", 4)
+        t.set_option("group_sub", 1)
     pos = t.lower_bound(q)
     assert np.array_equal(pos, oracle.lower_bound(keys, q))
     for opt, val in (("lanes", 4), ("coop", 0)):  # the cooperative lower bound with 4 lanes; one lane per query
@@ -109,8 +112,40 @@ def test_grouped_slice_levels(bpt, oracle, kb, n, bits):
     for lanes in (4, 8):
         t.set_option("group_lanes", lanes)
         assert np.array_equal(t.point_search(q, bpt.GROUPED), want), lanes
+    t.set_option("group_lanes", 4)
+    for slice_bytes in (16384, 32768, 65536):  # narrower buckets
+        t.set_option("group_slice", slice_bytes)
+        assert np.array_equal(t.point_search(q, bpt.GROUPED), want), slice_bytes
+    t.set_option("group_slice", 131072)
     t.set_option("coop", 0)
     assert np.array_equal(t.point_search(q, bpt.GROUPED), want)
+
+
+@pytest.mark.parametrize("kb", [4, 8])
+@pytest.mark.parametrize("tail", [1, 3, 4, 5, 8, 9, 15, 16])
+def test_grouped_sub_leaf_edges(bpt, oracle, kb, tail):
+    """Sub-leaf slices at the edges: a last leaf holding `tail` keys (its
+    later sub-leaves empty), runs of a duplicate key across sub-leaf and leaf
+    boundaries, the all-ones key as a real key, and queries past the largest
+    key, each against the leaf-maxima path and the oracle."""
+    rng = np.random.default_rng(tail * 7 + kb)
+    kdt = np.uint32 if kb == 4 else np.uint64
+    lf = 16
+    n = lf * 3000 + tail
+    keys = np.sort(rng.integers(0, 40000, size=n, dtype=np.uint64)).astype(kdt)  # about 1.2 per key
+    keys[2 * lf * 7 + 3: 2 * lf * 7 + 40] = keys[2 * lf * 7 + 3]  # a run over several sub-leaves
+    keys = np.sort(keys)
+    keys[-3:] = np.iinfo(kdt).max
+    vals = np.arange(n, dtype=np.uint64).astype(kdt)
+    t = bpt.Tree(keys, vals, kb, kb, fanout=32, leaf_cap=lf)
+    q = np.concatenate([keys[::3], np.arange(0, 40100, 7, dtype=np.uint64).astype(kdt),
+                        np.array([np.iinfo(kdt).max, np.iinfo(kdt).max - 1, 40000, 0], dtype=kdt)])
+    rng.shuffle(q)
+    want = _expect(oracle, keys, vals, q, kb)
+    assert np.array_equal(t.point_search(q, bpt.GROUPED), want)
+    t.set_option("group_sub", 0)
+    assert np.array_equal(t.point_search(q, bpt.GROUPED), want)
+    t.close()
 
 
 def test_grouped_skewed_queries(bpt, oracle):
@@ -125,10 +160,15 @@ def test_grouped_skewed_queries(bpt, oracle):
                         rng.integers(0, 1 << 32, size=100_000, dtype=np.uint64).astype(np.uint32)])
     rng.shuffle(q)
     want = _expect(oracle, keys, vals, q, 4)
-    for chunk, xcd in ((0, 1), (16384, 1), (256, 1), (4096, 0)):
+    for chunk, xcd, slice_bytes, threads, blocks in ((0, 1, 131072, 1024, 1), (16384, 1, 131072, 1024, 1),
+                                                     (256, 1, 131072, 512, 1), (4096, 0, 65536, 1024, 1),
+                                                     (0, 1, 32768, 768, 2), (1024, 0, 16384, 512, 4)):
         t.set_option("group_chunk", chunk)
         t.set_option("group_xcd", xcd)
-        assert np.array_equal(t.point_search(q, bpt.GROUPED), want), (chunk, xcd)
+        t.set_option("group_slice", slice_bytes)
+        t.set_option("group_threads", threads)
+        t.set_option("group_blocks", blocks)
+        assert np.array_equal(t.point_search(q, bpt.GROUPED), want), (chunk, xcd, slice_bytes, threads, blocks)
 
 
 def test_workload_c1_scaled_and_page_tree(bpt, oracle, wl):
