@@ -38,7 +38,7 @@ namespace bpt {
 
 constexpr int kBBlock = 256;
 constexpr uint32_t kMaxSliceBytes = 131072; // LDS slice of the search kernel (at most)
-constexpr uint32_t kMaxBuckets = 1024;     // runs per partition tile
+constexpr uint32_t kMaxBuckets = 1024;     // runs per partition tile (one partition thread each)
 constexpr size_t kMaxLds = 160 * 1024;     // LDS of a workgroup (gfx950)
 constexpr uint64_t kBucketChunk = 1ull << 28;  // queries per partition (u32 slots)
 
@@ -66,6 +66,7 @@ static inline uint64_t align256(uint64_t b) { return (b + 255) / 256 * 256; }
 // the (tile, bucket) runs long, so the runs are written and read back as
 // whole lines; a tile's staging uses up to ~144 KiB of the 160 KiB of LDS.
 constexpr int kPBlock = 1024;
+static_assert(kMaxBuckets <= (uint32_t)kPBlock, "one partition thread per bucket");
 template <class K> constexpr uint32_t tile_of() { return sizeof(K) == 4 ? 16384u : 8192u; }
 template <class K> constexpr int per_thread() { return (int)(tile_of<K>() / kPBlock); }
 
@@ -328,8 +329,18 @@ __global__ __launch_bounds__(kPBlock) void k_bkt_scatter(const K *__restrict__ q
             key[e] = i < cnt ? __builtin_nontemporal_load(&q[lo + i]) : K(0);
         }
     };
+    // thread b < nb (nb <= kMaxBuckets = kPBlock) holds bucket b's entries of
+    // a tile's histogram row and offset row, loaded with the tile's queries
+    uint32_t hrow = 0, orow = 0;
+    auto load_rows = [&](uint32_t tile) {
+        if (tile < t1 && threadIdx.x < d.nb) {
+            hrow = hist[(uint64_t)tile * d.nb + threadIdx.x];  // the tile's count of bucket b
+            orow = off[(uint64_t)tile * d.nb + threadIdx.x];   // first slot of its run
+        }
+    };
     K key[E];  // issued first: their latency overlaps the staging and the scan
     load_tile(t0, key);
+    load_rows(t0);
     for (uint32_t b = threadIdx.x; b < d.nb; b += blockDim.x) bsep[b] = bsep_g[b];
     for (uint32_t p = threadIdx.x; p <= (1u << kBucketTabBits); p += blockDim.x) btab[p] = btab_g[p];
     const PrefixIndex<K> pi{(K)hdr[0], (uint32_t)hdr[1]};
@@ -337,12 +348,11 @@ __global__ __launch_bounds__(kPBlock) void k_bkt_scatter(const K *__restrict__ q
         const uint64_t lo = (uint64_t)tile * d.tile;
         const uint32_t cnt = (uint32_t)(std::min<uint64_t>(lo + d.tile, nq) - lo);
         __syncthreads();  // the previous tile's write-out has read sk, sbl and delta
-        for (uint32_t b = threadIdx.x; b < d.nb; b += blockDim.x)
-            lcur[b] = hist[(uint64_t)tile * d.nb + b];  // this tile's count of bucket b
+        if (threadIdx.x < d.nb) lcur[threadIdx.x] = hrow;
+        const uint32_t orun = orow;
         __syncthreads();
         block_exclusive_scan(lcur, d.nb, wsum);  // first tile slot of every bucket
-        for (uint32_t b = threadIdx.x; b < d.nb; b += blockDim.x)
-            delta[b] = off[(uint64_t)tile * d.nb + b] - lcur[b];
+        if (threadIdx.x < d.nb) delta[threadIdx.x] = orun - lcur[threadIdx.x];
         __syncthreads();
         uint32_t bk[E];
 #pragma unroll
@@ -356,7 +366,8 @@ __global__ __launch_bounds__(kPBlock) void k_bkt_scatter(const K *__restrict__ q
                 sbl[slot] = (bk[e] << 16) | i;
             }
         }
-        load_tile(tile + gridDim.x, key);  // the next tile's queries arrive during the write-out
+        load_tile(tile + gridDim.x, key);  // the next tile's queries and rows arrive during the write-out
+        load_rows(tile + gridDim.x);
         __syncthreads();
         // tile-sorted element j goes to its bucket run: consecutive j, consecutive slots
 #pragma unroll
@@ -592,10 +603,14 @@ __global__ __launch_bounds__(kPBlock) void k_bkt_unpermute(const uint32_t *__res
     V *outt = reinterpret_cast<V *>(lds_raw + (2 * d.nb + 1 + 3) / 4);
     uint16_t *bid = reinterpret_cast<uint16_t *>(outt);
     const uint32_t tile = xcd_contiguous(blockIdx.x, gridDim.x);
-    for (uint32_t b = threadIdx.x; b < d.nb; b += blockDim.x) lstart[b] = hist[(uint64_t)tile * d.nb + b];
+    // thread b < nb (nb <= kMaxBuckets = kPBlock) loads bucket b's count and run
+    // offset of the tile together
+    const uint32_t hrow = threadIdx.x < d.nb ? hist[(uint64_t)tile * d.nb + threadIdx.x] : 0u;
+    const uint32_t orow = threadIdx.x < d.nb ? off[(uint64_t)tile * d.nb + threadIdx.x] : 0u;
+    if (threadIdx.x < d.nb) lstart[threadIdx.x] = hrow;
     __syncthreads();
     const uint32_t total = block_exclusive_scan(lstart, d.nb, wsum);
-    for (uint32_t b = threadIdx.x; b < d.nb; b += blockDim.x) delta[b] = off[(uint64_t)tile * d.nb + b] - lstart[b];
+    if (threadIdx.x < d.nb) delta[threadIdx.x] = orow - lstart[threadIdx.x];
     if (threadIdx.x == 0) lstart[d.nb] = total;
     __syncthreads();
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
