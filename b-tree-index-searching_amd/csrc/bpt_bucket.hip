@@ -116,6 +116,27 @@ __device__ __forceinline__ uint32_t lower_bound_window(const K *a, uint32_t lo, 
     return hi > lo ? lo + lower_bound_flat<K>(a + lo, hi - lo, q) : lo;
 }
 
+// Prefix table of a sorted LDS array a[0, n), n >= 1, built by all threads of
+// the block, with pi = prefix_index(a[0], a[n-1], bits): tab[p] = the lower
+// bound of prefix_start(p) for p < 2^bits, tab[2^bits] = n. With f(x) =
+// (x - base) >> shift, a[e] >= prefix_start(p) exactly when f(a[e]) >= p, so
+// entry e takes the prefixes in (f(a[e-1]), f(a[e])] and n takes those past
+// f(a[n-1]) (up to half the table, written by all threads): one pass over the
+// array, every table entry written once, instead of a binary search per table
+// entry. A gap in the keys makes one entry's range long (it is written by that
+// entry's thread).
+template <class K>
+__device__ void build_prefix_table(const K *a, uint32_t n, const PrefixIndex<K> &pi, uint32_t bits, uint16_t *tab) {
+    const uint32_t np = 1u << bits;
+    auto f = [&](K x) -> uint32_t { return (uint32_t)((K)(x - pi.base) >> pi.shift); };  // < np for x in a
+    for (uint32_t e = threadIdx.x; e < n; e += blockDim.x) {
+        const uint32_t lo = e == 0 ? 0u : f(a[e - 1]) + 1;
+        const uint32_t hi = f(a[e]) + 1;
+        for (uint32_t p = lo; p < hi; ++p) tab[p] = (uint16_t)e;
+    }
+    for (uint32_t p = f(a[n - 1]) + 1 + threadIdx.x; p <= np; p += blockDim.x) tab[p] = (uint16_t)n;
+}
+
 constexpr uint32_t kBucketTabBits = 10;  // prefix table of the bucket separators
 constexpr uint32_t kMaxSliceTabBits = 13;  // prefix table of a search slice (at most)
 
@@ -430,7 +451,7 @@ __device__ __forceinline__ void subleaf_issue(const TreeView &t, K myq, uint32_t
 }
 
 template <class K, class V>
-__device__ __forceinline__ V subleaf_finish(const TreeView &t, const SubLeafBatch<K> &b, V sentinel) {
+__device__ __forceinline__ V subleaf_finish(const SubLeafBatch<K> &b, uint32_t nsb, uint32_t nrem, V sentinel) {
     constexpr int G = kSubLanes;
     constexpr uint32_t SUB = 64 / (2 * sizeof(K));
     constexpr uint32_t PPL = 16 / (2 * sizeof(K));  // slots per lane
@@ -450,8 +471,9 @@ __device__ __forceinline__ V subleaf_finish(const TreeView &t, const SubLeafBatc
             key = local == i ? piece_key<K>(b.c[k].w, i) : key;
             val = local == i ? piece_val<V>(b.c[k].w, i) : val;
         }
-        // slot sb*SUB + r of the tree holds a key iff it is below n
-        const bool hit = (local < PPL) & (key == b.qk[k]) & ((uint64_t)b.sb[k] * SUB + r < t.n);
+        // slot sb*SUB + r of the tree holds a key iff it is below n = nsb*SUB + nrem
+        const bool valid = (b.sb[k] < nsb) | ((b.sb[k] == nsb) & (r < nrem));
+        const bool hit = (local < PPL) & (key == b.qk[k]) & valid;
         const V cand = hit ? val : sentinel;
         const uint32_t owner = r / PPL < (uint32_t)G ? r / PPL : (uint32_t)G - 1;  // r = SUB: sentinel
         const V got = __shfl(cand, (int)(g * G + owner), 64);
@@ -480,7 +502,11 @@ __global__ __launch_bounds__(kMaxSearchBlock) void k_bkt_search(TreeView t, Buck
     __syncthreads();
     // slice source: the sub-leaf maxima, or level s (its entries = nodes of level s-1)
     const K *level = reinterpret_cast<const K *>(t.inner + (SUBL ? t.sub_off : t.off[d.level]));
-    const uint64_t level_count = SUBL ? t.count[0] * d.spl : t.count[d.level - 1];
+    // (sub-leaves past the last key hold no key: they are left out of the slices)
+    const uint64_t level_count = SUBL ? (t.n + t.sub_slots - 1) / t.sub_slots : t.count[d.level - 1];
+    // slot sb*SUB + r holds a key iff sb < nsb, or sb == nsb and r < nrem (32-bit: sub-leaf indices are)
+    const uint32_t nsb = SUBL ? (uint32_t)(t.n / t.sub_slots) : 0u;
+    const uint32_t nrem = SUBL ? (uint32_t)(t.n % t.sub_slots) : 0u;
     const int top = (int)d.level - 1;
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
     const uint64_t nchunks = (nq + d.chunk - 1) / d.chunk;
@@ -521,9 +547,7 @@ __global__ __launch_bounds__(kMaxSearchBlock) void k_bkt_search(TreeView t, Buck
                 for (uint32_t e = nvec * per16 + threadIdx.x; e < eb; e += blockDim.x) slice[e] = level[first + e];
                 __syncthreads();
                 pi = prefix_index<K>(slice[0], slice[eb - 1], d.sbits);
-                for (uint32_t p = threadIdx.x; p <= (1u << d.sbits); p += blockDim.x)
-                    stab[p] = (uint16_t)(p == (1u << d.sbits)
-                                             ? eb : lower_bound_flat<K>(slice, eb, prefix_start<K>(pi, p)));
+                build_prefix_table<K>(slice, eb, pi, d.sbits, stab);
                 __syncthreads();
                 cur = b;
             }
@@ -558,7 +582,7 @@ __global__ __launch_bounds__(kMaxSearchBlock) void k_bkt_search(TreeView t, Buck
                     for (int j = 0; j < kPieceBatches; ++j) {
                         SubLeafBatch<K> bt;
                         subleaf_issue<K, V, LF, OFF32>(t, qv[j], locate(qv[j]), bt);
-                        rv[j] = subleaf_finish<K, V>(t, bt, sentinel);
+                        rv[j] = subleaf_finish<K, V>(bt, nsb, nrem, sentinel);
                     }
                 } else {
 #pragma unroll

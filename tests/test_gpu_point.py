@@ -148,6 +148,28 @@ def test_grouped_sub_leaf_edges(bpt, oracle, kb, tail):
     t.close()
 
 
+@pytest.mark.parametrize("kb", [4, 8])
+def test_grouped_keys_with_gaps(bpt, oracle, kb):
+    """Keys in two dense clusters at both ends of the key space and a few in
+    between: slices whose prefix tables jump over empty key ranges."""
+    rng = np.random.default_rng(11 + kb)
+    kdt = np.uint32 if kb == 4 else np.uint64
+    top = int(np.iinfo(kdt).max)
+    keys = np.unique(np.concatenate([
+        np.arange(0, 300000, dtype=np.uint64), np.uint64(top) - np.arange(0, 300000, dtype=np.uint64),
+        rng.integers(1 << 20, top >> 1, size=40, dtype=np.uint64)])).astype(kdt)
+    vals = rng.integers(0, top, size=len(keys), dtype=np.uint64).astype(kdt)
+    t = bpt.Tree(keys, vals, kb, kb)
+    q = np.concatenate([keys[rng.integers(0, len(keys), size=200000)],
+                        rng.integers(0, top, size=50000, dtype=np.uint64).astype(kdt),
+                        np.array([0, top, 300000, top - 300000], dtype=np.uint64).astype(kdt)])
+    want = _expect(oracle, keys, vals, q, kb)
+    for sub in (1, 0):
+        t.set_option("group_sub", sub)
+        assert np.array_equal(t.point_search(q, bpt.GROUPED), want), sub
+    t.close()
+
+
 def test_grouped_skewed_queries(bpt, oracle):
     """Queries piled into a few buckets, below the smallest and above the
     largest key, and repeated: runs of very uneven length."""
