@@ -40,21 +40,40 @@ enum : uint32_t { kModelEmpty = 1, kModelFallback = 2 };
 
 struct LeafModel {  // 48 bytes
     uint64_t k0, k1;
-    double p0, slope;
+    uint32_t p0;     // position of the first key routed here
+    float slope;     // positions per 2^shift key units
     int32_t elo, ehi;
     uint32_t flags, cnt;
+    uint32_t shift, pad_;  // keys are compared in units of 2^shift (spans of 2^47 and more)
 };
 
-// The lookup's copy of a LeafModel, 32 bytes: the base position fits 32 bits
-// (n < 2^31), the error bounds 16 bits (a model whose bounds do not is
-// answered by the fallback). Empty: k0 > k1. Fallback: elo > ehi.
-struct PackedModel {
-    uint64_t k0, k1;
-    double slope;
-    uint32_t p0;
-    int16_t elo, ehi;
+// The lookup's copy of a LeafModel, 16 bytes (65536 models: 1 MB, small
+// enough to stay in a CU's L2 beside the tree's traffic):
+//   w[0]      bits 0..31 of k0 >> shift
+//   w[1]      bits 0..15: bits 32..47 of k0 >> shift; bits 16..20: shift;
+//             bits 21..31: the window width ehi - elo rounded up to
+//             m * 2^e (7-bit m, 4-bit e), so widths below 128 are exact
+//   w[2]      p0 + elo, the window's first slot at offset 0 (int32);
+//             INT32_MIN: a fallback model; INT32_MAX: an empty one (a miss)
+//   w[3]      slope (float bits)
+// shift makes a model's span in key units below 2^47, so for its own keys
+// delta = ((key >> shift) - (k0 >> shift)) mod 2^48 is exact.
+struct alignas(16) PackedModel {
+    uint32_t w[4];
 };
-static_assert(sizeof(PackedModel) == 32, "32-byte packed leaf model");
+static_assert(sizeof(PackedModel) == 16, "16-byte packed leaf model");
+constexpr uint64_t kDeltaMask = (1ull << 48) - 1;
+constexpr int32_t kPackedFallback = INT32_MIN;
+constexpr int32_t kPackedEmpty = INT32_MAX;
+constexpr uint32_t kMaxPackedWidth = 127u << 15;
+
+// Width code of w[1] bits 21..31: the least m * 2^e >= w with m < 128.
+__host__ __device__ inline uint32_t width_code(uint32_t w) {
+    uint32_t e = 0;
+    while (((w + (1u << e) - 1) >> e) > 127u) ++e;
+    return ((w + (1u << e) - 1) >> e) | (e << 7);
+}
+__host__ __device__ inline uint32_t width_of(uint32_t code) { return (code & 127u) << (code >> 7); }
 
 struct LearnedView {
     const PackedModel *models;
@@ -243,15 +262,17 @@ __device__ __forceinline__ float mlp_forward(const float *P, float x) {
 }
 
 // LDS image: root 0: [bx nbreak][alpha nbreak+1][beta nbreak+1];
-// root 1: [params kNP]; roots 2 and 3 need none.
-template <class K>
+// root 1: [params kNP]; roots 2 and 3 need none. LAYERWISE = false leaves the
+// layer-by-layer network (root 1) out of the code, and with it the registers
+// its hidden layer takes; such callers never see root 1.
+template <class K, bool LAYERWISE = true>
 __device__ __forceinline__ uint32_t route(const LearnedView &v, const float *sh, K key) {
     const float x = key_feature<K>(key);
     float y;
     if (v.root == 0) {
         const uint32_t s = v.nbreak ? lower_bound_flat<float>(sh, v.nbreak, x) : 0u;
         y = fmaf(sh[v.nbreak + s], x, sh[2 * v.nbreak + 1 + s]);
-    } else if (v.root == 1) {
+    } else if (LAYERWISE && v.root == 1) {
         y = mlp_forward(sh, x);
     } else if (v.root == 2) {
         y = x;
@@ -262,16 +283,34 @@ __device__ __forceinline__ uint32_t route(const LearnedView &v, const float *sh,
     return f <= 0.f ? 0u : (f >= (float)(v.nmodels - 1) ? v.nmodels - 1 : (uint32_t)f);
 }
 
+// Offset of a key from its model's first position: floor(slope * delta),
+// delta = (key - k0) mod 2^48, clamped to +-2^40. One FP32 multiply and a
+// floor, computed by this one function at fit time and at query time, so the
+// error bounds hold exactly for the query-time arithmetic.
+__device__ __forceinline__ int64_t model_offset(float slope, uint64_t delta) {
+    float x = floorf(slope * (float)delta);
+    x = fminf(fmaxf(x, -1099511627776.0f), 1099511627776.0f);
+    return (int64_t)x;
+}
+
 __device__ __forceinline__ int64_t model_pred(const LeafModel &m, uint64_t key) {
-    return (int64_t)floor(m.p0 + m.slope * (double)(key - m.k0));
+    return (int64_t)m.p0 + model_offset(m.slope, ((key >> m.shift) - (m.k0 >> m.shift)) & kDeltaMask);
 }
 
-// Same prediction from the packed copy ((double)p0 is exact: p0 < 2^31).
-__device__ __forceinline__ int64_t model_pred(const PackedModel &m, uint64_t key) {
-    return (int64_t)floor((double)m.p0 + m.slope * (double)(key - m.k0));
-}
+__device__ __forceinline__ bool packed_fallback(const PackedModel &m) { return (int32_t)m.w[2] == kPackedFallback; }
 
-__device__ __forceinline__ bool packed_fallback(const PackedModel &m) { return m.elo > m.ehi; }
+// The window [lo, hi] in slots of `key` under packed model m (lo > hi: none;
+// an empty model's window starts past every tree).
+__device__ __forceinline__ void packed_window(const PackedModel &m, uint64_t key, uint64_t n, int64_t *lo,
+                                              int64_t *hi) {
+    const uint32_t sh = (m.w[1] >> 16) & 31u;
+    const uint64_t k0s = (uint64_t)m.w[0] | ((uint64_t)(m.w[1] & 0xFFFFu) << 32);
+    const int64_t a =
+        (int64_t)(int32_t)m.w[2] + model_offset(__uint_as_float(m.w[3]), ((key >> sh) - k0s) & kDeltaMask);
+    const int64_t b = a + (int64_t)width_of(m.w[1] >> 21);
+    *lo = a < 0 ? 0 : a;
+    *hi = b > (int64_t)n - 1 ? (int64_t)n - 1 : b;
+}
 
 __device__ void stage_learned(const LearnedView &v, float *sh) {
     if (v.root == 0) {
@@ -323,8 +362,11 @@ __global__ void k_fit_models(TreeView t, uint32_t nmodels, const uint32_t *__res
         const uint64_t a = pmin[j], b = pmax[j];
         m.k0 = (uint64_t)slot_key<K, V, LF>(t, a);
         m.k1 = (uint64_t)slot_key<K, V, LF>(t, b);
-        m.p0 = (double)a;
-        m.slope = m.k1 > m.k0 ? (double)(b - a) / (double)(m.k1 - m.k0) : 0.0;
+        m.p0 = (uint32_t)a;
+        const uint64_t span = m.k1 - m.k0;
+        m.shift = span >> 47 ? 64u - (uint32_t)__clzll((long long)span) - 47u : 0u;  // span >> shift < 2^47
+        const uint64_t ds = (m.k1 >> m.shift) - (m.k0 >> m.shift);
+        m.slope = ds ? (float)((double)(b - a) / (double)ds) : 0.0f;
         m.elo = 0x7fffffff;
         m.ehi = -0x7fffffff;
     }
@@ -350,7 +392,7 @@ __global__ void k_finalize_models(LeafModel *__restrict__ models, uint32_t nmode
     LeafModel &m = models[j];
     if (m.flags & kModelEmpty) return;
     const uint64_t w = (uint64_t)((int64_t)m.ehi - (int64_t)m.elo + 1);
-    if (w > max_window) {
+    if (w > max_window || w - 1 > kMaxPackedWidth) {  // the packed width is at most 127 * 2^15
         m.flags |= kModelFallback;
         atomicAdd(&acc[0], 1ull);
         atomicAdd(&acc[1], (unsigned long long)m.cnt);
@@ -360,29 +402,28 @@ __global__ void k_finalize_models(LeafModel *__restrict__ models, uint32_t nmode
     atomicAdd(&acc[4], 1ull);
 }
 
-// Also counts the packed models the lookup sends to the fallback pass (those
-// over the window cap and those whose bounds exceed 16 bits) in *fallbacks.
+// Also counts the packed models the lookup sends to the fallback pass in
+// *fallbacks.
 __global__ void k_pack_models(const LeafModel *__restrict__ models, uint32_t nmodels, PackedModel *__restrict__ out,
                               unsigned long long *__restrict__ fallbacks) {
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= nmodels) return;
     const LeafModel m = models[j];
     PackedModel p;
-    if (m.flags & kModelEmpty) {
-        p.k0 = ~0ull;
-        p.k1 = 0;
-        p.slope = 0.0;
-        p.p0 = 0;
-        p.elo = p.ehi = 0;
-    } else {
-        p.k0 = m.k0;
-        p.k1 = m.k1;
-        p.slope = m.slope;
-        p.p0 = (uint32_t)m.p0;
-        const bool fb = (m.flags & kModelFallback) || m.elo < -32767 || m.ehi > 32767;
-        p.elo = fb ? (int16_t)32767 : (int16_t)m.elo;
-        p.ehi = fb ? (int16_t)-32768 : (int16_t)m.ehi;
-        if (fb) atomicAdd(fallbacks, 1ull);
+    p.w[0] = p.w[1] = p.w[3] = 0;
+    p.w[2] = (uint32_t)kPackedEmpty;
+    if (!(m.flags & kModelEmpty)) {
+        if (m.flags & kModelFallback) {
+            p.w[2] = (uint32_t)kPackedFallback;
+            atomicAdd(fallbacks, 1ull);
+        } else {
+            const uint64_t k0s = m.k0 >> m.shift;
+            p.w[0] = (uint32_t)k0s;
+            p.w[1] = (uint32_t)((k0s >> 32) & 0xFFFFu) | (m.shift << 16) |
+                     (width_code((uint32_t)(m.ehi - m.elo)) << 21);
+            p.w[2] = (uint32_t)((int32_t)m.p0 + m.elo);  // elo <= 0: the first key's error is 0
+            p.w[3] = __float_as_uint(m.slope);
+        }
     }
     out[j] = p;
 }
@@ -404,11 +445,9 @@ __global__ __launch_bounds__(256) void k_point_learned(TreeView t, LearnedView v
         V res = sentinel;
         if (packed_fallback(m)) {
             res = leaf_lookup<K, V, LF>(t, descend<K, F>(t, lds, key), key, sentinel);
-        } else if ((uint64_t)key >= m.k0 && (uint64_t)key <= m.k1) {
-            const int64_t pred = model_pred(m, (uint64_t)key);
-            int64_t lo = pred + m.elo, hi = pred + m.ehi;
-            lo = lo < 0 ? 0 : lo;
-            hi = hi > (int64_t)t.n - 1 ? (int64_t)t.n - 1 : hi;
+        } else {
+            int64_t lo, hi;
+            packed_window(m, (uint64_t)key, t.n, &lo, &hi);
             if (lo <= hi) {
                 // branch-free lower bound over slots [lo, hi]
                 uint64_t base = (uint64_t)lo, len = (uint64_t)(hi - lo + 1);
@@ -426,43 +465,41 @@ __global__ __launch_bounds__(256) void k_point_learned(TreeView t, LearnedView v
 }
 
 // Cooperative form (equal key/value widths). Phase A, one lane per query:
-// route, model and window [lo, hi] in slots -> leaf range [la, lb]; queries of
-// fallback models are queued for the plain descent. Phase B, eight lanes per
-// query, the group's eight queries in flight together: the window's level-1
-// nodes [la/F, lb/F] are ranked by their maxima (the level-2 entries, two per
-// lane: a window of 256 slots spans a few nodes, one line), which picks the
-// node; one cooperative step through that node picks the leaf, and the leaves
-// are matched as in the plain cooperative descent. A window thus costs one
-// line of node maxima instead of a line per 16 leaves of leaf maxima. Trees
-// of one internal level rank the leaf maxima over [la, lb] directly. The
-// fallback descent reads every level from global memory, so the kernel's LDS
-// holds only the root's tables.
-template <class K, class V, int F, int LF>
+// route, model and window [lo, hi] in slots -> leaf range [la, lb] -> level-1
+// node range [ja, jb]; queries of fallback models are queued for the plain
+// descent. A window over several level-1 nodes picks its node by counting
+// the maxima of nodes ja..jb-1 (the level-2 entries, contiguous) below the
+// query: the lane reads the first two itself, both loads in flight together,
+// and searches on only for windows over more than three nodes. A window inside
+// one node reads no maxima. Phase B is the plain descent's cooperative step
+// from that level-1 node: eight lanes per query, the group's eight queries in
+// flight together, through the node and then the leaf. Phase A keeps only the
+// lane's own query state, so the kernel holds as many waves per CU as the
+// plain descent. The fallback descent reads every level from global memory,
+// so the kernel's LDS holds only the root's tables.
+// LAYERWISE selects the root-1 form of route (the network evaluated layer by
+// layer), whose hidden layer would cost the other roots two waves per SIMD.
+template <class K, class V, int F, int LF, bool NT, bool OFF32, bool LAYERWISE>
 __global__ __launch_bounds__(256) void k_point_learned_coop(TreeView t, LearnedView v, const K *__restrict__ q,
                                                             V *__restrict__ out, uint64_t nq, V sentinel,
-                                                            uint32_t lds_tree_bytes, uint32_t *__restrict__ fb_count,
+                                                            uint32_t *__restrict__ fb_count,
                                                             uint32_t *__restrict__ fb_list, uint32_t fb_cap) {
     static_assert(sizeof(K) == sizeof(V), "cooperative leaf needs equal key/value widths");
-    constexpr uint32_t SPL = 4;  // separators per lane, 32 leaves per group step
     extern __shared__ uint4 lds_raw[];
-    uint8_t *lds = reinterpret_cast<uint8_t *>(lds_raw);
-    float *sh = reinterpret_cast<float *>(lds + lds_tree_bytes);
+    float *sh = reinterpret_cast<float *>(lds_raw);
     __shared__ uint32_t fb_n;  // this block's fallback queries (its own segment of fb_list)
     if (threadIdx.x == 0) fb_n = 0;
-    stage_top_levels(t, lds);
     stage_learned(v, sh);
     uint32_t *my_list = fb_list + (uint64_t)blockIdx.x * fb_cap;
-    const uint32_t lane = threadIdx.x & 63, g = lane >> 3, sub = lane & 7;
+    const uint32_t lane = threadIdx.x & 63;
     const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-    const K *seps = reinterpret_cast<const K *>(t.inner + t.off[1]);  // leaf maxima
-    const uint32_t nleaves = (uint32_t)t.count[0];
+    // level-2 entries: the maxima of the level-1 nodes (trees of height >= 2)
+    const K *nmax = reinterpret_cast<const K *>(t.inner + t.off[t.height >= 2 ? 2 : 1]);
     for (uint64_t base = wave * 64; base < nq; base += nwaves * 64) {
         const uint64_t my = base + lane;
         const K myq = my < nq ? __builtin_nontemporal_load(&q[my]) : K(0);
-        // phase A: la > lb marks a miss (or a query handed to the fallback pass)
-        const PackedModel m = v.models[route<K>(v, sh, myq)];
-        uint32_t la = 1, lb = 0;
+        const PackedModel m = v.models[route<K, LAYERWISE>(v, sh, myq)];
         const bool fb = my < nq && packed_fallback(m);
         const uint64_t fbm = __ballot(fb);
         if (fbm) {  // one LDS atomic per wave appends this wave's fallback queries
@@ -471,112 +508,37 @@ __global__ __launch_bounds__(256) void k_point_learned_coop(TreeView t, LearnedV
             basei = __shfl(basei, __ffsll((long long)fbm) - 1, 64);
             if (fb) my_list[basei + __popcll(fbm & ((1ull << lane) - 1))] = (uint32_t)my;
         }
-        if (fb) {
-            // answered by the fallback pass
-        } else if ((uint64_t)myq >= m.k0 && (uint64_t)myq <= m.k1) {
-            const int64_t pred = model_pred(m, (uint64_t)myq);
-            int64_t lo = pred + m.elo, hi = pred + m.ehi;
-            lo = lo < 0 ? 0 : lo;
-            hi = hi > (int64_t)t.n - 1 ? (int64_t)t.n - 1 : hi;
+        // an empty window (a miss) and a queued query walk node 0 with the group
+        bool live = false;
+        uint32_t node = 0;
+        if (!fb && my < nq) {
+            int64_t lo, hi;
+            packed_window(m, (uint64_t)myq, t.n, &lo, &hi);
             if (lo <= hi) {
-                la = (uint32_t)(lo / LF);
-                lb = (uint32_t)(hi / LF);
-            }
-        }
-        K qk[8];
-        uint32_t a[8], b[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            qk[k] = __shfl(myq, (int)(g * 8 + k), 64);
-            a[k] = (uint32_t)__shfl((int)la, (int)(g * 8 + k), 64);
-            b[k] = (uint32_t)__shfl((int)lb, (int)(g * 8 + k), 64);
-        }
-        uint32_t leaf[8];
-        if (t.height >= 2) {
-            // phase B.1: the window's level-1 nodes ranked by their maxima (the
-            // level-2 entries: a few, in one line), then one cooperative step
-            // through the chosen level-1 node, as in the plain descent
-            constexpr uint32_t EPL = 2;  // node maxima per lane, 16 per group step
-            constexpr int NB = F * (int)sizeof(K) / 8;
-            const K *nmax = reinterpret_cast<const K *>(t.inner + t.off[2]);
-            const uint32_t nn = (uint32_t)t.count[1];
-            K ev[8][EPL];
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const uint32_t ja = a[k] / F, jb = b[k] / F;
-#pragma unroll
-                for (uint32_t j = 0; j < EPL; ++j) {
-                    const uint32_t e = ja + sub * EPL + j;
-                    // a window inside one level-1 node needs no maxima
-                    ev[k][j] = (a[k] <= b[k] && ja < jb && e <= jb && e < nn) ? nmax[e] : key_max<K>();
-                }
-            }
-            uint32_t node[8];
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const uint32_t ja = a[k] / F, jb = b[k] / F;
-                uint32_t c = 0;
-#pragma unroll
-                for (uint32_t j = 0; j < EPL; ++j) c += (uint32_t)(ev[k][j] < qk[k]) & (uint32_t)(ja + sub * EPL + j <= jb);
-                uint32_t r = group8_sum(c);
-                uint32_t jj = ja;
-                while (r == 8 * EPL && jj + 8 * EPL <= jb) {  // windows over more than 16 level-1 nodes
-                    jj += 8 * EPL;
-                    c = 0;
-                    for (uint32_t j = 0; j < EPL; ++j) {
-                        const uint32_t e = jj + sub * EPL + j;
-                        c += (e <= jb && e < nn) ? (uint32_t)(nmax[e] < qk[k]) : 0u;
+                live = true;
+                const uint32_t ja = (uint32_t)(lo / LF) / F, jb = (uint32_t)(hi / LF) / F;
+                node = ja;
+                if (ja < jb) {
+                    const K e0 = nmax[ja];
+                    const K e1 = ja + 1 < jb ? nmax[ja + 1] : key_max<K>();
+                    node = ja + (uint32_t)(e0 < myq) + (uint32_t)(e1 < myq);
+                    if (node == ja + 2 && node < jb) {  // wider windows: lower bound over the rest
+                        uint32_t len = jb - node;
+                        while (len > 0) {
+                            const uint32_t half = len >> 1;
+                            if (nmax[node + half] < myq) {
+                                node += half + 1;
+                                len -= half + 1;
+                            } else {
+                                len = half;
+                            }
+                        }
                     }
-                    r = group8_sum(c);
                 }
-                const uint32_t nd = jj + r;
-                node[k] = a[k] > b[k] ? 0u : (ja == jb ? ja : (nd > jb ? jb : nd));
-            }
-            const uint8_t *lv = t.inner + t.off[1];
-            Chunk<NB> c1[8];
-#pragma unroll
-            for (int k = 0; k < 8; ++k) load_chunk<NB>(lv + (uint64_t)node[k] * (F * sizeof(K)) + sub * NB, c1[k]);
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const uint32_t ch = node[k] * F + group8_sum(chunk_count_lt<K, NB>(c1[k], qk[k]));
-                leaf[k] = ch < nleaves - 1 ? ch : nleaves - 1;
-            }
-        } else {
-            // phase B.1 of a one-level tree: rank among the leaf maxima over the
-            // window (all eight queries' loads in flight)
-            K sv[8][SPL];
-    #pragma unroll
-            for (int k = 0; k < 8; ++k) {
-    #pragma unroll
-                for (uint32_t j = 0; j < SPL; ++j) {
-                    const uint32_t l = a[k] + sub * SPL + j;
-                    sv[k][j] = (l <= b[k] && l < nleaves) ? seps[l] : key_max<K>();
-                }
-            }
-    #pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                uint32_t c = 0;
-    #pragma unroll
-                for (uint32_t j = 0; j < SPL; ++j) c += (uint32_t)(sv[k][j] < qk[k]) & (uint32_t)(a[k] + sub * SPL + j <= b[k]);
-                uint32_t r = group8_sum(c);
-                uint32_t aa = a[k];
-                while (r == 8 * SPL && aa + 8 * SPL <= b[k]) {  // windows wider than 64 leaves
-                    aa += 8 * SPL;
-                    c = 0;
-                    for (uint32_t j = 0; j < SPL; ++j) {
-                        const uint32_t l = aa + sub * SPL + j;
-                        c += (l <= b[k] && l < nleaves) ? (uint32_t)(seps[l] < qk[k]) : 0u;
-                    }
-                    r = group8_sum(c);
-                }
-                const uint32_t lf = aa + r;
-                leaf[k] = a[k] > b[k] ? 0u : (lf > b[k] ? b[k] : lf);
             }
         }
-        // phase B.2: leaf lines (an empty window, la > lb, is a miss)
-        const V got = coop_leaf<K, V, LF, true, kOpValue, 8, false>(t, qk, leaf, sentinel);
-        const V res = la <= lb ? got : sentinel;
-        if (my < nq && !fb) __builtin_nontemporal_store(res, &out[my]);
+        const V got = coop_finish64<K, V, F, LF, NT, kOpValue, 8, OFF32>(t, myq, node, 1, sentinel);
+        if (my < nq && !fb) __builtin_nontemporal_store(live ? got : sentinel, &out[my]);
     }
     __syncthreads();
     if (threadIdx.x == 0) fb_count[blockIdx.x] = fb_n;
@@ -865,8 +827,19 @@ bpt_status point_learned(const bpt_tree *t, const void *d_q, uint64_t nq, void *
                 uint32_t *fb_count = static_cast<uint32_t *>(d_ws);
                 uint32_t *fb_list = reinterpret_cast<uint32_t *>(static_cast<uint8_t *>(d_ws) +
                                                                  ((uint64_t)blocks * 4 + 255) / 256 * 256);
-                k_point_learned_coop<K, V, F, LF><<<blocks, t->block, lds, s>>>(
-                    gv, t->learned->view, (const K *)d_q, (V *)d_out, nq, ~V(0), 0, fb_count, fb_list, cap);
+                const bool off32 = t->addr32 && t->leaf_bytes <= (1ull << 32) && t->inner_bytes <= (1ull << 32);
+                auto go = [&](auto nt, auto o32, auto lw) {
+                    k_point_learned_coop<K, V, F, LF, decltype(nt)::value, decltype(o32)::value, decltype(lw)::value>
+                        <<<blocks, t->block, lds, s>>>(gv, t->learned->view, (const K *)d_q, (V *)d_out, nq, ~V(0),
+                                                       fb_count, fb_list, cap);
+                };
+                auto with_o32 = [&](auto nt) {
+                    if (t->learned->view.root == 1) go(nt, std::false_type{}, std::true_type{});
+                    else if (off32) go(nt, std::true_type{}, std::false_type{});
+                    else go(nt, std::false_type{}, std::false_type{});
+                };
+                if (t->leaf_nt) with_o32(std::true_type{});
+                else with_o32(std::false_type{});
                 if (t->learned->packed_fallbacks) {
                     const size_t tl = t->view.lds_bytes ? t->view.lds_bytes : 16;
                     k_point_coop_list<K, V, F, LF><<<blocks, t->block, tl, s>>>(
