@@ -466,47 +466,66 @@ __global__ __launch_bounds__(256) void k_point_learned(TreeView t, LearnedView v
 
 // Cooperative form (equal key/value widths). Phase A, one lane per query:
 // route, model and window [lo, hi] in slots -> leaf range [la, lb] -> level-1
-// node range [ja, jb]; queries of fallback models are queued for the plain
-// descent. A window over several level-1 nodes picks its node by counting
-// the maxima of nodes ja..jb-1 (the level-2 entries, contiguous) below the
-// query: the lane reads the first two itself, both loads in flight together,
-// and searches on only for windows over more than three nodes. A window inside
-// one node reads no maxima. Phase B is the plain descent's cooperative step
-// from that level-1 node: eight lanes per query, the group's eight queries in
-// flight together, through the node and then the leaf. Phase A keeps only the
-// lane's own query state, so the kernel holds as many waves per CU as the
-// plain descent. The fallback descent reads every level from global memory,
-// so the kernel's LDS holds only the root's tables.
+// node range [ja, jb]. A window over several level-1 nodes picks its node by
+// counting the maxima of nodes ja..jb-1 (the level-2 entries, contiguous)
+// below the query: the lane reads the first two itself, both loads in flight
+// together, and searches on only for windows over more than three nodes. A
+// window inside one node reads no maxima. Phase B is the plain descent's
+// cooperative step from that level-1 node: eight lanes per query, the group's
+// eight queries in flight together, through the node and then the leaf.
+// Phase A keeps only the lane's own query state, so the kernel holds as many
+// waves per CU as the plain descent.
+// Queries of fallback models are queued by their wave in its own LDS queue
+// (key and position); whenever 64 are queued the wave answers them with the
+// plain cooperative descent from the LDS-resident top levels, all 64 lanes
+// busy, and the rest at its end. No second pass re-reads them from memory.
+// LDS: top levels [lds_bytes] | root tables | per wave: keys[kFbQ], positions[kFbQ]
+constexpr uint32_t kFbQ = 128;  // queue entries per wave (at most 63 + 64 at once)
+
+template <class K>
+__host__ __device__ constexpr size_t fbq_bytes_per_wave() { return kFbQ * (sizeof(K) + sizeof(uint32_t)); }
+
 // LAYERWISE selects the root-1 form of route (the network evaluated layer by
 // layer), whose hidden layer would cost the other roots two waves per SIMD.
 template <class K, class V, int F, int LF, bool NT, bool OFF32, bool LAYERWISE>
 __global__ __launch_bounds__(256) void k_point_learned_coop(TreeView t, LearnedView v, const K *__restrict__ q,
                                                             V *__restrict__ out, uint64_t nq, V sentinel,
-                                                            uint32_t *__restrict__ fb_count,
-                                                            uint32_t *__restrict__ fb_list, uint32_t fb_cap) {
+                                                            uint32_t root_off, uint32_t fbq_off) {
     static_assert(sizeof(K) == sizeof(V), "cooperative leaf needs equal key/value widths");
     extern __shared__ uint4 lds_raw[];
-    float *sh = reinterpret_cast<float *>(lds_raw);
-    __shared__ uint32_t fb_n;  // this block's fallback queries (its own segment of fb_list)
-    if (threadIdx.x == 0) fb_n = 0;
+    uint8_t *lds = reinterpret_cast<uint8_t *>(lds_raw);
+    float *sh = reinterpret_cast<float *>(lds + root_off);
+    const uint32_t lane = threadIdx.x & 63, wib = threadIdx.x >> 6;
+    K *fbk = reinterpret_cast<K *>(lds + fbq_off + wib * fbq_bytes_per_wave<K>());
+    uint32_t *fbp = reinterpret_cast<uint32_t *>(fbk + kFbQ);
+    stage_top_levels(t, lds);
     stage_learned(v, sh);
-    uint32_t *my_list = fb_list + (uint64_t)blockIdx.x * fb_cap;
-    const uint32_t lane = threadIdx.x & 63;
     const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
     // level-2 entries: the maxima of the level-1 nodes (trees of height >= 2)
     const K *nmax = reinterpret_cast<const K *>(t.inner + t.off[t.height >= 2 ? 2 : 1]);
+    uint32_t nfb = 0;  // queued fallback queries of this wave (the same in every lane)
+    // answers the first cnt (<= 64) queued queries, one per lane
+    auto drain = [&](uint32_t cnt) {
+        const bool has = lane < cnt;
+        const K key = has ? fbk[lane] : K(0);
+        const uint32_t pos = has ? fbp[lane] : 0u;
+        const V r = coop_lookup64<K, V, F, LF, NT, kOpValue, 8, OFF32>(t, lds, key, sentinel);
+        if (has) __builtin_nontemporal_store(r, &out[pos]);
+    };
     for (uint64_t base = wave * 64; base < nq; base += nwaves * 64) {
         const uint64_t my = base + lane;
         const K myq = my < nq ? __builtin_nontemporal_load(&q[my]) : K(0);
         const PackedModel m = v.models[route<K, LAYERWISE>(v, sh, myq)];
         const bool fb = my < nq && packed_fallback(m);
         const uint64_t fbm = __ballot(fb);
-        if (fbm) {  // one LDS atomic per wave appends this wave's fallback queries
-            uint32_t basei = 0;
-            if (lane == (uint32_t)(__ffsll((long long)fbm) - 1)) basei = atomicAdd(&fb_n, (uint32_t)__popcll(fbm));
-            basei = __shfl(basei, __ffsll((long long)fbm) - 1, 64);
-            if (fb) my_list[basei + __popcll(fbm & ((1ull << lane) - 1))] = (uint32_t)my;
+        if (fbm) {
+            if (fb) {
+                const uint32_t slot = nfb + (uint32_t)__popcll(fbm & ((1ull << lane) - 1));
+                fbk[slot] = myq;
+                fbp[slot] = (uint32_t)my;
+            }
+            nfb += (uint32_t)__popcll(fbm);
         }
         // an empty window (a miss) and a queued query walk node 0 with the group
         bool live = false;
@@ -539,33 +558,19 @@ __global__ __launch_bounds__(256) void k_point_learned_coop(TreeView t, LearnedV
         }
         const V got = coop_finish64<K, V, F, LF, NT, kOpValue, 8, OFF32>(t, myq, node, 1, sentinel);
         if (my < nq && !fb) __builtin_nontemporal_store(live ? got : sentinel, &out[my]);
+        if (nfb >= 64) {
+            drain(64);
+            const uint32_t rest = nfb - 64;  // < 64: moved to the front of the queue
+            const K k2 = lane < rest ? fbk[64 + lane] : K(0);
+            const uint32_t p2 = lane < rest ? fbp[64 + lane] : 0u;
+            if (lane < rest) {
+                fbk[lane] = k2;
+                fbp[lane] = p2;
+            }
+            nfb = rest;
+        }
     }
-    __syncthreads();
-    if (threadIdx.x == 0) fb_count[blockIdx.x] = fb_n;
-}
-
-// Fallback pass: block b runs the plain cooperative descent over the queries
-// block b of the learned kernel queued (same grid, one segment per block).
-template <class K, class V, int F, int LF>
-__global__ __launch_bounds__(256) void k_point_coop_list(TreeView t, const K *__restrict__ q, V *__restrict__ out,
-                                                         const uint32_t *__restrict__ count,
-                                                         const uint32_t *__restrict__ fb_list, uint32_t fb_cap,
-                                                         V sentinel) {
-    extern __shared__ uint4 lds_raw[];
-    uint8_t *lds = reinterpret_cast<uint8_t *>(lds_raw);
-    const uint32_t n = count[blockIdx.x];
-    if (n == 0) return;  // the whole block
-    stage_top_levels(t, lds);
-    const uint32_t *list = fb_list + (uint64_t)blockIdx.x * fb_cap;
-    const uint32_t lane = threadIdx.x & 63;
-    const uint32_t w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    for (uint64_t base = (uint64_t)w * 64; base < n; base += (uint64_t)nw * 64) {
-        const uint64_t j = base + lane;
-        const uint32_t idx = j < n ? list[j] : 0u;
-        const K key = j < n ? q[idx] : K(0);
-        const V r = coop_lookup64<K, V, F, LF, true, kOpValue>(t, lds, key, sentinel);
-        if (j < n) out[idx] = r;
-    }
+    if (nfb) drain(nfb);
 }
 
 // ------------------------------------------------------------------ host side
@@ -789,24 +794,13 @@ static bpt_status learned_build_impl(bpt_tree *t, const bpt_learned_config &cfg,
     return st;
 }
 
-// Fallback queues of the cooperative kernel: one count and one segment of
-// `cap` indices per block of its persistent grid.
-static void fb_plan(const bpt_tree *t, uint64_t nq, uint32_t *blocks, uint32_t *cap) {
-    *blocks = search_grid(t, nq);
-    const uint64_t per_iter = (uint64_t)(*blocks) * t->block;
-    *cap = (uint32_t)(((nq + per_iter - 1) / per_iter) * t->block);
-}
-
-uint64_t point_learned_ws(const bpt_tree *t, uint64_t nq) {
-    if (!t || t->key_bytes != t->val_bytes || nq == 0) return 0;
-    uint32_t blocks, cap;
-    fb_plan(t, nq, &blocks, &cap);
-    return ((uint64_t)blocks * 4 + 255) / 256 * 256 + (uint64_t)blocks * cap * 4;
-}
+uint64_t point_learned_ws(const bpt_tree *, uint64_t) { return 0; }  // fallback queues live in LDS
 
 bpt_status point_learned(const bpt_tree *t, const void *d_q, uint64_t nq, void *d_out, void *d_ws, uint64_t ws_bytes,
                          hipStream_t s) {
     BPT_CHECK(t->learned, BPT_ERR_UNSUPPORTED, "no learned model attached to this tree (bpt_learned_build)");
+    (void)d_ws;  // no scratch: fallback queues live in LDS
+    (void)ws_bytes;
     if (nq == 0) return BPT_OK;
     if (t->n == 0) return launch_plain(t, d_q, nq, d_out, s);
     return dispatch_tree(t, [&](auto kt, auto vt, auto fc, auto lc) -> bpt_status {
@@ -819,19 +813,14 @@ bpt_status point_learned(const bpt_tree *t, const void *d_q, uint64_t nq, void *
         const size_t lds = learned_lds(t->learned->view);
         if constexpr (sizeof(K) == sizeof(V)) {
             if (t->coop && t->view.height >= 1 && t->view.count[0] < (1ull << 32) && nq < (1ull << 32)) {
-                BPT_CHECK(d_ws && ws_bytes >= point_learned_ws(t, nq), BPT_ERR_INVALID,
-                          "learned search needs a workspace of %llu bytes",
-                          (unsigned long long)point_learned_ws(t, nq));
-                uint32_t blocks, cap;
-                fb_plan(t, nq, &blocks, &cap);
-                uint32_t *fb_count = static_cast<uint32_t *>(d_ws);
-                uint32_t *fb_list = reinterpret_cast<uint32_t *>(static_cast<uint8_t *>(d_ws) +
-                                                                 ((uint64_t)blocks * 4 + 255) / 256 * 256);
+                const uint32_t root_off = (t->view.lds_bytes + 15) / 16 * 16;
+                const uint32_t fbq_off = root_off + (uint32_t)((lds + 15) / 16 * 16);
+                const size_t lds_all = fbq_off + (size_t)(t->block / 64) * fbq_bytes_per_wave<K>();
                 const bool off32 = t->addr32 && t->leaf_bytes <= (1ull << 32) && t->inner_bytes <= (1ull << 32);
                 auto go = [&](auto nt, auto o32, auto lw) {
                     k_point_learned_coop<K, V, F, LF, decltype(nt)::value, decltype(o32)::value, decltype(lw)::value>
-                        <<<blocks, t->block, lds, s>>>(gv, t->learned->view, (const K *)d_q, (V *)d_out, nq, ~V(0),
-                                                       fb_count, fb_list, cap);
+                        <<<search_grid(t, nq), t->block, lds_all, s>>>(t->view, t->learned->view, (const K *)d_q,
+                                                                        (V *)d_out, nq, ~V(0), root_off, fbq_off);
                 };
                 auto with_o32 = [&](auto nt) {
                     if (t->learned->view.root == 1) go(nt, std::false_type{}, std::true_type{});
@@ -840,11 +829,6 @@ bpt_status point_learned(const bpt_tree *t, const void *d_q, uint64_t nq, void *
                 };
                 if (t->leaf_nt) with_o32(std::true_type{});
                 else with_o32(std::false_type{});
-                if (t->learned->packed_fallbacks) {
-                    const size_t tl = t->view.lds_bytes ? t->view.lds_bytes : 16;
-                    k_point_coop_list<K, V, F, LF><<<blocks, t->block, tl, s>>>(
-                        t->view, (const K *)d_q, (V *)d_out, fb_count, fb_list, cap, ~V(0));
-                }
                 BPT_HIP(hipGetLastError());
                 return BPT_OK;
             }

@@ -2,8 +2,8 @@
 // device memory (code under HIP graph capture, tight serving loops). It has
 // no counterpart in the reference.
 //
-// The grouped and learned modes need scratch (partitioned queries, their
-// results, tile counts; fallback lists). bpt_point_search takes it from the
+// The grouped mode needs scratch (partitioned queries, their results, tile
+// counts); the plain and learned modes need none. bpt_point_search takes it from the
 // tree's workspace, which grows with hipMalloc. This entry point uses a
 // file-scope device array instead: every GPU the library runs on holds its
 // own copy of it until the process ends. Host code never passes the array's

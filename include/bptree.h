@@ -173,8 +173,9 @@ typedef enum bpt_point_mode {
  * Grouped: replaces genGrouping + kernelSearchingKeyGroupTwoStep
  * (grouping.cu:1677-1851, BPTreeExactSearch.cu:206-294); groups on the top
  * `group_bits` key bits (bpt_tree_set_group_bits, default 12).
- * Grouped and learned modes use a workspace owned by the tree: concurrent
- * calls on one tree must share one stream (or use bpt_point_search_ws). */
+ * The grouped mode uses a workspace owned by the tree: concurrent calls on
+ * one tree must share one stream (or use bpt_point_search_ws). Plain and
+ * learned need none (bpt_point_search_ws_size returns 0 for them). */
 bpt_status bpt_point_search(bpt_tree *t, const void *d_q, uint64_t nq, void *d_out, int mode,
                             bpt_stream s);
 /* Same, with a caller-provided workspace of at least bpt_point_search_ws_size bytes. */
@@ -184,9 +185,9 @@ bpt_status bpt_point_search_ws(const bpt_tree *t, const void *d_q, uint64_t nq, 
 bpt_status bpt_tree_set_group_bits(bpt_tree *t, uint32_t bits);
 /* No counterpart in the reference: the same search for callers that cannot
  * allocate device memory (code under HIP graph capture, tight serving
- * loops). It calls no hipMalloc / hipFree. The grouped and learned modes use
- * a static device scratch of BPT_NOALLOC_SCRATCH_BYTES per GPU (held until
- * the process ends) instead of the tree's workspace; a batch whose workspace
+ * loops). It calls no hipMalloc / hipFree. The grouped mode uses a static
+ * device scratch of BPT_NOALLOC_SCRATCH_BYTES per GPU (held until the process
+ * ends) instead of the tree's workspace; a batch whose workspace
  * (bpt_point_search_ws_size) is larger fails with BPT_ERR_CAPACITY (about 6M
  * queries on a 4-byte-key tree, 3M on an 8-byte one). All calls on one GPU
  * share its scratch, so calls that overlap on one GPU (other streams or
