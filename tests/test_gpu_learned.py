@@ -63,6 +63,30 @@ def test_learned_forced_fallback_still_exact(bpt, oracle, lognormal_tree):
     assert np.array_equal(t.point_search(q, bpt.LEARNED), want)
 
 
+@pytest.mark.parametrize("block", [64, 128, 256])
+def test_learned_fallback_queue_shapes(bpt, oracle, lognormal_tree, block):
+    """Queries of fallback models wait in a per-wave LDS queue and are answered
+    64 at a time, the rest at the wave's end. With max_window 4 almost every
+    model falls back, so queues fill and spill over every batch; batch sizes
+    around one and two waves leave partial queues; 1, 2 and 4 waves per block
+    give each block 1, 2 or 4 queues. Every answer equals the oracle, and the
+    learned mode needs no workspace."""
+    t, keys, vals, q = lognormal_tree
+    rng = np.random.default_rng(17)
+    t.set_launch(block=block)
+    try:
+        for max_window in (4, 256):
+            t.learned_build(leaf_models=4096, max_window=max_window, train_steps=100)
+            assert t.point_search_ws_size(1 << 20, bpt.LEARNED) == 0
+            for nq in (1, 63, 64, 65, 129, 1000, 100_003):
+                qq = q[rng.integers(0, len(q), nq)]
+                want = oracle.point_closed(keys, vals, qq, SENT)
+                got = t.point_search(qq, bpt.LEARNED)
+                assert np.array_equal(got, want), (block, max_window, nq)
+    finally:
+        t.set_launch(block=256)
+
+
 def test_trained_network_gains_over_no_network(bpt, lognormal_tree):
     t, keys, vals, q = lognormal_tree
     trained = t.learned_build(leaf_models=16384, train_steps=600, root=0)
