@@ -212,6 +212,29 @@ __global__ __launch_bounds__(kBBlock) void k_bkt_btab(TreeView t, BucketDev d, c
     }
 }
 
+// Entries per bucket in the table of slice prefix tables: 2^sbits + 1,
+// rounded up to whole 16-byte pieces (room for one word past the table).
+__host__ __device__ inline uint32_t stab_stride(uint32_t sbits) { return ((1u << sbits) + 2 + 7) / 8 * 8; }
+
+// The prefix table of every bucket's search slice, built once per call from
+// the slice source in global memory: stabs[b * stride ...] is bucket b's
+// table, spi[2b], spi[2b + 1] its base and shift. The search blocks copy a
+// bucket's table with its slice instead of each rebuilding it.
+template <class K>
+__global__ __launch_bounds__(kPBlock) void k_bkt_stabs(BucketDev d, const K *__restrict__ level, uint64_t level_count,
+                                                       uint16_t *__restrict__ stabs, uint64_t *__restrict__ spi) {
+    const uint32_t b = blockIdx.x;
+    const uint64_t first = (uint64_t)b * d.epb;
+    const uint32_t eb = (uint32_t)std::min<uint64_t>(d.epb, level_count - first);
+    const K *a = level + first;
+    const PrefixIndex<K> pi = prefix_index<K>(a[0], a[eb - 1], d.sbits);
+    build_prefix_table<K>(a, eb, pi, d.sbits, stabs + (uint64_t)b * stab_stride(d.sbits));
+    if (threadIdx.x == 0) {
+        spi[2 * b] = (uint64_t)pi.base;
+        spi[2 * b + 1] = pi.shift;
+    }
+}
+
 template <class K>
 __global__ __launch_bounds__(kPBlock) void k_bkt_hist(const K *__restrict__ q, uint64_t nq, BucketDev d,
                                                       const K *__restrict__ bsep_g, const uint64_t *__restrict__ hdr,
@@ -482,7 +505,7 @@ __device__ __forceinline__ V subleaf_finish(const SubLeafBatch<K> &b, uint32_t n
     return res;
 }
 
-// LDS: bstart[nb + 1] | slice[epb] | stab[2^sbits + 1]
+// LDS: bstart[nb + 1] | slice[epb] | stab[stab_stride] (16-byte aligned pieces)
 // Work: the bucket-ordered queries in chunks of d.chunk. With d.xcd the
 // chunks are split into 8 contiguous parts, one per XCD, and the blocks of an
 // XCD stride through their part together, so an XCD works on about one bucket
@@ -492,12 +515,14 @@ constexpr int kPieceBatches = 8;   // batches of 64 queries a wave loads at once
 
 template <class K, class V, int F, int LF, bool COOP, int G, bool OFF32, bool SUBL>
 __global__ __launch_bounds__(kMaxSearchBlock) void k_bkt_search(TreeView t, BucketDev d, const K *__restrict__ qs,
-                                                        const uint32_t *__restrict__ bstart_g, uint64_t nq,
+                                                        const uint32_t *__restrict__ bstart_g,
+                                                        const uint16_t *__restrict__ stabs,
+                                                        const uint64_t *__restrict__ spi, uint64_t nq,
                                                         V *__restrict__ res, V sentinel) {
     extern __shared__ uint4 lds_raw[];
     uint32_t *bstart = reinterpret_cast<uint32_t *>(lds_raw);
     K *slice = reinterpret_cast<K *>(lds_raw + (d.nb + 1 + 3) / 4);
-    uint16_t *stab = reinterpret_cast<uint16_t *>(slice + d.epb);
+    uint16_t *stab = reinterpret_cast<uint16_t *>(lds_raw + (d.nb + 1 + 3) / 4 + (d.epb * sizeof(K) + 15) / 16);
     for (uint32_t b = threadIdx.x; b <= d.nb; b += blockDim.x) bstart[b] = bstart_g[b];
     __syncthreads();
     // slice source: the sub-leaf maxima, or level s (its entries = nodes of level s-1)
@@ -545,9 +570,11 @@ __global__ __launch_bounds__(kMaxSearchBlock) void k_bkt_search(TreeView t, Buck
                 const uint4 *src4 = reinterpret_cast<const uint4 *>(level + first);
                 for (uint32_t e = threadIdx.x; e < nvec; e += blockDim.x) reinterpret_cast<uint4 *>(slice)[e] = src4[e];
                 for (uint32_t e = nvec * per16 + threadIdx.x; e < eb; e += blockDim.x) slice[e] = level[first + e];
-                __syncthreads();
-                pi = prefix_index<K>(slice[0], slice[eb - 1], d.sbits);
-                build_prefix_table<K>(slice, eb, pi, d.sbits, stab);
+                // the bucket's prefix table, built once per call by k_bkt_stabs
+                const uint32_t tw = stab_stride(d.sbits) / 8;  // 16-byte pieces
+                const uint4 *tsrc = reinterpret_cast<const uint4 *>(stabs + (uint64_t)b * stab_stride(d.sbits));
+                for (uint32_t e = threadIdx.x; e < tw; e += blockDim.x) reinterpret_cast<uint4 *>(stab)[e] = tsrc[e];
+                pi = PrefixIndex<K>{(K)spi[2 * b], (uint32_t)spi[2 * b + 1]};
                 __syncthreads();
                 cur = b;
             }
@@ -676,7 +703,7 @@ struct BucketPlan {
     uint32_t grid;                 // search blocks
     uint64_t hist_entries;
     uint64_t off_qs, off_lidx, off_res, off_hist, off_off, off_cols, off_bsep, off_bstart, off_hdr, off_btab,
-        total;
+        off_stabs, off_spi, total;
     size_t lds_hist, lds_scatter, lds_search, lds_unpermute;
 };
 
@@ -718,7 +745,8 @@ static BucketPlan bucket_plan(const bpt_tree *t, uint64_t nq) {
     d.ntiles = (uint32_t)std::max<uint64_t>(1, (nq + d.tile - 1) / d.tile);
     d.ngroups = (d.ntiles + kColGroup - 1) / kColGroup;
     // search blocks per CU: as asked, at most as many as the CU's LDS holds
-    const size_t slice_lds = ((d.nb + 1 + 3) / 4) * 16 + (size_t)d.epb * t->key_bytes + ((1u << d.sbits) + 1) * 2;
+    const size_t slice_lds = ((d.nb + 1 + 3) / 4) * 16 + ((size_t)d.epb * t->key_bytes + 15) / 16 * 16 +
+                             (size_t)stab_stride(d.sbits) * 2;
     const uint64_t per_cu = std::max<uint64_t>(1, std::min<uint64_t>(t->group_blocks, kMaxLds / slice_lds));
     const uint64_t cap = (uint64_t)device_cu_count() * per_cu;
     if (t->group_chunk) {
@@ -746,6 +774,8 @@ static BucketPlan bucket_plan(const bpt_tree *t, uint64_t nq) {
     p.off_bstart = off; off += align256(((uint64_t)d.nb + 1) * 4);
     p.off_hdr = off;    off += align256(16);
     p.off_btab = off;   off += align256(((1ull << kBucketTabBits) + 1) * 2);
+    p.off_stabs = off;  off += align256((uint64_t)d.nb * stab_stride(d.sbits) * 2);
+    p.off_spi = off;    off += align256((uint64_t)d.nb * 16);
     p.total = off;
     const size_t kb = t->key_bytes, vb = t->val_bytes;
     const size_t btab_bytes = ((1u << kBucketTabBits) + 1) * 2;
@@ -775,12 +805,22 @@ static bpt_status bucket_chunk(const bpt_tree *t, const K *q, uint64_t nq, V *ou
     uint32_t *bstart = reinterpret_cast<uint32_t *>(ws + p.off_bstart);
     uint64_t *hdr = reinterpret_cast<uint64_t *>(ws + p.off_hdr);
     uint16_t *btab = reinterpret_cast<uint16_t *>(ws + p.off_btab);
+    uint16_t *stabs = reinterpret_cast<uint16_t *>(ws + p.off_stabs);
+    uint64_t *spi = reinterpret_cast<uint64_t *>(ws + p.off_spi);
     BPT_CHECK(p.lds_scatter <= kMaxLds && p.lds_search <= kMaxLds && p.lds_unpermute <= kMaxLds, BPT_ERR_INVALID,
               "bucket plan exceeds LDS");
     k_bkt_seps<K><<<(d.nb + 255) / 256, 256, 0, s>>>(t->view, d, bsep);
     BPT_HIP(hipGetLastError());
     k_bkt_btab<K><<<((1u << kBucketTabBits) + 256) / 256, kBBlock, 0, s>>>(t->view, d, bsep, hdr, btab);
     BPT_HIP(hipGetLastError());
+    {  // slice prefix tables: the slice source as the search kernel reads it
+        const TreeView &v = t->view;
+        const bool subl = d.spl != 0;
+        const K *level = reinterpret_cast<const K *>(v.inner + (subl ? v.sub_off : v.off[d.level]));
+        const uint64_t level_count = subl ? (v.n + v.sub_slots - 1) / v.sub_slots : v.count[d.level - 1];
+        k_bkt_stabs<K><<<d.nb, kPBlock, 0, s>>>(d, level, level_count, stabs, spi);
+        BPT_HIP(hipGetLastError());
+    }
     k_bkt_hist<K><<<d.ntiles, kPBlock, p.lds_hist, s>>>(q, nq, d, bsep, hdr, btab, hist);
     BPT_HIP(hipGetLastError());
     k_bkt_colsum<<<d.ngroups, kBBlock, 0, s>>>(hist, d, cols);
@@ -799,7 +839,7 @@ static bpt_status bucket_chunk(const bpt_tree *t, const K *q, uint64_t nq, V *ou
     auto go = [&](auto cp, auto lanes, auto o32, auto subl) {
         k_bkt_search<K, V, F, LF, decltype(cp)::value, decltype(lanes)::value, decltype(o32)::value,
                      decltype(subl)::value>
-            <<<p.grid, t->group_threads, p.lds_search, s>>>(t->view, d, qs, bstart, nq, res, sentinel);
+            <<<p.grid, t->group_threads, p.lds_search, s>>>(t->view, d, qs, bstart, stabs, spi, nq, res, sentinel);
     };
     auto go_leaf = [&](auto cp, auto lanes, auto o32) { go(cp, lanes, o32, std::false_type{}); };
     if constexpr (sizeof(K) == sizeof(V) && LF * sizeof(K) > 32) {  // leaves of several sub-leaves
