@@ -548,6 +548,20 @@ __global__ __launch_bounds__(kMaxSearchBlock) void k_bkt_search(TreeView t, Buck
     uint32_t eb = 0;
     uint64_t first = 0;
     PrefixIndex<K> pi{K(0), 0};
+    // Pieces are handed out by a block-wide ticket counter, so the waves finish
+    // a bucket together before the next slice is staged. Tickets number the
+    // pieces of the block's segments in order; seg_base is the first ticket of
+    // the current segment (the same in every wave).
+    __shared__ uint32_t tickets;
+    if (threadIdx.x == 0) tickets = 0;
+    __syncthreads();
+    auto claim = [&]() -> uint32_t {
+        uint32_t tk = 0;
+        if (lane == 0) tk = atomicAdd(&tickets, 1u);
+        return (uint32_t)__builtin_amdgcn_readfirstlane((int)tk);
+    };
+    uint32_t my_ticket = claim();
+    uint32_t seg_base = 0;
     for (uint64_t chunk = cfirst; chunk < c1; chunk += cstep) {
         uint64_t pos = chunk * d.chunk;
         const uint64_t end = std::min<uint64_t>(pos + d.chunk, nq);
@@ -586,7 +600,9 @@ __global__ __launch_bounds__(kMaxSearchBlock) void k_bkt_search(TreeView t, Buck
             // Loads past the segment's end re-read its last query (in bounds;
             // their results are dropped).
             const uint64_t piece = (uint64_t)kPieceBatches * 64;
-            for (uint64_t p0 = pos + (uint64_t)w * piece; p0 < seg_end; p0 += (uint64_t)nw * piece) {
+            const uint32_t npieces = (uint32_t)((seg_end - pos + piece - 1) / piece);
+            for (; my_ticket < seg_base + npieces; my_ticket = claim()) {
+                const uint64_t p0 = pos + (uint64_t)(my_ticket - seg_base) * piece;
                 K qv[kPieceBatches];
 #pragma unroll
                 for (int j = 0; j < kPieceBatches; ++j) {
@@ -632,6 +648,7 @@ __global__ __launch_bounds__(kMaxSearchBlock) void k_bkt_search(TreeView t, Buck
                     if (i < seg_end) __builtin_nontemporal_store(rv[j], &res[i]);
                 }
             }
+            seg_base += npieces;
             pos = seg_end;
         }
     }
