@@ -146,26 +146,28 @@ def bench_c3(args):
     doff = bptree.DeviceArray(nr, np.uint64)
     L = bptree.lib()
     forms, results = {}, {}
+    # the caller-owned id buffer, kept across calls and grown when a step needs more
+    ids_buf = [bptree.DeviceArray(1 << 20, np.uint32)]
     for mode, name in ((0, "basic"), (1, "grid2d"), (2, "zorder")):
         def step():
             bptree.check(L.bpt_rect_count(tree.h, dr.ptr, nr, mode, dcount.ptr, None))
             tot = bptree.C.c_uint64(0)
             bptree.check(L.bpt_exclusive_scan_u64(dcount.ptr, nr, doff.ptr, bptree.C.byref(tot), None))
-            dids = bptree.DeviceArray(tot.value, np.uint32)
-            bptree.check(L.bpt_rect_ids(tree.h, dr.ptr, nr, mode, doff.ptr, dids.ptr, None))
-            return dids
+            if tot.value > len(ids_buf[0]):
+                ids_buf[0].free()
+                ids_buf[0] = bptree.DeviceArray(tot.value, np.uint32)
+            bptree.check(L.bpt_rect_ids(tree.h, dr.ptr, nr, mode, doff.ptr, ids_buf[0].ptr, None))
+            return tot.value
         for _ in range(args.warmup):
-            step().free()
+            step()
         bptree.sync()
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            d = step()
+            step()
             bptree.sync()
-            d.free()
         dt = (time.perf_counter() - t0) / args.steps
-        d = step()
-        results[name] = (dcount.to_host(), d.to_host())
-        d.free()
+        tot = step()
+        results[name] = (dcount.to_host(), ids_buf[0].to_host(count=tot))
         forms[name] = {"rects_per_s": nr / dt, "ms": dt * 1e3}
         log(f"c3 {name}: {dt * 1e3:.2f} ms, {nr / dt / 1e6:.1f} M rects/s")
     base = results["basic"]
@@ -197,7 +199,7 @@ def bench_c3(args):
                         "traffic": None, "note": "reported, not gated: bytes depend on the decomposition"},
                        extra)
     tree.close()
-    for a in (dr, dcount, doff):
+    for a in (dr, dcount, doff, ids_buf[0]):
         a.free()
     return line
 
