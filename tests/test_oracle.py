@@ -110,6 +110,38 @@ def test_refcompat_range_count_matches_formula(oracle):
     assert (got <= canon).all() and (got < canon).any()
 
 
+def test_refcompat_range_count_end_leaf_clamp(oracle):
+    """The last leaf of a page tree of PAGESIZE 8, with lower bounds before it
+    (so start and end leaves differ; with both in the last leaf the reference
+    reads past the leaf). Counted against the canonical inclusive count, the
+    reference's loops skip the start leaf's last slot (Q3) and stop before the
+    end position, pulled back by one when it equals the parent page's child
+    count (Q4, BtreeRangeSearch.cu:284); the two set restatements (C oracle and
+    _refcompat_formula) must agree.
+    * 147 keys: upper bounds past the largest key. The descent clamps to the
+      last leaf (3 keys, its parent has 3 leaves): position 3, pulled back to
+      2, and the end leaf's last slot is not scanned either way: canonical - 2.
+    * 136 keys: the last parent has 1 leaf (8 keys). Upper bounds between that
+      leaf's first and second keys land on position 1 = the parent's size,
+      pulled back to 0: the leaf's first key, inside the range, is dropped:
+      canonical - 2 again, one of them by Q4 alone (without it the end slots
+      would be [0, 1))."""
+    rng = np.random.default_rng(11)
+    P = 8
+    for n, hi_of in ((147, lambda k, m: k[-1] + rng.integers(1, 5000, size=m, dtype=np.uint64)),
+                     (136, lambda k, m: rng.integers(int(k[128]) + 1, int(k[129]), size=m, dtype=np.uint64))):
+        k = np.sort(rng.choice(100000, size=n, replace=False).astype(np.uint64))
+        pt = oracle.PageTree(k, k, P)
+        assert pt.levels == 2
+        second_last_max = k[(len(k) - 1) // P * P - 1]
+        a = rng.integers(0, int(second_last_max) + 1, size=300, dtype=np.uint64)
+        b = hi_of(k, 300)
+        got = pt.range_count_refcompat(a, b)
+        assert np.array_equal(got, _refcompat_formula(k, P, a, b)), n
+        _, canon = oracle.range_closed(k, a, b)
+        assert np.array_equal(got, canon - 2), n
+
+
 def test_range_closed_bruteforce(oracle):
     rng = np.random.default_rng(4)
     k = np.sort(rng.integers(0, 500, size=300, dtype=np.uint64))  # with duplicates
