@@ -715,12 +715,40 @@ __global__ __launch_bounds__(kPBlock) void k_bkt_unpermute(const uint32_t *__res
 }
 
 // ------------------------------------------------------------------ host side
+// The bucket tables of a plan (they depend on the tree and the plan's
+// nb, level, lpb, epb, sbits and spl, not on the queries): bucket
+// separators, their prefix table (hdr, btab) and every bucket's slice prefix
+// table (stabs, spi), at these offsets of one block.
+struct TableLayout {
+    uint64_t bsep, hdr, btab, stabs, spi, total;
+};
+
+static TableLayout table_layout(const BucketDev &d, uint32_t key_bytes) {
+    TableLayout L{};
+    uint64_t off = 0;
+    L.bsep = off;   off += align256((uint64_t)d.nb * key_bytes);
+    L.hdr = off;    off += align256(16);
+    L.btab = off;   off += align256(((1ull << kBucketTabBits) + 1) * 2);
+    L.stabs = off;  off += align256((uint64_t)d.nb * stab_stride(d.sbits) * 2);
+    L.spi = off;    off += align256((uint64_t)d.nb * 16);
+    L.total = off;
+    return L;
+}
+
+static void table_sig(const BucketDev &d, uint64_t (&sig)[6]) {
+    sig[0] = d.nb;
+    sig[1] = d.level;
+    sig[2] = d.lpb;
+    sig[3] = d.epb;
+    sig[4] = d.sbits;
+    sig[5] = d.spl;
+}
+
 struct BucketPlan {
     BucketDev d;
     uint32_t grid;                 // search blocks
     uint64_t hist_entries;
-    uint64_t off_qs, off_lidx, off_res, off_hist, off_off, off_cols, off_bsep, off_bstart, off_hdr, off_btab,
-        off_stabs, off_spi, total;
+    uint64_t off_qs, off_lidx, off_res, off_hist, off_off, off_cols, off_bstart, off_tables, total;
     size_t lds_hist, lds_scatter, lds_search, lds_unpermute;
 };
 
@@ -787,12 +815,8 @@ static BucketPlan bucket_plan(const bpt_tree *t, uint64_t nq) {
     p.off_hist = off;   off += align256(p.hist_entries * 4);
     p.off_off = off;    off += align256(p.hist_entries * 4);
     p.off_cols = off;   off += align256((uint64_t)d.ngroups * d.nb * 4);
-    p.off_bsep = off;   off += align256((uint64_t)d.nb * t->key_bytes);
     p.off_bstart = off; off += align256(((uint64_t)d.nb + 1) * 4);
-    p.off_hdr = off;    off += align256(16);
-    p.off_btab = off;   off += align256(((1ull << kBucketTabBits) + 1) * 2);
-    p.off_stabs = off;  off += align256((uint64_t)d.nb * stab_stride(d.sbits) * 2);
-    p.off_spi = off;    off += align256((uint64_t)d.nb * 16);
+    p.off_tables = off; off += table_layout(d, t->key_bytes).total;  // used when the tree's are not
     p.total = off;
     const size_t kb = t->key_bytes, vb = t->val_bytes;
     const size_t btab_bytes = ((1u << kBucketTabBits) + 1) * 2;
@@ -808,6 +832,27 @@ uint64_t point_grouped_ws(const bpt_tree *t, uint64_t nq) {
     return bucket_plan(t, std::min(nq, kBucketChunk)).total;
 }
 
+// Builds the bucket tables of plan d for tree t into the block at tb.
+template <class K>
+static bpt_status fill_tables(const bpt_tree *t, const BucketDev &d, uint8_t *tb, hipStream_t s) {
+    const TableLayout L = table_layout(d, t->key_bytes);
+    K *bsep = reinterpret_cast<K *>(tb + L.bsep);
+    k_bkt_seps<K><<<(d.nb + 255) / 256, 256, 0, s>>>(t->view, d, bsep);
+    BPT_HIP(hipGetLastError());
+    k_bkt_btab<K><<<((1u << kBucketTabBits) + 256) / 256, kBBlock, 0, s>>>(
+        t->view, d, bsep, reinterpret_cast<uint64_t *>(tb + L.hdr), reinterpret_cast<uint16_t *>(tb + L.btab));
+    BPT_HIP(hipGetLastError());
+    // slice prefix tables: the slice source as the search kernel reads it
+    const TreeView &v = t->view;
+    const bool subl = d.spl != 0;
+    const K *level = reinterpret_cast<const K *>(v.inner + (subl ? v.sub_off : v.off[d.level]));
+    const uint64_t level_count = subl ? (v.n + v.sub_slots - 1) / v.sub_slots : v.count[d.level - 1];
+    k_bkt_stabs<K><<<d.nb, kPBlock, 0, s>>>(d, level, level_count, reinterpret_cast<uint16_t *>(tb + L.stabs),
+                                            reinterpret_cast<uint64_t *>(tb + L.spi));
+    BPT_HIP(hipGetLastError());
+    return BPT_OK;
+}
+
 template <class K, class V, int F, int LF>
 static bpt_status bucket_chunk(const bpt_tree *t, const K *q, uint64_t nq, V *out, uint8_t *ws, hipStream_t s) {
     const BucketPlan p = bucket_plan(t, nq);
@@ -818,26 +863,21 @@ static bpt_status bucket_chunk(const bpt_tree *t, const K *q, uint64_t nq, V *ou
     uint32_t *hist = reinterpret_cast<uint32_t *>(ws + p.off_hist);
     uint32_t *offs = reinterpret_cast<uint32_t *>(ws + p.off_off);
     uint32_t *cols = reinterpret_cast<uint32_t *>(ws + p.off_cols);
-    K *bsep = reinterpret_cast<K *>(ws + p.off_bsep);
     uint32_t *bstart = reinterpret_cast<uint32_t *>(ws + p.off_bstart);
-    uint64_t *hdr = reinterpret_cast<uint64_t *>(ws + p.off_hdr);
-    uint16_t *btab = reinterpret_cast<uint16_t *>(ws + p.off_btab);
-    uint16_t *stabs = reinterpret_cast<uint16_t *>(ws + p.off_stabs);
-    uint64_t *spi = reinterpret_cast<uint64_t *>(ws + p.off_spi);
     BPT_CHECK(p.lds_scatter <= kMaxLds && p.lds_search <= kMaxLds && p.lds_unpermute <= kMaxLds, BPT_ERR_INVALID,
               "bucket plan exceeds LDS");
-    k_bkt_seps<K><<<(d.nb + 255) / 256, 256, 0, s>>>(t->view, d, bsep);
-    BPT_HIP(hipGetLastError());
-    k_bkt_btab<K><<<((1u << kBucketTabBits) + 256) / 256, kBBlock, 0, s>>>(t->view, d, bsep, hdr, btab);
-    BPT_HIP(hipGetLastError());
-    {  // slice prefix tables: the slice source as the search kernel reads it
-        const TreeView &v = t->view;
-        const bool subl = d.spl != 0;
-        const K *level = reinterpret_cast<const K *>(v.inner + (subl ? v.sub_off : v.off[d.level]));
-        const uint64_t level_count = subl ? (v.n + v.sub_slots - 1) / v.sub_slots : v.count[d.level - 1];
-        k_bkt_stabs<K><<<d.nb, kPBlock, 0, s>>>(d, level, level_count, stabs, spi);
-        BPT_HIP(hipGetLastError());
-    }
+    // the tree's own tables when they were built for this plan, else this call's
+    uint64_t sig[6];
+    table_sig(d, sig);
+    const bool own = t->bkt_valid && std::equal(sig, sig + 6, t->bkt_sig);
+    uint8_t *tb = own ? t->d_bkt : ws + p.off_tables;
+    if (!own) BPT_TRY(fill_tables<K>(t, d, tb, s));
+    const TableLayout L = table_layout(d, t->key_bytes);
+    const K *bsep = reinterpret_cast<const K *>(tb + L.bsep);
+    const uint64_t *hdr = reinterpret_cast<const uint64_t *>(tb + L.hdr);
+    const uint16_t *btab = reinterpret_cast<const uint16_t *>(tb + L.btab);
+    const uint16_t *stabs = reinterpret_cast<const uint16_t *>(tb + L.stabs);
+    const uint64_t *spi = reinterpret_cast<const uint64_t *>(tb + L.spi);
     k_bkt_hist<K><<<d.ntiles, kPBlock, p.lds_hist, s>>>(q, nq, d, bsep, hdr, btab, hist);
     BPT_HIP(hipGetLastError());
     k_bkt_colsum<<<d.ngroups, kBBlock, 0, s>>>(hist, d, cols);
@@ -888,6 +928,31 @@ static bpt_status bucket_chunk(const bpt_tree *t, const K *q, uint64_t nq, V *ou
     BPT_HIP(hipGetLastError());
     k_bkt_unpermute<K, V><<<d.ntiles, kPBlock, p.lds_unpermute, s>>>(hist, offs, d, lidx, res, nq, out);
     BPT_HIP(hipGetLastError());
+    return BPT_OK;
+}
+
+bpt_status bucket_tables_refresh(bpt_tree *t, hipStream_t s) {
+    t->bkt_valid = false;
+    if (t->n == 0 || t->view.height == 0) return BPT_OK;
+    const BucketPlan p = bucket_plan(t, 1);  // the tables do not depend on the batch
+    const TableLayout L = table_layout(p.d, t->key_bytes);
+    if (L.total > t->bkt_alloc) {
+        if (t->d_bkt) (void)hipFree(t->d_bkt);
+        t->d_bkt = nullptr;
+        t->bkt_alloc = 0;
+        if (t->alloc_limit && L.total > t->alloc_limit) return BPT_OK;
+        if (hipMalloc(&t->d_bkt, L.total) != hipSuccess) {
+            (void)hipGetLastError();  // no tables of its own: searches build them per call
+            t->d_bkt = nullptr;
+            return BPT_OK;
+        }
+        t->bkt_alloc = L.total;
+    }
+    BPT_TRY(dispatch_tree(t, [&](auto kt, auto, auto, auto) -> bpt_status {
+        return fill_tables<typename decltype(kt)::type>(t, p.d, t->d_bkt, s);
+    }));
+    table_sig(p.d, t->bkt_sig);
+    t->bkt_valid = true;
     return BPT_OK;
 }
 

@@ -268,11 +268,13 @@ bpt_status tree_rebuild(bpt_tree *t, const void *d_keys, const void *d_vals, uin
         learned_free(t->learned);
         t->learned = nullptr;
     }
+    t->bkt_valid = false;
     st = dispatch_tree(t, [&](auto kt, auto vt, auto fc, auto lc) {
         using K = typename decltype(kt)::type;
         using V = typename decltype(vt)::type;
         return build_impl<K, V, decltype(fc)::value, decltype(lc)::value>(t, d_keys, d_vals, s);
     });
+    if (st == BPT_OK) st = bucket_tables_refresh(t, s);
     if (st == BPT_OK) {
         hipError_t e = hipStreamSynchronize(s);
         if (e != hipSuccess) st = hip_fail(e, "rebuild sync", __FILE__, __LINE__);
@@ -336,6 +338,7 @@ bpt_status bpt_tree_build(const bpt_build_desc *desc, const void *d_keys, const 
             return build_impl<K, V, decltype(fc)::value, decltype(lc)::value>(t, d_keys, d_vals, s);
         });
     }
+    if (st == BPT_OK) st = bucket_tables_refresh(t, s);
     if (st == BPT_OK) {
         hipError_t e = hipStreamSynchronize(s);
         if (e != hipSuccess) st = hip_fail(e, "build sync", __FILE__, __LINE__);
@@ -353,6 +356,7 @@ bpt_status bpt_tree_free(bpt_tree *t) {
     if (t->learned) learned_free(t->learned);
     if (t->d_leaves) (void)hipFree(t->d_leaves);
     if (t->d_inner) (void)hipFree(t->d_inner);
+    if (t->d_bkt) (void)hipFree(t->d_bkt);
     delete t;
     return BPT_OK;
 }

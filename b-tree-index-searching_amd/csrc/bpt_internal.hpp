@@ -94,6 +94,15 @@ struct bpt_tree {
     uint32_t group_slice = 131072;   // grouped search: LDS slice bytes per bucket (16 to 128 KiB)
     uint32_t group_sub = 1;          // grouped search: slices of sub-leaf maxima when the tree has them
     uint64_t alloc_limit = 0;        // largest leaf / inner buffer a rebuild may allocate (0: none)
+    // Grouped search: the bucket tables of the default plan (bucket
+    // separators, their prefix table, every bucket's slice prefix table),
+    // built with the tree and by every update, so that a search need not
+    // rebuild them. bkt_sig holds the plan parameters they were built for; a
+    // plan with other parameters (other options) builds its own per call.
+    uint8_t *d_bkt = nullptr;
+    uint64_t bkt_alloc = 0;
+    uint64_t bkt_sig[6] = {};
+    bool bkt_valid = false;
     bpt::TreeView view{};
     bpt::Workspace ws;             // grouped / learned scratch owned by the tree
     bpt::LearnedModel *learned = nullptr;
@@ -684,6 +693,10 @@ int group_start_pass_2d(uint64_t maxa, uint64_t maxb);
 bpt_status point_grouped(const bpt_tree *t, const void *d_q, uint64_t nq, void *d_out,
                          void *d_ws, uint64_t ws_bytes, hipStream_t s);
 uint64_t point_grouped_ws(const bpt_tree *t, uint64_t nq);
+// (Re)builds t's bucket tables on stream s (the caller synchronizes s before
+// the tree is searched). Without memory for them the tree keeps none and
+// grouped searches build them per call: not an error.
+bpt_status bucket_tables_refresh(bpt_tree *t, hipStream_t s);
 bpt_status point_learned(const bpt_tree *t, const void *d_q, uint64_t nq, void *d_out,
                          void *d_ws, uint64_t ws_bytes, hipStream_t s);
 uint64_t point_learned_ws(const bpt_tree *t, uint64_t nq);

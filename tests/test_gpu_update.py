@@ -160,3 +160,38 @@ def test_failed_rebuild_allocation_leaves_tree_unchanged(bpt, oracle):
     assert t.info()["n"] == len(m.k)
     assert np.array_equal(t.point_search(q), oracle.point_closed(m.k, m.v, q, (1 << 64) - 1))
     t.close()
+
+
+def test_grouped_tables_follow_updates(bpt, oracle):
+    """Grouped search reads the bucket tables the tree builds with itself and
+    rebuilds after every update; a plan with other options builds its own per
+    call. Both equal the oracle while updates change the number of buckets
+    (uint32 tree: 262144 keys per bucket, so 1, then 3, then 1 bucket)."""
+    rng = np.random.default_rng(11)
+    keys = np.unique(rng.integers(0, 1 << 32, 100_000, dtype=np.uint64)).astype(np.uint32)
+    vals = rng.integers(0, 1 << 32, len(keys), dtype=np.uint64).astype(np.uint32)
+    t = bpt.Tree(keys, vals, 4, 4)
+    m = Model(keys, vals)
+
+    def check(step):
+        q = np.concatenate([m.k[rng.integers(0, len(m.k), 40_000)],
+                            rng.integers(0, 1 << 32, 10_000, dtype=np.uint64).astype(np.uint32)])
+        want = oracle.point_closed(m.k, m.v, q, (1 << 32) - 1).astype(np.uint32)
+        print(f"step {step}: n={len(m.k)}", flush=True)
+        assert np.array_equal(t.point_search(q, bpt.GROUPED), want), step
+        t.set_option("group_slice", 65536)            # another plan: tables built per call
+        assert np.array_equal(t.point_search(q, bpt.GROUPED), want), step
+        t.set_option("group_slice", 131072)
+        assert np.array_equal(t.point_search(q, bpt.GROUPED), want), step
+
+    check("build")
+    k = rng.integers(0, 1 << 32, 500_000, dtype=np.uint64).astype(np.uint32)
+    v = rng.integers(0, 1 << 32, len(k), dtype=np.uint64).astype(np.uint32)
+    t.upsert(k, v)
+    m.upsert(k, v)
+    check("upsert")
+    e = m.k[rng.permutation(len(m.k))[:450_000]]
+    t.erase(e)
+    m.erase(e)
+    check("erase")
+    t.close()
