@@ -936,17 +936,18 @@ bpt_status bucket_tables_refresh(bpt_tree *t, hipStream_t s) {
     if (t->n == 0 || t->view.height == 0) return BPT_OK;
     const BucketPlan p = bucket_plan(t, 1);  // the tables do not depend on the batch
     const TableLayout L = table_layout(p.d, t->key_bytes);
-    if (L.total > t->bkt_alloc) {
+    if (L.total > t->bkt_alloc) {  // with a quarter to spare, so that growing updates rarely reallocate
         if (t->d_bkt) (void)hipFree(t->d_bkt);
         t->d_bkt = nullptr;
         t->bkt_alloc = 0;
-        if (t->alloc_limit && L.total > t->alloc_limit) return BPT_OK;
-        if (hipMalloc(&t->d_bkt, L.total) != hipSuccess) {
+        const uint64_t want = align256(L.total + L.total / 4);
+        if (t->alloc_limit && want > t->alloc_limit) return BPT_OK;
+        if (hipMalloc(&t->d_bkt, want) != hipSuccess) {
             (void)hipGetLastError();  // no tables of its own: searches build them per call
             t->d_bkt = nullptr;
             return BPT_OK;
         }
-        t->bkt_alloc = L.total;
+        t->bkt_alloc = want;
     }
     BPT_TRY(dispatch_tree(t, [&](auto kt, auto, auto, auto) -> bpt_status {
         return fill_tables<typename decltype(kt)::type>(t, p.d, t->d_bkt, s);
