@@ -89,7 +89,10 @@ typedef struct bpt_build_desc {
 
 /* Builds from n keys sorted non-decreasing and their values, both on the
  * device; the inputs are not modified and remain the caller's. n == 0 gives
- * an empty tree on which every lookup misses. Synchronizes `s`. */
+ * an empty tree on which every lookup misses. Synchronizes `s`. The tree also
+ * keeps the grouped search's bucket tables for its default options (5 MB
+ * for 64M uint32 keys; rebuilt by every update); without device memory for
+ * them it keeps none, and grouped searches build them per call. */
 bpt_status bpt_tree_build(const bpt_build_desc *desc, const void *d_keys, const void *d_vals,
                           uint64_t n, bpt_tree **out, bpt_stream s);
 bpt_status bpt_tree_free(bpt_tree *t);
