@@ -827,8 +827,17 @@ static BucketPlan bucket_plan(const bpt_tree *t, uint64_t nq) {
     return p;
 }
 
+// Batches below t->group_min run the plain descent (same results): the
+// partition's fixed cost (seven launches) and the slice staging are repaid
+// only by large batches. On MI355X the grouped form overtakes the plain
+// descent at about 2^24 queries on 16M-64M-key trees (2^22 on 4M keys, 2^25 on
+// trees whose leaves fit the L2).
+static bool grouped_runs_plain(const bpt_tree *t, uint64_t nq) {
+    return t->n == 0 || t->view.height == 0 || nq < t->group_min;
+}
+
 uint64_t point_grouped_ws(const bpt_tree *t, uint64_t nq) {
-    if (t->n == 0 || t->view.height == 0) return 0;
+    if (grouped_runs_plain(t, nq)) return 0;
     return bucket_plan(t, std::min(nq, kBucketChunk)).total;
 }
 
@@ -960,7 +969,7 @@ bpt_status bucket_tables_refresh(bpt_tree *t, hipStream_t s) {
 bpt_status point_grouped(const bpt_tree *t, const void *d_q, uint64_t nq, void *d_out, void *d_ws,
                          uint64_t ws_bytes, hipStream_t s) {
     if (nq == 0) return BPT_OK;
-    if (t->n == 0 || t->view.height == 0) return launch_plain(t, d_q, nq, d_out, s);
+    if (grouped_runs_plain(t, nq)) return launch_plain(t, d_q, nq, d_out, s);
     BPT_CHECK(d_ws && ws_bytes >= point_grouped_ws(t, nq), BPT_ERR_INVALID,
               "grouped search needs a workspace of %llu bytes",
               (unsigned long long)point_grouped_ws(t, nq));
@@ -970,7 +979,10 @@ bpt_status point_grouped(const bpt_tree *t, const void *d_q, uint64_t nq, void *
         constexpr int F = decltype(fc)::value, LF = decltype(lc)::value;
         for (uint64_t b = 0; b < nq; b += kBucketChunk) {
             const uint64_t m = std::min(kBucketChunk, nq - b);
-            BPT_TRY((bucket_chunk<K, V, F, LF>(t, (const K *)d_q + b, m, (V *)d_out + b, (uint8_t *)d_ws, s)));
+            if (grouped_runs_plain(t, m))  // a short last piece
+                BPT_TRY(launch_plain(t, (const K *)d_q + b, m, (V *)d_out + b, s));
+            else
+                BPT_TRY((bucket_chunk<K, V, F, LF>(t, (const K *)d_q + b, m, (V *)d_out + b, (uint8_t *)d_ws, s)));
         }
         return BPT_OK;
     });

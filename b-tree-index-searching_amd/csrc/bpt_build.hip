@@ -443,6 +443,11 @@ bpt_status bpt_tree_set_option(bpt_tree *t, const char *key, int64_t value) {
         t->group_sub = value ? 1 : 0;
         return BPT_OK;
     }
+    if (strcmp(key, "group_min") == 0) {
+        BPT_CHECK(value >= 0, BPT_ERR_INVALID, "group_min must be >= 0");
+        t->group_min = (uint64_t)value;
+        return BPT_OK;
+    }
     if (strcmp(key, "group_xcd") == 0) {
         t->group_xcd = value ? 1 : 0;
         return BPT_OK;

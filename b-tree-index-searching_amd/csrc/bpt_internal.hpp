@@ -93,6 +93,7 @@ struct bpt_tree {
     uint32_t group_threads = 1024;   // grouped search: threads per search block (512, 768 or 1024)
     uint32_t group_slice = 131072;   // grouped search: LDS slice bytes per bucket (16 to 128 KiB)
     uint32_t group_sub = 1;          // grouped search: slices of sub-leaf maxima when the tree has them
+    uint64_t group_min = 1ull << 24; // grouped search: smaller batches run the plain descent
     uint64_t alloc_limit = 0;        // largest leaf / inner buffer a rebuild may allocate (0: none)
     // Grouped search: the bucket tables of the default plan (bucket
     // separators, their prefix table, every bucket's slice prefix table),

@@ -153,9 +153,13 @@ bpt_status bpt_tree_erase(bpt_tree *t, const void *d_keys, uint64_t n, bpt_strea
  * "group_sub" (1 = slices of sub-leaf maxima, so that a query reads 64
  * bytes of its leaf, the default where the tree's leaves span several
  * 64-byte runs and its key and value widths match; 0 = leaf maxima and
- * whole leaves) and
+ * whole leaves),
  * "group_xcd" (1 = each XCD searches its own contiguous part of the
- * bucket-ordered queries, the default). "alloc_limit" (bytes; 0, the
+ * bucket-ordered queries, the default) and
+ * "group_min" (queries; a grouped batch below it runs the plain descent,
+ * with the same results and no workspace; default 2^24, about where the
+ * grouped kernels overtake the plain descent on MI355X for trees of 16M-64M
+ * keys; 0 runs the grouped kernels at every size). "alloc_limit" (bytes; 0, the
  * default, means none) caps each leaf or inner buffer that an update's
  * rebuild allocates: a larger one fails with BPT_ERR_NOMEM, as an exhausted
  * device would, and the tree stays unchanged. */
@@ -192,7 +196,8 @@ bpt_status bpt_tree_set_group_bits(bpt_tree *t, uint32_t bits);
  * device scratch of BPT_NOALLOC_SCRATCH_BYTES per GPU (held until the process
  * ends) instead of the tree's workspace; a batch whose workspace
  * (bpt_point_search_ws_size) is larger fails with BPT_ERR_CAPACITY (about 6M
- * queries on a 4-byte-key tree, 3M on an 8-byte one). All calls on one GPU
+ * queries on a 4-byte-key tree, 3M on an 8-byte one; batches below the
+ * tree's "group_min" need no scratch, as they run the plain descent). All calls on one GPU
  * share its scratch, so calls that overlap on one GPU (other streams or
  * threads) race; calls in one stream are safe. The first call on a device
  * looks the scratch's address up. */

@@ -40,6 +40,7 @@ def test_point_plain_and_grouped(bpt, oracle, cfg, n):
     keys, vals, q = _case(n, kb, vb, seed=n * 31 + F + LF)
     print(f"case n={n} cfg={cfg}", flush=True)
     t = bpt.Tree(keys, vals, kb, vb, fanout=F, leaf_cap=LF)
+    t.set_option("group_min", 0)  # the grouped kernels at every batch size
     want = _expect(oracle, keys, vals, q, vb)
     assert np.array_equal(t.point_search(q, bpt.PLAIN), want)
     t.set_option("lanes", 4)  # four lanes per query
@@ -77,6 +78,7 @@ def test_point_duplicates_first_occurrence(bpt, oracle):
     vals = np.arange(len(keys), dtype=np.uint32)
     q = rng.integers(0, 320, size=4000, dtype=np.uint64).astype(np.uint32)
     t = bpt.Tree(keys, vals)
+    t.set_option("group_min", 0)  # the grouped kernels at every batch size
     want = _expect(oracle, keys, vals, q, 4)   # first occurrence = lower bound
     assert np.array_equal(t.point_search(q), want)
     assert np.array_equal(t.point_search(q, bpt.GROUPED), want)
@@ -89,10 +91,30 @@ def test_unsorted_build_rejected(bpt):
     assert e.value.status == 2
 
 
+def test_grouped_small_batches_run_plain(bpt, oracle, wl):
+    """Below the tree's group_min (2^24 by default) a grouped batch runs the
+    plain descent: it needs no workspace and gives the same results; from
+    group_min on, the grouped kernels run with their workspace."""
+    keys, vals = wl.point_keys(1 << 20, workload=2)
+    q = wl.uniform_hits(keys, 1 << 18)
+    want = oracle.point_closed(keys, vals, q, 0xFFFFFFFF).astype(np.uint32)
+    t = bpt.Tree(keys, vals)
+    assert t.point_search_ws_size(1 << 18, bpt.GROUPED) == 0
+    assert t.point_search_ws_size(1 << 24, bpt.GROUPED) > 0
+    assert np.array_equal(t.point_search(q, bpt.GROUPED), want)
+    t.set_option("group_min", 1 << 10)
+    assert t.point_search_ws_size(1 << 18, bpt.GROUPED) > 0
+    assert np.array_equal(t.point_search(q, bpt.GROUPED), want)
+    with pytest.raises(bpt.BptError):
+        t.set_option("group_min", -1)
+    t.close()
+
+
 @pytest.mark.parametrize("bits", [1, 8, 12, 13])
 def test_grouped_bits_and_launch_shapes(bpt, oracle, bits):
     keys, vals, q = _case(200000, 4, 4, seed=bits)
     t = bpt.Tree(keys, vals)
+    t.set_option("group_min", 0)  # the grouped kernels at every batch size
     t.set_group_bits(bits)
     want = _expect(oracle, keys, vals, q, 4)
     assert np.array_equal(t.point_search(q, bpt.GROUPED), want)
@@ -107,6 +129,7 @@ def test_grouped_slice_levels(bpt, oracle, kb, n, bits):
     search reads the levels below it from global memory."""
     keys, vals, q = _case(n, kb, kb, seed=n + bits)
     t = bpt.Tree(keys, vals, kb, kb)
+    t.set_option("group_min", 0)  # the grouped kernels at every batch size
     t.set_group_bits(bits)
     want = _expect(oracle, keys, vals, q, kb)
     for lanes in (4, 8):
@@ -138,6 +161,7 @@ def test_grouped_sub_leaf_edges(bpt, oracle, kb, tail):
     keys[-3:] = np.iinfo(kdt).max
     vals = np.arange(n, dtype=np.uint64).astype(kdt)
     t = bpt.Tree(keys, vals, kb, kb, fanout=32, leaf_cap=lf)
+    t.set_option("group_min", 0)  # the grouped kernels at every batch size
     q = np.concatenate([keys[::3], np.arange(0, 40100, 7, dtype=np.uint64).astype(kdt),
                         np.array([np.iinfo(kdt).max, np.iinfo(kdt).max - 1, 40000, 0], dtype=kdt)])
     rng.shuffle(q)
@@ -160,6 +184,7 @@ def test_grouped_keys_with_gaps(bpt, oracle, kb):
         rng.integers(1 << 20, top >> 1, size=40, dtype=np.uint64)])).astype(kdt)
     vals = rng.integers(0, top, size=len(keys), dtype=np.uint64).astype(kdt)
     t = bpt.Tree(keys, vals, kb, kb)
+    t.set_option("group_min", 0)  # the grouped kernels at every batch size
     q = np.concatenate([keys[rng.integers(0, len(keys), size=200000)],
                         rng.integers(0, top, size=50000, dtype=np.uint64).astype(kdt),
                         np.array([0, top, 300000, top - 300000], dtype=np.uint64).astype(kdt)])
@@ -177,6 +202,7 @@ def test_grouped_skewed_queries(bpt, oracle):
     keys = np.unique(rng.integers(1000, 1 << 31, size=2_000_000, dtype=np.uint64)).astype(np.uint32)
     vals = np.arange(len(keys), dtype=np.uint32)
     t = bpt.Tree(keys, vals)
+    t.set_option("group_min", 0)  # the grouped kernels at every batch size
     hot = keys[rng.integers(100000, 100400, size=900_000)]
     q = np.concatenate([hot, np.full(50_000, 7, np.uint32), np.full(50_000, 0xFFFFFFF0, np.uint32),
                         rng.integers(0, 1 << 32, size=100_000, dtype=np.uint64).astype(np.uint32)])
@@ -201,6 +227,7 @@ def test_workload_c1_scaled_and_page_tree(bpt, oracle, wl):
     pt = oracle.PageTree(keys.astype(np.uint64), vals.astype(np.uint64), 32)
     want = pt.point(q.astype(np.uint64), 0xFFFFFFFF).astype(np.uint32)
     t = bpt.Tree(keys, vals)
+    t.set_option("group_min", 0)  # the grouped kernels at every batch size
     assert np.array_equal(t.point_search(q), want)
     assert np.array_equal(t.point_search(q, bpt.GROUPED), want)
     hits = (want != 0xFFFFFFFF).mean()

@@ -64,6 +64,7 @@ def test_upsert_erase_sequence(bpt, oracle, kb, vb, F, LF):
     keys[1000:1040] = keys[1000]                      # a run of duplicate keys in the built tree
     vals = rng.integers(0, (1 << (8 * vb)) - 1, n, dtype=np.uint64).astype(vdt)
     t = bpt.Tree(keys, vals, kb, vb, fanout=F, leaf_cap=LF)
+    t.set_option("group_min", 0)  # the grouped kernels at every batch size
     m = Model(keys, vals)
 
     def batch(size, frac_existing):
@@ -118,6 +119,7 @@ def test_updates_keep_options_and_drop_learned(bpt, oracle):
     keys = np.unique(rng.integers(0, 1 << 62, 400_000, dtype=np.uint64))
     vals = np.arange(len(keys), dtype=np.uint64)
     t = bpt.Tree(keys, vals, 8, 8, fanout=16, leaf_cap=8)
+    t.set_option("group_min", 0)  # the grouped kernels at every batch size
     t.set_option("lanes", 4)
     t.learned_build(leaf_models=1024, train_steps=50, train_samples=1 << 14)
     m = Model(keys, vals)
@@ -141,6 +143,7 @@ def test_failed_rebuild_allocation_leaves_tree_unchanged(bpt, oracle):
     keys = np.unique(rng.integers(0, 1 << 62, 200_000, dtype=np.uint64))
     vals = np.arange(len(keys), dtype=np.uint64)
     t = bpt.Tree(keys, vals, 8, 8, fanout=16, leaf_cap=8)
+    t.set_option("group_min", 0)  # the grouped kernels at every batch size
     t.learned_build(leaf_models=1024, train_steps=50, train_samples=1 << 14)
     t.set_option("alloc_limit", t.info()["leaf_bytes"])   # the current leaf buffer, no more
     k = rng.integers(0, 1 << 62, 100_000, dtype=np.uint64)   # grows the tree by half
@@ -171,6 +174,7 @@ def test_grouped_tables_follow_updates(bpt, oracle):
     keys = np.unique(rng.integers(0, 1 << 32, 100_000, dtype=np.uint64)).astype(np.uint32)
     vals = rng.integers(0, 1 << 32, len(keys), dtype=np.uint64).astype(np.uint32)
     t = bpt.Tree(keys, vals, 4, 4)
+    t.set_option("group_min", 0)  # the grouped kernels at every batch size
     m = Model(keys, vals)
 
     def check(step):

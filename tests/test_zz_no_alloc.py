@@ -31,6 +31,7 @@ def test_noalloc_matches_point_search(bpt, oracle, kb):
     keys = np.unique(rng.integers(0, (1 << (8 * kb)) - 1, n, dtype=np.uint64)).astype(kdt)
     vals = np.arange(len(keys), dtype=kdt)
     t = bpt.Tree(keys, vals, kb, kb)
+    t.set_option("group_min", 0)  # the grouped kernels at every batch size
     modes = [bpt.PLAIN, bpt.GROUPED]
     if kb == 8:
         t.learned_build(leaf_models=4096, train_steps=100, train_samples=1 << 16)
