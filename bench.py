@@ -65,6 +65,36 @@ def time_form(tree, dq, dout, mode, steps, warmup):
     return wall, ev
 
 
+def batch_sweep(tree, q):
+    """Wall ms per call of the plain descent, the grouped kernels (group_min
+    0) and the grouped mode with the tree's default group_min, for the first
+    n queries, n = 2^16 .. 2^26; every form's results are compared with
+    plain's."""
+    rows = []
+    for lg in (16, 18, 20, 21, 22, 23, 24, 25, 26):
+        n = 1 << lg
+        dq = bptree.DeviceArray.from_host(q[:n])
+        dout = bptree.DeviceArray(n, np.uint32)
+        steps = 50 if n <= (1 << 20) else 10
+        row = {"queries": n}
+        ref = None
+        for name, mode, gmin in (("plain", bptree.PLAIN, None), ("grouped_kernels", bptree.GROUPED, 0),
+                                 ("grouped_mode", bptree.GROUPED, 1 << 24)):
+            if gmin is not None:
+                tree.set_option("group_min", gmin)
+            wall, _ = time_form(tree, dq, dout, mode, steps, 3)
+            got = dout.to_host()
+            if ref is None:
+                ref = got
+            row[f"{name}_ms"] = wall * 1e3
+            row[f"{name}_mismatches"] = int((got != ref).sum())
+        rows.append(row)
+        log(f"batch sweep: {row}")
+        dq.free()
+        dout.free()
+    return rows
+
+
 def host_info(threads):
     cpu_model = ""
     try:
@@ -560,6 +590,9 @@ def main():
                     help="skip the 2-D rectangle, forest and learned-index configurations after the headline")
     ap.add_argument("--grouped-only", action="store_true",
                     help="time only the grouped form on the uniform set (for profiling its kernels)")
+    ap.add_argument("--batch-sweep", action="store_true",
+                    help="plain vs grouped kernels vs the grouped mode's dispatch over batch sizes 2^16..2^26 "
+                         "on the 64M-key tree (wall time per call)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run the 1 GiB device copy kernel (HBM counter calibration)")
     ap.add_argument("--no-autotune", action="store_true",
@@ -628,6 +661,10 @@ def main():
     copy_gbps = 2 * (1 << 30) / (float(np.median(copy_ms[1:])) * 1e-3) / 1e9
     log(f"1 GiB device copy (k_copy_u4, read + write): {copy_gbps:.0f} GB/s")
 
+    if args.batch_sweep:
+        print(json.dumps({"batch_sweep": batch_sweep(tree, q)}), flush=True)
+        tree.close()
+        return
     dq = bptree.DeviceArray.from_host(q)
     dout = bptree.DeviceArray(nq, np.uint32)
     if args.grouped_only:
