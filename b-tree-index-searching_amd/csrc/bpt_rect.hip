@@ -107,14 +107,17 @@ __device__ __forceinline__ uint64_t lb_after_leaf(const TreeView &t, const uint8
 }
 
 // Walks rectangle r from p = lower_bound(zmin); calls emit(slot) for every
-// point inside, in code order.
+// point inside, in code order. Gives up (returns false) before its
+// budget+1-th key read; the points emitted so far are then a prefix.
 template <int F, int LF, class Emit>
-__device__ __forceinline__ void rect_walk_from(const TreeView &t, const uint8_t *lds, uint4 r, uint64_t p,
-                                               Emit &&emit) {
+__device__ __forceinline__ bool rect_walk_budget(const TreeView &t, const uint8_t *lds, uint4 r, uint64_t p,
+                                                 uint32_t budget, Emit &&emit) {
     const uint32_t x0 = r.x, y0 = r.y, x1 = r.z, y1 = r.w;
-    if (x0 > x1 || y0 > y1) return;
+    if (x0 > x1 || y0 > y1) return true;
     const uint64_t zmax = spread(x1) | (spread(y1) << 1);
+    uint32_t reads = 0;
     while (p < t.n) {
+        if (reads++ == budget) return false;
         const uint64_t k = slot_key<uint64_t, uint32_t, LF>(t, p);
         if (k > zmax) break;
         const uint32_t x = compact(k), y = compact(k >> 1);
@@ -135,6 +138,13 @@ __device__ __forceinline__ void rect_walk_from(const TreeView &t, const uint8_t 
             p = lb_after_leaf<F, LF>(t, lds, p / LF, nz);
         }
     }
+    return true;
+}
+
+template <int F, int LF, class Emit>
+__device__ __forceinline__ void rect_walk_from(const TreeView &t, const uint8_t *lds, uint4 r, uint64_t p,
+                                               Emit &&emit) {
+    (void)rect_walk_budget<F, LF>(t, lds, r, p, ~0u, emit);
 }
 
 template <int F, int LF, class Emit>
@@ -164,12 +174,25 @@ __global__ __launch_bounds__(256) void k_rect_count(TreeView t, const uint4 *__r
 // lower_bound(Zmin), nearly all of a rectangle's tree reads when few points
 // fall inside -- run cooperatively: each wave takes 64 rectangles at a time,
 // G lanes per inner node, then one lane per rectangle for its leaf and walk.
+// A walk that needs more than kRectBudget key reads would hold its wave up:
+// its rectangle is queued instead (per block in LDS, then appended to the
+// global list `deferred` with one atomic per block) and counted afterwards
+// by a whole wave (k_rect_count_deferred). When a block's queue is full the
+// lane finishes the walk itself.
+constexpr uint32_t kRectBudget = 8;      // key reads of a walk before it is deferred
+constexpr uint32_t kRectQueue = 1024;    // deferred rectangles queued per block
 template <int F, int LF, int G, bool OFF32>
 __global__ __launch_bounds__(256) void k_rect_count_coop(TreeView t, const uint4 *__restrict__ rects,
                                                          const uint32_t *__restrict__ order, uint64_t nr,
-                                                         uint64_t *__restrict__ counts) {
+                                                         uint64_t *__restrict__ counts,
+                                                         uint32_t *__restrict__ deferred,
+                                                         uint64_t *__restrict__ ndeferred) {
     extern __shared__ uint4 lds_raw[];
     uint8_t *lds = reinterpret_cast<uint8_t *>(lds_raw);
+    __shared__ uint32_t dq[kRectQueue];
+    __shared__ uint32_t dn;
+    __shared__ uint64_t dbase;
+    if (threadIdx.x == 0) dn = 0;
     stage_top_levels(t, lds);
     const uint32_t lane = threadIdx.x & 63;
     const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -182,12 +205,61 @@ __global__ __launch_bounds__(256) void k_rect_count_coop(TreeView t, const uint4
         const uint64_t zmin = spread(r.x) | (spread(r.y) << 1);
         const uint32_t leaf = coop_leaf_of<uint64_t, F, G, OFF32>(t, lds, zmin);
         uint64_t c = 0;
-        if (r.x <= r.z && r.y <= r.w)
-            rect_walk_from<F, LF>(t, lds, r, leaf_lower_bound<uint64_t, uint32_t, LF>(t, leaf, zmin),
-                                  [&](uint64_t) { ++c; });
+        if (valid && r.x <= r.z && r.y <= r.w) {
+            const uint64_t p = leaf_lower_bound<uint64_t, uint32_t, LF>(t, leaf, zmin);
+            if (!rect_walk_budget<F, LF>(t, lds, r, p, kRectBudget, [&](uint64_t) { ++c; })) {
+                const uint32_t at = atomicAdd(&dn, 1u);
+                if (at < kRectQueue) {
+                    dq[at] = (uint32_t)i;
+                    continue;  // counted by k_rect_count_deferred
+                }
+                c = 0;
+                rect_walk_from<F, LF>(t, lds, r, p, [&](uint64_t) { ++c; });
+            }
+        }
         if (valid) counts[i] = c;
     }
+    __syncthreads();
+    const uint32_t n = dn < kRectQueue ? dn : kRectQueue;
+    if (n == 0) return;  // the same in every thread of the block
+    if (threadIdx.x == 0) dbase = atomicAdd(reinterpret_cast<unsigned long long *>(ndeferred), (unsigned long long)n);
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) deferred[dbase + k] = dq[k];
 }
+
+// Counts of the deferred rectangles: each wave takes one at a time, lane l
+// walks sub-rectangle (l % 8, l / 8) of an 8 x 8 split, and the lanes' counts
+// are summed (the sub-rectangles cover the rectangle once). (Cooperative
+// descents to the sub-rectangles' first codes measured no faster: the pass
+// waits on its longest walks.)
+template <int F, int LF>
+__global__ __launch_bounds__(256) void k_rect_count_deferred(TreeView t, const uint4 *__restrict__ rects,
+                                                             const uint32_t *__restrict__ deferred,
+                                                             const uint64_t *__restrict__ ndeferred,
+                                                             uint64_t *__restrict__ counts) {
+    extern __shared__ uint4 lds_raw[];
+    uint8_t *lds = reinterpret_cast<uint8_t *>(lds_raw);
+    stage_top_levels(t, lds);
+    const uint32_t lane = threadIdx.x & 63, kx = lane & 7, ky = lane >> 3;
+    const uint64_t nd = *ndeferred;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    for (uint64_t k = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; k < nd; k += nwaves) {
+        const uint32_t i = deferred[k];
+        const uint4 r = rects[i];
+        const uint64_t w = (uint64_t)r.z - r.x + 1, h = (uint64_t)r.w - r.y + 1;
+        const uint64_t xa = r.x + w * kx / 8, xe = r.x + w * (kx + 1) / 8;
+        const uint64_t ya = r.y + h * ky / 8, ye = r.y + h * (ky + 1) / 8;
+        uint64_t c = 0;
+        if (xa < xe && ya < ye) {
+            const uint4 sub = make_uint4((uint32_t)xa, (uint32_t)ya, (uint32_t)(xe - 1), (uint32_t)(ye - 1));
+            rect_walk<F, LF>(t, lds, sub, [&](uint64_t) { ++c; });
+        }
+        for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, 64);
+        if (lane == 0) counts[i] = c;
+    }
+}
+
+__global__ void k_rect_zero(uint64_t *p) { *p = 0; }
 
 constexpr uint32_t kRectSortSmall = 8;    // runs sorted by their own lane, in registers
 constexpr uint64_t kRectSplit = 4;        // longer runs are walked by a whole wave
@@ -380,6 +452,12 @@ __global__ void k_rect_zbounds(const uint4 *__restrict__ rects, uint64_t nr, uin
 
 // Grouped query order for rectangles: keys from (Zmin, Zmax). The scratch
 // space is the tree's own workspace (kept between calls, grown on demand).
+static uint64_t rect_order_bytes(uint64_t nr, int mode) {
+    if (mode == BPT_GROUP_NONE) return 0;
+    const uint64_t zb = (nr * 8 + 255) / 256 * 256;
+    return (2 * zb + query_order_ws_any(nr) + 255) / 256 * 256;
+}
+
 static bpt_status rect_order(bpt_tree *t, const uint32_t *d_rects, uint64_t nr, int mode, uint32_t **order,
                              hipStream_t s) {
     *order = nullptr;
@@ -387,7 +465,7 @@ static bpt_status rect_order(bpt_tree *t, const uint32_t *d_rects, uint64_t nr, 
     BPT_CHECK(mode == BPT_GROUP_GRID2D || mode == BPT_GROUP_ZORDER, BPT_ERR_INVALID, "group mode %d", mode);
     BPT_CHECK(nr < (1ull << 32), BPT_ERR_CAPACITY, "more than 2^32 rectangles");
     const uint64_t zb = (nr * 8 + 255) / 256 * 256;
-    const uint64_t bytes = 2 * zb + query_order_ws_any(nr);
+    const uint64_t bytes = rect_order_bytes(nr, mode);
     BPT_TRY(t->ws.reserve(bytes));
     void *ws = t->ws.ptr;
     uint64_t *zlo = reinterpret_cast<uint64_t *>(ws);
@@ -423,6 +501,12 @@ static bpt_status rect_run(bpt_tree *t, const uint32_t *d_rects, uint64_t nr, in
     }
     // the count pass walks in the requested order; the ids pass walks only the
     // rectangles with hits, in the caller's order (results are the same)
+    // the count pass's scratch: the query order, then the deferred-walk list
+    // (reserved together, so that the order's pointers stay valid)
+    const bool coop = !d_ids && t->coop && t->view.count[0] < (1ull << 32);
+    BPT_CHECK(nr < (1ull << 32), BPT_ERR_CAPACITY, "more than 2^32 rectangles");
+    const uint64_t obytes = d_ids ? 0 : rect_order_bytes(nr, mode);
+    if (coop) BPT_TRY(t->ws.reserve(obytes + 256 + nr * 4));
     uint32_t *order = nullptr;
     bpt_status st = d_ids ? BPT_OK : rect_order(t, d_rects, nr, mode, &order, s);
     if (st == BPT_OK) {
@@ -442,13 +526,19 @@ static bpt_status rect_run(bpt_tree *t, const uint32_t *d_rects, uint64_t nr, in
                 BPT_TRY(resident_grid(t, k_rect_ids<F, LF>, nr, ids_lds, &grid));
                 k_rect_ids<F, LF><<<grid, t->block, ids_lds, s>>>(t->view, (const uint4 *)d_rects, order, nr,
                                                                 d_offsets, last, d_ids);
-            } else if (t->coop && t->view.count[0] < (1ull << 32)) {
+            } else if (coop) {
                 const bool off32 = t->addr32 && t->inner_bytes <= (1ull << 32);
+                uint8_t *dw = static_cast<uint8_t *>(t->ws.ptr) + obytes;
+                uint64_t *ndef = reinterpret_cast<uint64_t *>(dw);
+                uint32_t *def = reinterpret_cast<uint32_t *>(dw + 256);
+                k_rect_zero<<<1, 1, 0, s>>>(ndef);
+                BPT_HIP(hipGetLastError());
                 auto go = [&](auto lanes, auto o32) -> bpt_status {
                     constexpr auto kernel = k_rect_count_coop<F, LF, decltype(lanes)::value, decltype(o32)::value>;
                     uint32_t grid = 0;
                     BPT_TRY(resident_grid(t, kernel, nr, lds, &grid));
-                    kernel<<<grid, t->block, lds, s>>>(t->view, (const uint4 *)d_rects, order, nr, d_counts);
+                    kernel<<<grid, t->block, lds, s>>>(t->view, (const uint4 *)d_rects, order, nr, d_counts, def,
+                                                       ndef);
                     return BPT_OK;
                 };
                 if (t->coop_lanes == 4) {
@@ -456,6 +546,11 @@ static bpt_status rect_run(bpt_tree *t, const uint32_t *d_rects, uint64_t nr, in
                 } else {
                     BPT_TRY(off32 ? go(IC<8>{}, std::true_type{}) : go(IC<8>{}, std::false_type{}));
                 }
+                BPT_HIP(hipGetLastError());
+                uint32_t grid = 0;
+                BPT_TRY(resident_grid(t, k_rect_count_deferred<F, LF>, nr, lds, &grid));
+                k_rect_count_deferred<F, LF><<<grid, t->block, lds, s>>>(t->view, (const uint4 *)d_rects, def, ndef,
+                                                                        d_counts);
             } else {
                 k_rect_count<F, LF><<<search_grid(t, nr), t->block, lds, s>>>(
                     t->view, (const uint4 *)d_rects, order, nr, d_counts);

@@ -105,6 +105,36 @@ def test_rect_id_runs_at_sort_limits(bpt):
         assert np.array_equal(got, want_ids), mode
 
 
+def test_rect_long_walks_deferred(bpt):
+    """Every rectangle holds 16 points of a 512 x 512 grid, more than the
+    count's walk budget, so every one is deferred to the whole-wave count;
+    with 4M of them each block's deferral queue overflows and its lanes
+    finish the rest themselves. Counts and sorted ids against the grid's
+    closed form."""
+    side = 512
+    g = np.arange(side * side, dtype=np.uint64)
+    gx, gy = g % side, g // side
+    z = np.zeros(len(g), dtype=np.uint64)
+    for b in range(9):
+        z |= ((gx >> np.uint64(b)) & np.uint64(1)) << np.uint64(2 * b)
+        z |= ((gy >> np.uint64(b)) & np.uint64(1)) << np.uint64(2 * b + 1)
+    rng = np.random.default_rng(21)
+    pid = rng.permutation(len(g)).astype(np.uint32)   # id of the point at grid index g
+    o = np.argsort(z)
+    t = bpt.Tree(z[o], pid[o], 8, 4)
+    nr = 1 << 22
+    x0 = rng.integers(0, side - 3, nr, dtype=np.uint64)
+    y0 = rng.integers(0, side - 3, nr, dtype=np.uint64)
+    rects = np.stack([x0, y0, x0 + 3, y0 + 3], axis=1).astype(np.uint32)
+    c, off, got = t.rect_search(rects, group_mode=0)
+    assert np.all(c == 16)
+    d = np.arange(4, dtype=np.uint64)
+    cells = (y0[:, None, None] + d[None, :, None]) * side + (x0[:, None, None] + d[None, None, :])
+    want = np.sort(pid[cells.reshape(nr, 16)], axis=1).reshape(-1)
+    assert np.array_equal(got, want)
+    t.close()
+
+
 def test_rect_last_rectangle_large(bpt, oracle, wl):
     """The ids pass learns the last rectangle's count separately (it has no
     next offset): a last rectangle covering the whole domain (every point)
