@@ -338,10 +338,13 @@ __device__ __forceinline__ void wave_sort_global(uint32_t *p, uint64_t c) {
 // The hit count of one rectangle by one wave: lane l walks sub-rectangle
 // (l % 8, l / 8) of an 8 x 8 split (the split of the ids pass), and the
 // lanes' counts are summed. For the last rectangle of an ids pass, which has
-// no next offset to give its count.
+// no next offset to give its count; it also clears the pass's two list
+// lengths.
 template <int F, int LF>
 __global__ __launch_bounds__(64) void k_rect_count_split(TreeView t, const uint4 *__restrict__ rect,
-                                                         uint64_t *__restrict__ count) {
+                                                         uint64_t *__restrict__ count,
+                                                         uint64_t *__restrict__ nlists) {
+    if (threadIdx.x < 2) nlists[threadIdx.x] = 0;
     extern __shared__ uint4 lds_raw[];
     uint8_t *lds = reinterpret_cast<uint8_t *>(lds_raw);
     stage_top_levels(t, lds);
@@ -359,22 +362,83 @@ __global__ __launch_bounds__(64) void k_rect_count_split(TreeView t, const uint4
     if (lane == 0) *count = c;
 }
 
-// Ids of the rectangles with hits. A rectangle's count is the gap to the
-// next offset (offsets are the exclusive scan of the counts), and the last
+// The rectangles with hits, listed for the ids pass in two lists: narrow
+// (1 to kRectSplit ids) and wide (more). A rectangle's count is the gap to
+// the next offset (offsets are the exclusive scan of the counts); the last
 // rectangle's, which has no next offset, is *last_count (counted just
-// before), so rectangles without points are skipped without a walk.
-// Each wave takes 64 rectangles at a time: their lanes walk and write the ids
-// in code order. A rectangle with more than kRectSplit ids is walked by the
-// whole wave instead: lane l takes sub-rectangle (l % 8, l / 8) of an 8 x 8
-// split of it (empty where a side is shorter than 8), and the lanes write
-// through a counter in LDS; the sub-rectangles cover the rectangle once, so
-// the run holds the same ids. Then runs of up to kRectSortSmall ids are
-// sorted by their own lane in registers, and longer runs by the whole wave,
-// one after another: in LDS up to kRectSortCap ids, in place in global
-// memory beyond.
+// before). Each block scans a contiguous range, collects its entries in LDS
+// and appends them with one atomic per list and flush; the lists' order does
+// not matter (every rectangle writes its own run).
+constexpr uint32_t kHitsPerThread = 16;
+constexpr uint32_t kHitsRound = 256 * kHitsPerThread;  // rectangles per block and round
+constexpr uint32_t kHitsCap = 2 * kHitsRound;          // LDS entries per list
+__global__ __launch_bounds__(256) void k_rect_hits(const uint64_t *__restrict__ offsets, uint64_t nr,
+                                                   const uint64_t *__restrict__ last_count,
+                                                   uint32_t *__restrict__ narrow, uint32_t *__restrict__ wide,
+                                                   uint64_t *__restrict__ nlists) {
+    __shared__ uint32_t ln[kHitsCap], lw[kHitsCap];
+    __shared__ uint32_t cn, cw;
+    __shared__ uint64_t bn, bw;
+    const uint64_t per = (nr + gridDim.x - 1) / gridDim.x;
+    const uint64_t lo = std::min<uint64_t>(nr, (uint64_t)blockIdx.x * per), hi = std::min<uint64_t>(nr, lo + per);
+    const uint64_t lastc = *last_count;
+    if (threadIdx.x == 0) cn = cw = 0;
+    __syncthreads();
+    auto flush = [&]() {  // every thread of the block
+        if (threadIdx.x == 0) {
+            bn = cn ? atomicAdd(reinterpret_cast<unsigned long long *>(nlists), (unsigned long long)cn) : 0;
+            bw = cw ? atomicAdd(reinterpret_cast<unsigned long long *>(nlists + 1), (unsigned long long)cw) : 0;
+        }
+        __syncthreads();
+        for (uint32_t k = threadIdx.x; k < cn; k += blockDim.x) narrow[bn + k] = ln[k];
+        for (uint32_t k = threadIdx.x; k < cw; k += blockDim.x) wide[bw + k] = lw[k];
+        __syncthreads();
+        if (threadIdx.x == 0) cn = cw = 0;
+        __syncthreads();
+    };
+    for (uint64_t base = lo; base < hi; base += kHitsRound) {
+        uint64_t a[kHitsPerThread], b[kHitsPerThread];
+#pragma unroll
+        for (uint32_t e = 0; e < kHitsPerThread; ++e) {
+            const uint64_t i = base + e * blockDim.x + threadIdx.x;
+            a[e] = i < hi ? offsets[i] : 0;
+            b[e] = i < hi && i + 1 < nr ? offsets[i + 1] : 0;
+        }
+#pragma unroll
+        for (uint32_t e = 0; e < kHitsPerThread; ++e) {
+            const uint64_t i = base + e * blockDim.x + threadIdx.x;
+            if (i < hi) {
+                const uint64_t c = i + 1 < nr ? b[e] - a[e] : lastc;
+                if (c > kRectSplit) lw[atomicAdd(&cw, 1u)] = (uint32_t)i;
+                else if (c > 0) ln[atomicAdd(&cn, 1u)] = (uint32_t)i;
+            }
+        }
+        __syncthreads();
+        const bool full = cn > kHitsCap - kHitsRound || cw > kHitsCap - kHitsRound;  // room for the next round?
+        __syncthreads();  // every thread has read cn, cw before the next round adds to them
+        if (full) flush();
+    }
+    __syncthreads();
+    if (cn || cw) flush();
+}
+
+// Ids of the listed rectangles; rectangles without points cost nothing here.
+// Wide rectangles go one per wave at a time to the whole wave: lane l takes
+// sub-rectangle (l % 8, l / 8) of an 8 x 8 split of it (empty where a side is
+// shorter than 8), and the lanes write through a counter in LDS; the
+// sub-rectangles cover the rectangle once, so the run holds the same ids.
+// Narrow rectangles go 64 to a wave (from the last wave down, so that the
+// waves with wide ones take fewer): their lanes walk and write the ids in
+// code order, and a walk that needs more than kRectBudget key reads goes to
+// the whole wave as a wide one (its ids so far are written again). Runs of up
+// to kRectSortSmall ids are then sorted by one lane in registers, longer runs
+// by the whole wave: in LDS up to kRectSortCap ids, in place in global memory
+// beyond.
 template <int F, int LF>
 __global__ __launch_bounds__(256) void k_rect_ids(TreeView t, const uint4 *__restrict__ rects,
-                                                  const uint32_t *__restrict__ order, uint64_t nr,
+                                                  const uint32_t *__restrict__ narrow,
+                                                  const uint32_t *__restrict__ wide,
+                                                  const uint64_t *__restrict__ nlists, uint64_t nr,
                                                   const uint64_t *__restrict__ offsets,
                                                   const uint64_t *__restrict__ last_count,
                                                   uint32_t *__restrict__ ids) {
@@ -383,60 +447,64 @@ __global__ __launch_bounds__(256) void k_rect_ids(TreeView t, const uint4 *__res
     stage_top_levels(t, lds);
     uint32_t *buf = reinterpret_cast<uint32_t *>(lds + ((t.lds_bytes + 15) & ~15ull)) +
                     (threadIdx.x >> 6) * kRectSortCap;
-    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t lane = threadIdx.x & 63, kx = lane & 7, ky = lane >> 3;
     const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const uint64_t stride = ((uint64_t)gridDim.x * blockDim.x) & ~63ull;
-    for (uint64_t base = wave * 64; base < nr; base += stride) {
-        const uint64_t j = base + lane;
-        uint64_t o = 0, c = 0, i = 0, known = 0;
-        if (j < nr) {
-            i = order ? order[j] : j;
-            o = offsets[i];
-            known = i + 1 < nr ? offsets[i + 1] - o : *last_count;
-            if (known > 0 && known <= kRectSplit) {
-                uint32_t *out = ids + o;
-                rect_walk<F, LF>(t, lds, rects[i], [&](uint64_t slot) {
-                    out[c++] = slot_val<uint64_t, uint32_t, LF>(t, slot);
-                });
-            }
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    const uint64_t nn = nlists[0], nw = nlists[1];
+    auto count_of = [&](uint64_t i) -> uint64_t { return i + 1 < nr ? offsets[i + 1] - offsets[i] : *last_count; };
+    // the whole wave writes the sc ids of rectangle i at ids[so ...]
+    auto walk_wide = [&](uint64_t i, uint64_t so, uint64_t sc) {
+        uint32_t *ctr = buf;  // the sorts use buf only after the walk
+        const uint4 r = rects[i];
+        if (lane == 0) *ctr = 0;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        const uint64_t w = (uint64_t)r.z - r.x + 1, h = (uint64_t)r.w - r.y + 1;
+        const uint64_t xa = r.x + w * kx / 8, xe = r.x + w * (kx + 1) / 8;
+        const uint64_t ya = r.y + h * ky / 8, ye = r.y + h * (ky + 1) / 8;
+        if (xa < xe && ya < ye) {
+            const uint4 sub = make_uint4((uint32_t)xa, (uint32_t)ya, (uint32_t)(xe - 1), (uint32_t)(ye - 1));
+            rect_walk<F, LF>(t, lds, sub, [&](uint64_t slot) {
+                const uint32_t at = atomicAdd(ctr, 1u);
+                if (at < sc) ids[so + at] = slot_val<uint64_t, uint32_t, LF>(t, slot);
+            });
         }
-        uint64_t wide = __ballot(known > kRectSplit);
-        if (wide) {
-            uint32_t *ctr = buf;  // the sorts below use buf only after every walk
-            const uint32_t kx = lane & 7, ky = lane >> 3;
-            while (wide) {
-                const int src = __ffsll((unsigned long long)wide) - 1;
-                wide &= wide - 1;
-                const uint64_t so = __shfl(o, src, 64), sc = __shfl(known, src, 64);
-                const uint4 r = rects[__shfl(i, src, 64)];
-                if (lane == 0) *ctr = 0;
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-                const uint64_t w = (uint64_t)r.z - r.x + 1, h = (uint64_t)r.w - r.y + 1;
-                const uint64_t xa = r.x + w * kx / 8, xe = r.x + w * (kx + 1) / 8;
-                const uint64_t ya = r.y + h * ky / 8, ye = r.y + h * (ky + 1) / 8;
-                if (xa < xe && ya < ye) {
-                    const uint4 sub = make_uint4((uint32_t)xa, (uint32_t)ya, (uint32_t)(xe - 1), (uint32_t)(ye - 1));
-                    rect_walk<F, LF>(t, lds, sub, [&](uint64_t slot) {
-                        const uint32_t at = atomicAdd(ctr, 1u);
-                        if (at < sc) ids[so + at] = slot_val<uint64_t, uint32_t, LF>(t, slot);
-                    });
-                }
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-            }
-            if (known > kRectSplit) c = known;
-        }
-        if (c > 1 && c <= kRectSortSmall) sort_small(ids + o, (uint32_t)c);
         // the lanes' writes are visible to the whole wave (one CU, one L1)
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-        uint64_t longer = __ballot(c > kRectSortSmall);
-        while (longer) {
-            const int src = __ffsll((unsigned long long)longer) - 1;
-            longer &= longer - 1;
-            const uint64_t so = __shfl(o, src, 64), sc = __shfl(c, src, 64);
-            if (sc <= kRectSortCap) wave_sort_lds(ids + so, (uint32_t)sc, buf);
-            else wave_sort_global(ids + so, sc);
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    };
+    for (uint64_t k = wave; k < nw; k += nwaves) {
+        const uint64_t i = wide[k];
+        const uint64_t so = offsets[i], sc = count_of(i);
+        walk_wide(i, so, sc);
+        if (sc <= kRectSortSmall) {
+            if (lane == 0) sort_small(ids + so, (uint32_t)sc);
+        } else if (sc <= kRectSortCap) {
+            wave_sort_lds(ids + so, (uint32_t)sc, buf);
+        } else {
+            wave_sort_global(ids + so, sc);
         }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    }
+    for (uint64_t b = nwaves - 1 - wave; b * 64 < nn; b += nwaves) {
+        const uint64_t j = b * 64 + lane;
+        uint64_t o = 0, c = 0, i = 0, known = 0;
+        bool spread_me = false;
+        if (j < nn) {
+            i = narrow[j];
+            o = offsets[i];
+            known = count_of(i);
+            uint32_t *out = ids + o;
+            const uint4 r = rects[i];
+            const uint64_t zmin = spread(r.x) | (spread(r.y) << 1);
+            if (!rect_walk_budget<F, LF>(t, lds, r, lb_u64<F, LF>(t, lds, zmin), kRectBudget,
+                                         [&](uint64_t slot) { out[c++] = slot_val<uint64_t, uint32_t, LF>(t, slot); }))
+                spread_me = true;
+        }
+        for (uint64_t sp = __ballot(spread_me); sp; sp &= sp - 1) {
+            const int src = __ffsll((unsigned long long)sp) - 1;
+            walk_wide(__shfl(i, src, 64), __shfl(o, src, 64), __shfl(known, src, 64));
+        }
+        if (spread_me) c = known;
+        if (c > 1) sort_small(ids + o, (uint32_t)c);  // c <= kRectSplit <= kRectSortSmall
     }
 }
 
@@ -516,16 +584,26 @@ static bpt_status rect_run(bpt_tree *t, const uint32_t *d_rects, uint64_t nr, in
             if (d_ids) {
                 // the last rectangle's count (it has no next offset), so that a
                 // large one is walked by a whole wave like the others
-                BPT_TRY(t->ws.reserve(256));
+                // the last rectangle's count and the two list lengths, then the
+                // narrow and wide lists (nr entries of room each)
+                const uint64_t lb = (nr * 4 + 255) / 256 * 256;
+                BPT_TRY(t->ws.reserve(256 + 2 * lb));
                 uint64_t *last = static_cast<uint64_t *>(t->ws.ptr);
-                k_rect_count_split<F, LF><<<1, 64, lds, s>>>(t->view, (const uint4 *)d_rects + (nr - 1), last);
+                uint64_t *nlists = last + 1;
+                uint32_t *narrow = reinterpret_cast<uint32_t *>(static_cast<uint8_t *>(t->ws.ptr) + 256);
+                uint32_t *wide = reinterpret_cast<uint32_t *>(static_cast<uint8_t *>(t->ws.ptr) + 256 + lb);
+                k_rect_count_split<F, LF><<<1, 64, lds, s>>>(t->view, (const uint4 *)d_rects + (nr - 1), last,
+                                                             nlists);
+                BPT_HIP(hipGetLastError());
+                const uint32_t hg = (uint32_t)std::min<uint64_t>((nr + kHitsRound - 1) / kHitsRound, 512);
+                k_rect_hits<<<hg, 256, 0, s>>>(d_offsets, nr, last, narrow, wide, nlists);
                 BPT_HIP(hipGetLastError());
                 const size_t ids_lds = ((t->view.lds_bytes + 15) & ~(size_t)15) +
                                        (size_t)(t->block / 64) * kRectSortCap * sizeof(uint32_t);
                 uint32_t grid = 0;
                 BPT_TRY(resident_grid(t, k_rect_ids<F, LF>, nr, ids_lds, &grid));
-                k_rect_ids<F, LF><<<grid, t->block, ids_lds, s>>>(t->view, (const uint4 *)d_rects, order, nr,
-                                                                d_offsets, last, d_ids);
+                k_rect_ids<F, LF><<<grid, t->block, ids_lds, s>>>(t->view, (const uint4 *)d_rects, narrow, wide,
+                                                                nlists, nr, d_offsets, last, d_ids);
             } else if (coop) {
                 const bool off32 = t->addr32 && t->inner_bytes <= (1ull << 32);
                 uint8_t *dw = static_cast<uint8_t *>(t->ws.ptr) + obytes;
