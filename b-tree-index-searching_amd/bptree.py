@@ -466,7 +466,8 @@ class Tree:
             total = C.c_uint64(0)
             check(lib().bpt_exclusive_scan_u64(dcount.ptr, nr, doff.ptr, C.byref(total), stream))
             dids = DeviceArray(total.value, np.uint32)
-            check(lib().bpt_rect_ids(self.h, dr.ptr, nr, group_mode, doff.ptr, dids.ptr, stream))
+            if total.value:  # no ids, no buffer to write them to
+                check(lib().bpt_rect_ids(self.h, dr.ptr, nr, group_mode, doff.ptr, dids.ptr, stream))
             offsets, ids = doff.to_host(stream), dids.to_host(stream)
         if own:
             dr.free()

@@ -261,6 +261,13 @@ __global__ __launch_bounds__(256) void k_rect_count_deferred(TreeView t, const u
 
 __global__ void k_rect_zero(uint64_t *p) { *p = 0; }
 
+// counts of an empty tree (a kernel, not a memset: callers may capture the
+// search into a graph)
+__global__ void k_rect_zero_n(uint64_t *__restrict__ p, uint64_t n) {
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) p[i] = 0;
+}
+
 constexpr uint32_t kRectSortSmall = 8;    // runs sorted by their own lane, in registers
 constexpr uint64_t kRectSplit = 4;        // longer runs are walked by a whole wave
 constexpr uint32_t kRectSortCap = 1024;   // runs sorted by the whole wave in LDS
@@ -564,7 +571,10 @@ static bpt_status rect_run(bpt_tree *t, const uint32_t *d_rects, uint64_t nr, in
               "rectangle search needs a tree of 8-byte Morton keys and 4-byte ids");
     if (nr == 0) return BPT_OK;
     if (t->n == 0) {
-        if (d_counts) BPT_HIP(hipMemsetAsync(d_counts, 0, nr * 8, s));
+        if (d_counts) {
+            k_rect_zero_n<<<(uint32_t)std::min<uint64_t>((nr + 255) / 256, 4096), 256, 0, s>>>(d_counts, nr);
+            BPT_HIP(hipGetLastError());
+        }
         return BPT_OK;
     }
     // the count pass walks in the requested order; the ids pass walks only the

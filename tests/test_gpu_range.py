@@ -135,6 +135,16 @@ def test_rect_long_walks_deferred(bpt):
     t.close()
 
 
+def test_rect_empty_tree(bpt):
+    """On an empty tree every rectangle counts 0 and no id is written."""
+    t = bpt.Tree(np.zeros(0, dtype=np.uint64), np.zeros(0, dtype=np.uint32), 8, 4)
+    r = np.array([[0, 0, 0xFFFFFFFF, 0xFFFFFFFF], [5, 5, 9, 9], [7, 1, 3, 2]], dtype=np.uint32)
+    for mode in MODES:
+        c, o, got = t.rect_search(r, group_mode=mode)
+        assert np.array_equal(c, np.zeros(3, dtype=np.uint64)) and len(got) == 0, mode
+    t.close()
+
+
 def test_rect_last_rectangle_large(bpt, oracle, wl):
     """The ids pass learns the last rectangle's count separately (it has no
     next offset): a last rectangle covering the whole domain (every point)
