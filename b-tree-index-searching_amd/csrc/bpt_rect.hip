@@ -12,8 +12,10 @@
 //     the current leaf when the jump target is near and descending the tree
 //     again otherwise;
 //   * pass 1 counts, with the first descent (to Zmin) run cooperatively;
-//     the caller scans counts into offsets; pass 2 walks the rectangles with
-//     hits again, writes their ids and sorts each run ascending.
+//     walks longer than a budget are deferred to a pass that splits each
+//     such rectangle over a whole wave;
+//   * the caller scans counts into offsets; pass 2 lists the rectangles with
+//     hits, walks them again, writes their ids and sorts each run ascending.
 // The 2D-grouping and Z-order modes reorder the rectangles of the count pass
 // by the keys of grouping2d.cu:24-31 / zOrder2d.cu:27-54 applied to
 // (Zmin, Zmax), so that neighbouring lanes walk neighbouring parts of the
