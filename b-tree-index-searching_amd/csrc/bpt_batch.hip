@@ -180,6 +180,7 @@ __global__ __launch_bounds__(256) void k_forest_batch(const TreeView *__restrict
             for (uint64_t base = wi.begin + (uint64_t)w * 64; base < wi.end; base += (uint64_t)nw * 64) {
                 const uint64_t my = base + lane;
                 const uint64_t key = my < wi.end ? __builtin_nontemporal_load(&q[my]) : 0ull;
+                __builtin_amdgcn_s_setprio(2);  // until the leaf's loads are issued (coop_leaf)
                 const uint64_t r = coop_lookup64<uint64_t, uint64_t, F, LF, true, kOpValue, G>(tv, lds, key, ~0ull);
                 if (my < wi.end) __builtin_nontemporal_store(r, &out[my]);
             }

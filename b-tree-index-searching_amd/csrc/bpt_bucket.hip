@@ -624,7 +624,12 @@ __global__ __launch_bounds__(kMaxSearchBlock) void k_bkt_search(TreeView t, Buck
 #pragma unroll
                     for (int j = 0; j < kPieceBatches; ++j) {
                         SubLeafBatch<K> bt;
-                        subleaf_issue<K, V, LF, OFF32>(t, qv[j], locate(qv[j]), bt);
+                        // raised priority from the slice search to the issued
+                        // loads, the batch's critical path; the ranking is not
+                        __builtin_amdgcn_s_setprio(2);
+                        const uint32_t sub = locate(qv[j]);
+                        subleaf_issue<K, V, LF, OFF32>(t, qv[j], sub, bt);
+                        __builtin_amdgcn_s_setprio(0);
                         rv[j] = subleaf_finish<K, V>(bt, nsb, nrem, sentinel);
                     }
                 } else {

@@ -539,6 +539,11 @@ coop_leaf(const TreeView &t, const K (&qk)[G], const uint32_t (&node)[G], V sent
             }
         }
     }
+    // The leaf's loads are out; ranking it and storing the result are off the
+    // critical path: back to the base priority (the search kernels raise it
+    // while a batch walks to its leaf, so that the SIMD issues first for the
+    // waves whose next step is a memory access).
+    __builtin_amdgcn_s_setprio(0);
     // slot leaf*LF + r holds a key iff leaf < last_leaf or r < last_cnt
     const uint32_t last_leaf = (uint32_t)(t.count[0] - 1);
     const uint32_t last_cnt = (uint32_t)(t.n - (uint64_t)last_leaf * LF);

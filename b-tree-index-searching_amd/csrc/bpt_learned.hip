@@ -514,6 +514,7 @@ __global__ __launch_bounds__(256) void k_point_learned_coop(TreeView t, LearnedV
         if (has) __builtin_nontemporal_store(r, &out[pos]);
     };
     for (uint64_t base = wave * 64; base < nq; base += nwaves * 64) {
+        __builtin_amdgcn_s_setprio(2);  // until the leaf's loads are issued (coop_leaf)
         const uint64_t my = base + lane;
         const K myq = my < nq ? __builtin_nontemporal_load(&q[my]) : K(0);
         const PackedModel m = v.models[route<K, LAYERWISE>(v, sh, myq)];

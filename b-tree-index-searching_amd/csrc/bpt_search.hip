@@ -47,6 +47,7 @@ __global__ __launch_bounds__(256) void k_point_coop(TreeView t, const K *__restr
         const uint64_t my = base + lane;
         const K myq = next;
         next = my + stride < nq ? __builtin_nontemporal_load(&q[my + stride]) : K(0);
+        __builtin_amdgcn_s_setprio(2);  // until the leaf's loads are issued (coop_leaf)
         const V res = coop_lookup64<K, V, F, LF, NT, kOpValue, G, OFF32>(t, lds, myq, sentinel);
         if (my < nq) __builtin_nontemporal_store(res, &out[my]);
     }
