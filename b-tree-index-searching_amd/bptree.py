@@ -618,10 +618,11 @@ def forest_autotune(dk, dv, key_off, dq, q_off, dout, dlo=None, dhi=None, r_off=
 
 
 REQ_POINT, REQ_RANGE = 0, 1
+REQ_DEVICE = 1  # Request.flags: the request's buffers are device memory
 
 
 class Request(C.Structure):
-    _fields_ = [("kind", C.c_int32), ("pad_", C.c_int32), ("n", C.c_uint64), ("q", vp), ("out", vp),
+    _fields_ = [("kind", C.c_int32), ("flags", C.c_int32), ("n", C.c_uint64), ("q", vp), ("out", vp),
                 ("lo", vp), ("hi", vp), ("first", vp), ("count", vp)]
 
 
@@ -651,6 +652,22 @@ class Server:
     def range_request(lo, hi, first, count):
         return Request(REQ_RANGE, 0, len(lo), None, None, lo.ctypes.data, hi.ctypes.data,
                        first.ctypes.data if first is not None else None, count.ctypes.data)
+
+    @staticmethod
+    def point_request_dev(dq, dout, offset=0, n=None):
+        """A point request over DeviceArrays: queries dq[offset:offset+n],
+        values written to dout[offset:offset+n] (the whole arrays by default)."""
+        n = len(dq) - offset if n is None else n
+        return Request(REQ_POINT, REQ_DEVICE, n, dq.ptr + offset * dq.dtype.itemsize,
+                       dout.ptr + offset * dout.dtype.itemsize, None, None, None, None)
+
+    @staticmethod
+    def range_request_dev(dlo, dhi, dfirst, dcount, offset=0, n=None):
+        """A range request over DeviceArrays (dfirst may be None), elements
+        [offset, offset + n) of each (the whole arrays by default)."""
+        n = len(dlo) - offset if n is None else n
+        at = lambda a: a.ptr + offset * a.dtype.itemsize if a is not None else None  # noqa: E731
+        return Request(REQ_RANGE, REQ_DEVICE, n, None, None, at(dlo), at(dhi), at(dfirst), at(dcount))
 
     def submit(self, req, keep=()):
         t = C.c_uint64(0)

@@ -41,6 +41,17 @@ inline hipStream_t as_stream(bpt_stream s) { return reinterpret_cast<hipStream_t
 
 int device_cu_count();  // CUs of the current device (cached per device)
 
+// One piece of a descriptor copy: `bytes` (a multiple of 4) from src to dst,
+// both device memory aligned to 4 bytes (16 for the wide path).
+struct CopyDesc {
+    const void *src;
+    void *dst;
+    uint64_t bytes;
+};
+// Copies the nd pieces of d_desc (a device array) in one launch; max_bytes
+// bounds the largest piece (it sizes the grid).
+bpt_status launch_desc_copy(const CopyDesc *d_desc, uint32_t nd, uint64_t max_bytes, hipStream_t s);
+
 // ---------------------------------------------------------------- layout
 constexpr int kMaxLevels = BPT_MAX_LEVELS;
 

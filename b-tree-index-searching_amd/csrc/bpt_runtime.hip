@@ -1,4 +1,5 @@
 // Error reporting, device plumbing and timers of the C ABI.
+#include <algorithm>
 #include <cstdarg>
 #include <cstring>
 #include <vector>
@@ -72,6 +73,33 @@ __global__ __launch_bounds__(256) void k_copy_u4(const u32x4 *__restrict__ src, 
     const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += step)
         __builtin_nontemporal_store(__builtin_nontemporal_load(&src[i]), &dst[i]);
+}
+
+// Descriptor copy: blockIdx.y picks the piece, the x blocks stride over it
+// with 16-byte accesses when its ends and size allow, else 4-byte ones.
+__global__ __launch_bounds__(256) void k_desc_copy(const CopyDesc *__restrict__ d) {
+    const CopyDesc c = d[blockIdx.y];
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t first = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (((reinterpret_cast<uintptr_t>(c.src) | reinterpret_cast<uintptr_t>(c.dst) | c.bytes) & 15) == 0) {
+        const u32x4 *s = reinterpret_cast<const u32x4 *>(c.src);
+        u32x4 *o = reinterpret_cast<u32x4 *>(c.dst);
+        for (uint64_t i = first; i < c.bytes / 16; i += step) o[i] = s[i];
+    } else {
+        const uint32_t *s = reinterpret_cast<const uint32_t *>(c.src);
+        uint32_t *o = reinterpret_cast<uint32_t *>(c.dst);
+        for (uint64_t i = first; i < c.bytes / 4; i += step) o[i] = s[i];
+    }
+}
+
+bpt_status launch_desc_copy(const CopyDesc *d_desc, uint32_t nd, uint64_t max_bytes, hipStream_t s) {
+    if (nd == 0 || max_bytes == 0) return BPT_OK;
+    BPT_CHECK(nd <= 65535, BPT_ERR_INVALID, "%u copy pieces (at most 65535)", nd);
+    // one pass of a block moves 4 KiB with 16-byte accesses
+    const uint64_t bx = std::min<uint64_t>(std::max<uint64_t>((max_bytes + 4095) / 4096, 1), 128);
+    k_desc_copy<<<dim3((uint32_t)bx, nd), 256, 0, s>>>(d_desc);
+    BPT_HIP(hipGetLastError());
+    return BPT_OK;
 }
 }  // namespace bpt
 

@@ -19,6 +19,10 @@
 //  * Admitted work overlaps on the device: copy-in, search kernels and
 //    copy-out of different requests run on different streams.
 //  * Pinned staging buffers (hipHostMalloc) per slot carry the host copies.
+//  * Device requests (BPT_REQ_DEVICE) skip the host: one alone is searched
+//    in place; a fused run is gathered into the pool by one descriptor copy
+//    kernel, searched as one batch and scattered back by another. Host and
+//    device requests are never fused together.
 //  * Latency = completion time - submit time; stats give p50/p99.
 #include <hip/hip_runtime.h>
 
@@ -126,17 +130,47 @@ struct bpt_server {
 
 namespace {
 
+uint64_t al256(uint64_t b) { return (b + 255) / 256 * 256; }
+
+// Copy pieces of a fused run of k device requests: points gather the
+// queries and scatter the values (2k); ranges gather both bounds and scatter
+// the counts and the lower bounds asked for (at most 4k).
+uint64_t desc_count(int32_t kind, uint64_t k) { return kind == BPT_REQ_POINT ? 2 * k : 4 * k; }
+
+// Pool bytes of one launch: k requests of one kind and one buffer kind, n
+// queries in all. Host requests and fused device runs: the queries and
+// results in the pool, then the point search's workspace, then (device) the
+// copy descriptors. A device request alone is searched in place and needs
+// the workspace only.
+uint64_t group_bytes(const bpt_tree *t, int mode, int32_t kind, int32_t flags, uint64_t n, uint64_t k) {
+    const uint64_t kb = t->key_bytes, vb = t->val_bytes;
+    const uint64_t ws = kind == BPT_REQ_POINT ? al256(bpt_point_search_ws_size(t, n, mode)) : 0;
+    const bool device = (flags & BPT_REQ_DEVICE) != 0;
+    if (device && k == 1) return std::max<uint64_t>(ws, 256);
+    uint64_t b = kind == BPT_REQ_POINT ? al256(n * kb) + al256(n * vb) + ws : 2 * al256(n * kb) + 2 * al256(n * 8);
+    if (device) b += al256(desc_count(kind, k) * sizeof(bpt::CopyDesc));
+    return b;
+}
+
 uint64_t request_bytes(const bpt_tree *t, int mode, const bpt_request &r) {
-    const uint64_t kb = t->key_bytes, vb = t->val_bytes, a = 256;
-    auto al = [&](uint64_t b) { return (b + a - 1) / a * a; };
-    if (r.kind == BPT_REQ_POINT)
-        return al(r.n * kb) + al(r.n * vb) + al(bpt_point_search_ws_size(t, r.n, mode));
-    return 2 * al(r.n * kb) + 2 * al(r.n * 8);
+    return group_bytes(t, mode, r.kind, r.flags, r.n, 1);
 }
 
 uint64_t staging_bytes(const bpt_tree *t, const bpt_request &r) {
     const uint64_t kb = t->key_bytes, vb = t->val_bytes;
     return r.kind == BPT_REQ_POINT ? r.n * (kb + vb) : r.n * (2 * kb + 16);
+}
+
+// Grows the slot's pinned staging buffer to at least `need` bytes.
+bpt_status ensure_pinned(Slot &sl, uint64_t need) {
+    if (sl.pin_bytes >= need) return BPT_OK;
+    if (sl.pin) (void)hipHostFree(sl.pin);
+    sl.pin = nullptr;
+    sl.pin_bytes = 0;
+    const uint64_t bytes = std::max<uint64_t>(need, 1 << 20);
+    BPT_HIP(hipHostMalloc(&sl.pin, bytes, hipHostMallocDefault));
+    sl.pin_bytes = bytes;
+    return BPT_OK;
 }
 
 // Queries of the slot's group (all of one kind).
@@ -146,28 +180,90 @@ uint64_t group_queries(const Slot &sl) {
     return n;
 }
 
+// A run of device requests. Alone: searched in place. Fused: the pool holds
+// [queries][results][search workspace][descriptors] (points) or
+// [lo][hi][first][count][descriptors] (ranges); the descriptors go up
+// through the slot's pinned buffer, one copy kernel gathers the requests'
+// inputs, the search runs over the whole run, and another copy kernel
+// scatters the results to the requests' own buffers.
+bpt_status launch_device(bpt_server *S, Slot &sl) {
+    using bpt::CopyDesc;
+    const bpt_tree *t = S->tree;
+    const uint64_t kb = t->key_bytes, vb = t->val_bytes;
+    const bool point = sl.group[0]->req.kind == BPT_REQ_POINT;
+    const uint64_t n = group_queries(sl), k = sl.group.size();
+    uint8_t *dev = S->d_pool + sl.off;
+    hipStream_t s = sl.stream;
+    const uint64_t wsb = point ? bpt_point_search_ws_size(t, n, S->mode) : 0;
+    if (k == 1) {
+        const bpt_request &r = sl.group[0]->req;
+        if (point) return bpt_point_search_ws(t, r.q, n, r.out, S->mode, wsb ? dev : nullptr, wsb, s);
+        return bpt_range_count(t, r.lo, r.hi, n, r.first, r.count, reinterpret_cast<bpt_stream>(s));
+    }
+    BPT_TRY(ensure_pinned(sl, desc_count(sl.group[0]->req.kind, k) * sizeof(CopyDesc)));
+    CopyDesc *h = reinterpret_cast<CopyDesc *>(sl.pin);
+    uint64_t ng = 0, ns = 0, max_g = 0, max_s = 0;
+    auto gather = [&](const void *src, uint8_t *dst, uint64_t bytes) {
+        h[ng++] = CopyDesc{src, dst, bytes};
+        max_g = std::max(max_g, bytes);
+    };
+    std::vector<CopyDesc> sc;  // scatter pieces, placed after the gather pieces
+    sc.reserve(2 * k);
+    auto scatter = [&](const uint8_t *src, void *dst, uint64_t bytes) {
+        sc.push_back(CopyDesc{src, dst, bytes});
+        max_s = std::max(max_s, bytes);
+    };
+    uint64_t at = 0;
+    if (point) {
+        uint8_t *dq = dev, *dout = dev + al256(n * kb), *ws = dout + al256(n * vb);
+        CopyDesc *dd = reinterpret_cast<CopyDesc *>(ws + al256(wsb));
+        for (const Request *q : sl.group) {
+            gather(q->req.q, dq + at * kb, q->req.n * kb);
+            scatter(dout + at * vb, q->req.out, q->req.n * vb);
+            at += q->req.n;
+        }
+        for (const CopyDesc &c : sc) h[ng + ns++] = c;
+        BPT_HIP(hipMemcpyAsync(dd, h, (ng + ns) * sizeof(CopyDesc), hipMemcpyHostToDevice, s));
+        BPT_TRY(bpt::launch_desc_copy(dd, (uint32_t)ng, max_g, s));
+        BPT_TRY(bpt_point_search_ws(t, dq, n, dout, S->mode, wsb ? ws : nullptr, wsb, s));
+        BPT_TRY(bpt::launch_desc_copy(dd + ng, (uint32_t)ns, max_s, s));
+    } else {
+        uint8_t *dlo = dev, *dhi = dev + al256(n * kb), *df = dhi + al256(n * kb), *dc = df + al256(n * 8);
+        CopyDesc *dd = reinterpret_cast<CopyDesc *>(dc + al256(n * 8));
+        for (const Request *q : sl.group) {
+            const bpt_request &r = q->req;
+            gather(r.lo, dlo + at * kb, r.n * kb);
+            gather(r.hi, dhi + at * kb, r.n * kb);
+            if (r.first) scatter(df + at * 8, r.first, r.n * 8);
+            scatter(dc + at * 8, r.count, r.n * 8);
+            at += r.n;
+        }
+        for (const CopyDesc &c : sc) h[ng + ns++] = c;
+        BPT_HIP(hipMemcpyAsync(dd, h, (ng + ns) * sizeof(CopyDesc), hipMemcpyHostToDevice, s));
+        BPT_TRY(bpt::launch_desc_copy(dd, (uint32_t)ng, max_g, s));
+        BPT_TRY(bpt_range_count(t, dlo, dhi, n, reinterpret_cast<uint64_t *>(df), reinterpret_cast<uint64_t *>(dc),
+                                reinterpret_cast<bpt_stream>(s)));
+        BPT_TRY(bpt::launch_desc_copy(dd + ng, (uint32_t)ns, max_s, s));
+    }
+    return BPT_OK;
+}
+
 // Stages, launches and queues the copy-out of the slot's group, in group
 // order. Points: staging = [queries][results], device = [queries][results]
 // [search workspace]. Ranges: staging = [lo][hi][first][count], device the
-// same.
+// same. Device requests go to launch_device.
 bpt_status launch(bpt_server *S, Slot &sl) {
     const bpt_tree *t = S->tree;
     const uint64_t kb = t->key_bytes, vb = t->val_bytes;
     const bpt_request &r0 = sl.group[0]->req;
+    if (r0.flags & BPT_REQ_DEVICE) return launch_device(S, sl);
     const bool point = r0.kind == BPT_REQ_POINT;
     const uint64_t n = group_queries(sl);
     bpt_request whole = r0;
     whole.n = n;
-    const uint64_t need = staging_bytes(t, whole);
-    if (sl.pin_bytes < need) {
-        if (sl.pin) (void)hipHostFree(sl.pin);
-        sl.pin = nullptr;
-        sl.pin_bytes = 0;
-        BPT_HIP(hipHostMalloc(&sl.pin, std::max<uint64_t>(need, 1 << 20), hipHostMallocDefault));
-        sl.pin_bytes = std::max<uint64_t>(need, 1 << 20);
-    }
+    BPT_TRY(ensure_pinned(sl, staging_bytes(t, whole)));
     uint8_t *dev = S->d_pool + sl.off;
-    auto al = [](uint64_t b) { return (b + 255) / 256 * 256; };
+    auto al = al256;
     hipStream_t s = sl.stream;
     if (point) {
         uint64_t at = 0;
@@ -200,7 +296,7 @@ bpt_status launch(bpt_server *S, Slot &sl) {
 
 // Copies a finished group's results to the callers' buffers (no lock held).
 void copy_out(const bpt_server *S, const Slot &sl, bpt_status st) {
-    if (st != BPT_OK) return;
+    if (st != BPT_OK || (sl.group[0]->req.flags & BPT_REQ_DEVICE)) return;
     const uint64_t kb = S->tree->key_bytes, vb = S->tree->val_bytes;
     const bool point = sl.group[0]->req.kind == BPT_REQ_POINT;
     const uint64_t n = group_queries(sl);
@@ -228,15 +324,13 @@ bool take_work(bpt_server *S, Slot &sl) {
     uint64_t n = h->req.n;
     while (k < S->queue.size() && k < kFuseRequests) {
         const Request *r = S->queue[k];
-        if (r->req.kind != h->req.kind || n + r->req.n > kFuseQueries) break;
+        if (r->req.kind != h->req.kind || r->req.flags != h->req.flags || n + r->req.n > kFuseQueries) break;
         n += r->req.n;
         ++k;
     }
     uint64_t off = 0, bytes = 0;
     if (k > 1) {
-        bpt_request g = h->req;
-        g.n = n;
-        bytes = request_bytes(S->tree, S->mode, g);
+        bytes = group_bytes(S->tree, S->mode, h->req.kind, h->req.flags, n, k);
         if (!S->pool.alloc(bytes, &off)) k = 1;
     }
     if (k == 1) {
@@ -311,9 +405,24 @@ Request *make_request(bpt_server *S, const bpt_request *r, bpt_status *st) {
         bpt::set_error("empty request");
         return nullptr;
     }
+    if (r->flags != 0 && r->flags != BPT_REQ_DEVICE) {
+        bpt::set_error("request flags %d", r->flags);
+        return nullptr;
+    }
     if (r->kind == BPT_REQ_POINT ? !(r->q && r->out) : !(r->lo && r->hi && r->count)) {
         bpt::set_error("request buffers are NULL");
         return nullptr;
+    }
+    if (r->flags & BPT_REQ_DEVICE) {
+        const uint64_t kb = S->tree->key_bytes, vb = S->tree->val_bytes;
+        auto ok = [](const void *p, uint64_t a) { return reinterpret_cast<uintptr_t>(p) % a == 0; };
+        const bool aligned = r->kind == BPT_REQ_POINT
+                                 ? ok(r->q, kb) && ok(r->out, vb)
+                                 : ok(r->lo, kb) && ok(r->hi, kb) && ok(r->first, 8) && ok(r->count, 8);
+        if (!aligned) {
+            bpt::set_error("device request buffers not aligned to their element widths");
+            return nullptr;
+        }
     }
     const uint64_t bytes = request_bytes(S->tree, S->mode, *r);
     if (bytes > S->budget) {

@@ -345,20 +345,27 @@ bpt_status bpt_mlp_loss_grad(const float *h_params, uint32_t hidden, const float
 /* Continuous batching over a resident tree: `slots` streams, requests
  * admitted FIFO when a slot is free and their device bytes fit in a pool of
  * `mem_budget` bytes (they wait otherwise; one larger than the whole budget
- * is rejected with BPT_ERR_CAPACITY). Host buffers stay the caller's until
- * the request completes. Point requests use `mode` (bpt_point_mode). A run
- * of queued requests of one kind (up to 2^20 queries, 512 requests) is
- * answered in one launch; each caller gets its own results. */
+ * is rejected with BPT_ERR_CAPACITY). Buffers stay the caller's until the
+ * request completes. Point requests use `mode` (bpt_point_mode). A run of
+ * queued requests of one kind and one buffer kind (up to 2^20 queries, 512
+ * requests) is answered in one launch; each caller gets its own results.
+ * flags = BPT_REQ_DEVICE: every buffer of the request is device memory of
+ * the tree's device, aligned to its element width; there are no host copies
+ * (a request alone is searched in place, a fused run is gathered into the
+ * pool and its results scattered back by two copy kernels), and the results
+ * are in place when wait/replay returns. flags = 0: host buffers, staged
+ * through pinned memory. */
 typedef struct bpt_server bpt_server;
 enum { BPT_REQ_POINT = 0, BPT_REQ_RANGE = 1 };
+enum { BPT_REQ_DEVICE = 1 };
 typedef struct bpt_request {
     int32_t kind;   /* BPT_REQ_POINT or BPT_REQ_RANGE */
-    int32_t pad_;
+    int32_t flags;  /* 0 (host buffers) or BPT_REQ_DEVICE */
     uint64_t n;     /* queries in the request */
-    const void *q;  /* point: host keys */
-    void *out;      /* point: host values (sentinel on a miss) */
-    const void *lo, *hi;     /* range: host bounds (inclusive) */
-    uint64_t *first, *count; /* range: host lower bound (may be NULL) and count */
+    const void *q;  /* point: keys */
+    void *out;      /* point: values (sentinel on a miss) */
+    const void *lo, *hi;     /* range: bounds (inclusive) */
+    uint64_t *first, *count; /* range: lower bound (may be NULL) and count */
 } bpt_request;
 typedef struct bpt_server_stats {
     uint64_t completed, lookups;

@@ -144,6 +144,161 @@ def test_small_point_requests_are_fused(bpt, oracle, served_tree, mode):
             assert np.array_equal(m[1], c) and np.array_equal(m[0], f)
 
 
+def _device_requests(bpt, reqs, no_first=()):
+    """The requests' inputs in device memory: one DeviceArray per buffer
+    kind, each request a slice of it. Range requests listed in no_first ask
+    for counts only."""
+    pts = [r[1] for r in reqs if r[0] == "point"]
+    rgs = [r for r in reqs if r[0] == "range"]
+    cat = lambda xs: np.concatenate(xs) if xs else np.zeros(1, dtype=np.uint64)  # noqa: E731
+    d = {"q": bpt.DeviceArray.from_host(cat(pts)), "lo": bpt.DeviceArray.from_host(cat([r[1] for r in rgs])),
+         "hi": bpt.DeviceArray.from_host(cat([r[2] for r in rgs]))}
+    d["out"] = bpt.DeviceArray(len(d["q"]), np.uint64)
+    d["first"] = bpt.DeviceArray(len(d["lo"]), np.uint64)
+    d["count"] = bpt.DeviceArray(len(d["lo"]), np.uint64)
+    for a in (d["out"], d["first"], d["count"]):
+        a.fill_bytes(0x5A)
+    structs, where, pa, ra = [], [], 0, 0
+    for i, r in enumerate(reqs):
+        if r[0] == "point":
+            structs.append(bpt.Server.point_request_dev(d["q"], d["out"], pa, len(r[1])))
+            where.append(("point", pa, len(r[1])))
+            pa += len(r[1])
+        else:
+            first = None if i in no_first else d["first"]
+            structs.append(bpt.Server.range_request_dev(d["lo"], d["hi"], first, d["count"], ra, len(r[1])))
+            where.append(("range", ra, len(r[1]), i in no_first))
+            ra += len(r[1])
+    return d, structs, where
+
+
+def _device_results(d, where):
+    out, first, count = d["out"].to_host(), d["first"].to_host(), d["count"].to_host()
+    res = []
+    for w in where:
+        if w[0] == "point":
+            res.append(out[w[1]:w[1] + w[2]])
+        else:
+            res.append((None if w[3] else first[w[1]:w[1] + w[2]], count[w[1]:w[1] + w[2]]))
+    return res
+
+
+@pytest.mark.parametrize("mode", [0, 2])
+def test_device_requests_alone_and_fused(bpt, oracle, served_tree, mode):
+    """Requests whose buffers are device memory: run alone (searched in
+    place) and arriving together (fused runs gathered and scattered by the
+    copy kernels), with some range requests asking for counts only. Every
+    answer equals the oracle."""
+    t, keys, vals = served_tree
+    rng = np.random.default_rng(40 + mode)
+    reqs = _requests(keys, 40, seed=50 + mode)
+    for i in range(300):  # many small ones, so that runs of each kind fuse
+        n = int(rng.integers(1, 300))
+        if i % 29 >= 25:
+            s_ = rng.integers(0, len(keys), n)
+            reqs.append(("range", keys[s_].astype(np.uint64), keys[np.minimum(s_ + 77, len(keys) - 1)].astype(np.uint64)))
+        else:
+            reqs.append(("point", np.where(rng.random(n) < 0.8, keys[rng.integers(0, len(keys), n)],
+                                           rng.integers(0, 1 << 50, n, dtype=np.uint64)).astype(np.uint64)))
+    no_first = {i for i, r in enumerate(reqs) if r[0] == "range" and i % 3 == 0}
+    for fused in (False, True):
+        d, structs, where = _device_requests(bpt, reqs, no_first)
+        srv = bpt.Server(t, slots=1 if not fused else 4, mem_budget=1 << 28, mode=mode)
+        if fused:
+            srv.replay(structs, np.zeros(len(structs)))
+        else:
+            for st_ in structs:
+                srv.wait(srv.submit(st_))
+        st = srv.stats()
+        srv.close()
+        got = _device_results(d, where)
+        for a in d.values():
+            a.free()
+        assert st["completed"] == len(reqs)
+        if fused:
+            assert st["max_inflight"] > 4  # more requests in flight than slots: runs were fused
+        for r, m in zip(reqs, got):
+            if r[0] == "point":
+                assert np.array_equal(m, oracle.point_closed(keys, vals, r[1], SENT))
+            else:
+                f, c = oracle.range_closed(keys, r[1], r[2])
+                assert np.array_equal(m[1], c)
+                if m[0] is None:
+                    continue
+                assert np.array_equal(m[0], f)
+
+
+def test_device_requests_count_only_leave_first_untouched(bpt, served_tree):
+    """Fused count-only range requests leave their slices of the lower-bound
+    array as they were; the others fill theirs."""
+    t, keys, vals = served_tree
+    rng = np.random.default_rng(60)
+    reqs = []
+    for i in range(64):
+        s_ = rng.integers(0, len(keys), 50)
+        reqs.append(("range", keys[s_].astype(np.uint64), keys[np.minimum(s_ + 10, len(keys) - 1)].astype(np.uint64)))
+    no_first = set(range(0, 64, 2))
+    d, structs, where = _device_requests(bpt, reqs, no_first)
+    srv = bpt.Server(t, slots=2, mem_budget=1 << 26, mode=0)
+    srv.replay(structs, np.zeros(len(structs)))
+    srv.close()
+    first = d["first"].to_host()
+    for w in where:
+        sl = first[w[1]:w[1] + w[2]]
+        if w[3]:
+            assert (sl == 0x5A5A5A5A5A5A5A5A).all()
+        else:
+            assert (sl != 0x5A5A5A5A5A5A5A5A).all()
+    for a in d.values():
+        a.free()
+
+
+def test_host_and_device_requests_in_one_stream(bpt, oracle, served_tree):
+    """Host and device requests interleaved in one replay are never fused
+    together; both kinds get the oracle's answers."""
+    t, keys, vals = served_tree
+    rng = np.random.default_rng(70)
+    qs = [np.where(rng.random(n) < 0.9, keys[rng.integers(0, len(keys), n)],
+                   rng.integers(0, 1 << 50, n, dtype=np.uint64)).astype(np.uint64)
+          for n in rng.integers(1, 400, size=200)]
+    d, dstructs, where = _device_requests(bpt, [("point", q) for q in qs[1::2]])
+    houts = [np.zeros(len(q), dtype=np.uint64) for q in qs[0::2]]
+    structs = []
+    for i in range(len(qs)):
+        structs.append(bpt.Server.point_request(qs[i], houts[i // 2]) if i % 2 == 0 else dstructs[i // 2])
+    srv = bpt.Server(t, slots=2, mem_budget=1 << 26, mode=0)
+    srv.replay(structs, np.zeros(len(structs)))
+    assert srv.stats()["completed"] == len(qs)
+    srv.close()
+    dres = _device_results(d, where)
+    for a in d.values():
+        a.free()
+    for i, q in enumerate(qs):
+        got = houts[i // 2] if i % 2 == 0 else dres[i // 2]
+        assert np.array_equal(got, oracle.point_closed(keys, vals, q, SENT))
+
+
+def test_device_request_checks(bpt, served_tree):
+    """Bad flags and device buffers not aligned to their element width are
+    rejected at submit, before any launch."""
+    t, keys, vals = served_tree
+    dq = bpt.DeviceArray.from_host(keys[:64])
+    dout = bpt.DeviceArray(64, np.uint64)
+    srv = bpt.Server(t, slots=1, mem_budget=1 << 24, mode=0)
+    bad = bpt.Server.point_request_dev(dq, dout)
+    bad.flags = 2
+    with pytest.raises(bpt.BptError) as e:
+        srv.submit(bad)
+    assert e.value.status == 1 and "flags" in str(e.value)
+    mis = bpt.Request(bpt.REQ_POINT, bpt.REQ_DEVICE, 8, dq.ptr + 4, dout.ptr, None, None, None, None)
+    with pytest.raises(bpt.BptError) as e:
+        srv.submit(mis)
+    assert e.value.status == 1 and "aligned" in str(e.value)
+    srv.close()
+    dq.free()
+    dout.free()
+
+
 @pytest.fixture(scope="module")
 def c5_tree(bpt, wl):
     """The learned-index configuration's tree: 256M unique lognormal uint64
