@@ -7,9 +7,10 @@
 //    different launches (staging copies, HIP calls, waiting) runs in
 //    parallel.
 //  * A thread that takes a request at the head of the queue also takes the
-//    requests of the same kind right behind it (up to kFuseQueries queries and
-//    kFuseRequests requests) and answers them with one staged copy-in, one
-//    search and one copy-out; small requests then share the per-launch cost.
+//    requests of the same kind queued behind it, in order, passing over those
+//    of the other kind (up to kFuseQueries queries and kFuseRequests
+//    requests), and answers them with one staged copy-in, one search and one
+//    copy-out; small requests then share the per-launch cost.
 //  * Device memory for requests comes from one pool of `mem_budget` bytes
 //    (first-fit, 256-byte granules). A request is admitted when a slot is
 //    free AND its bytes fit in the pool (a fused group that does not fit
@@ -118,6 +119,7 @@ struct bpt_server {
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<Request *> queue;
+    std::vector<size_t> pick;  // take_work's scratch: queue positions of the next launch
     std::unordered_map<uint64_t, Request *> reqs;
     uint64_t next_ticket = 1;
     bool stop = false;
@@ -314,31 +316,44 @@ void copy_out(const bpt_server *S, const Slot &sl, bpt_status st) {
 }
 
 // Takes the next launch off the queue (S->mu held): the head request and
-// the requests of its kind that follow it, as far as the fusion limits
-// allow; the group's device bytes must fit the pool, else the head alone is
-// tried. Returns false when nothing can be admitted now.
+// the requests of its kind and buffer kind queued behind it, in queue order
+// (others in between are passed over, not reordered), as far as the fusion
+// limits allow; the group's device bytes must fit the pool, else the head
+// alone is tried. Returns false when nothing can be admitted now.
 bool take_work(bpt_server *S, Slot &sl) {
     if (S->queue.empty()) return false;
     const Request *h = S->queue.front();
-    size_t k = 1;
+    std::vector<size_t> &pick = S->pick;
+    pick.assign(1, 0);
     uint64_t n = h->req.n;
-    while (k < S->queue.size() && k < kFuseRequests) {
-        const Request *r = S->queue[k];
-        if (r->req.kind != h->req.kind || r->req.flags != h->req.flags || n + r->req.n > kFuseQueries) break;
+    for (size_t i = 1; i < S->queue.size() && pick.size() < kFuseRequests; ++i) {
+        const Request *r = S->queue[i];
+        if (r->req.kind != h->req.kind || r->req.flags != h->req.flags) continue;
+        if (n + r->req.n > kFuseQueries) break;  // the kind's order is kept: no later one jumps this one
         n += r->req.n;
-        ++k;
+        pick.push_back(i);
     }
     uint64_t off = 0, bytes = 0;
-    if (k > 1) {
-        bytes = group_bytes(S->tree, S->mode, h->req.kind, h->req.flags, n, k);
-        if (!S->pool.alloc(bytes, &off)) k = 1;
+    if (pick.size() > 1) {
+        bytes = group_bytes(S->tree, S->mode, h->req.kind, h->req.flags, n, pick.size());
+        if (!S->pool.alloc(bytes, &off)) pick.resize(1);
     }
-    if (k == 1) {
+    if (pick.size() == 1) {
         bytes = h->bytes;
         if (!S->pool.alloc(bytes, &off)) return false;
     }
-    sl.group.assign(S->queue.begin(), S->queue.begin() + k);
-    S->queue.erase(S->queue.begin(), S->queue.begin() + k);
+    sl.group.clear();
+    for (size_t i : pick) sl.group.push_back(S->queue[i]);
+    if (pick.size() == 1) {
+        S->queue.pop_front();
+    } else {  // the rest of the queue, in order
+        size_t j = 0, w = 0;
+        for (size_t i = 0; i < S->queue.size(); ++i) {
+            if (j < pick.size() && pick[j] == i) ++j;
+            else S->queue[w++] = S->queue[i];
+        }
+        S->queue.resize(w);
+    }
     sl.off = off;
     sl.bytes = bytes;
     return true;

@@ -465,18 +465,32 @@ def serving_run(tree, keys, n, miss_space, nreq, modes, workload):
     total_lookups = sum(int(r.n) for r in reqs)
     pq = np.concatenate([k[1] for k in keep if k[0] == "p"])
     want_p = tree.point_search(pq, bptree.PLAIN)
-    serving = {}
-    for mode in modes:
-        name = {bptree.PLAIN: "plain", bptree.GROUPED: "grouped", bptree.LEARNED: "learned"}[mode]
-        srv = bptree.Server(tree, slots=8, mem_budget=8 << 30, mode=mode)
-        srv.replay(reqs[:200], np.zeros(200))  # warm-up
+    # the same requests with device buffers (slices of one device array per
+    # buffer kind): no host staging, results left on the device
+    rk = [k for k in keep if k[0] == "r"]
+    dq = bptree.DeviceArray.from_host(pq)
+    dout = bptree.DeviceArray(len(pq), vdt)
+    dlo = bptree.DeviceArray.from_host(np.concatenate([k[1] for k in rk]))
+    dhi = bptree.DeviceArray.from_host(np.concatenate([k[2] for k in rk]))
+    dfirst, dcount = bptree.DeviceArray(len(dlo), np.uint64), bptree.DeviceArray(len(dlo), np.uint64)
+    dreqs, pa, ra = [], 0, 0
+    for k in keep:
+        if k[0] == "p":
+            dreqs.append(bptree.Server.point_request_dev(dq, dout, pa, len(k[1])))
+            pa += len(k[1])
+        else:
+            dreqs.append(bptree.Server.range_request_dev(dlo, dhi, dfirst, dcount, ra, len(k[1])))
+            ra += len(k[1])
+
+    def measure(srv, rq):
+        srv.replay(rq[:200], np.zeros(200))  # warm-up
         # saturation: every request due at once, the median of three replays
-        # (one replay takes about 15 ms, short enough to vary between runs)
+        # (one replay takes a few ms, short enough to vary between runs)
         sats, sat_stats = [], []
         for _ in range(3):
             srv.reset_stats()
             t0 = time.perf_counter()
-            srv.replay(reqs, np.zeros(nreq))
+            srv.replay(rq, np.zeros(nreq))
             sats.append(nreq / (time.perf_counter() - t0))
             sat_stats.append(srv.stats())
         mid = int(np.argsort(sats)[1])
@@ -486,16 +500,31 @@ def serving_run(tree, keys, n, miss_space, nreq, modes, workload):
         for frac in (0.5, 0.8):
             srv.reset_stats()
             arr = np.cumsum(rr.exponential(1.0 / (frac * sat), size=nreq))
-            srv.replay(reqs, arr)
+            srv.replay(rq, arr)
             stt = srv.stats()
             res[f"load_{frac}"] = {"rate_requests_per_s": frac * sat, "p50_ms": stt["p50_ms"], "p99_ms": stt["p99_ms"],
                                    "lookups_per_s": stt["lookups"] / stt["elapsed_s"] if stt["elapsed_s"] else None,
                                    "max_inflight": stt["max_inflight"]}
+        return res
+
+    serving = {}
+    for mode in modes:
+        name = {bptree.PLAIN: "plain", bptree.GROUPED: "grouped", bptree.LEARNED: "learned"}[mode]
+        srv = bptree.Server(tree, slots=8, mem_budget=8 << 30, mode=mode)
+        res = measure(srv, reqs)
         got_p = np.concatenate([k[2] for k in keep if k[0] == "p"])
         res["point_mismatches_vs_one_batch"] = int((got_p != want_p).sum())
+        dout.fill_bytes(0)
+        res_d = measure(srv, dreqs)
+        res_d["point_mismatches_vs_one_batch"] = int((dout.to_host() != want_p).sum())
+        res_d["note"] = "requests with device buffers (BPT_REQ_DEVICE): no host staging, results left on the device"
         srv.close()
         serving[name] = res
+        serving[name + "_device"] = res_d
         log(f"serving {name}: " + json.dumps(res))
+        log(f"serving {name}, device buffers: " + json.dumps(res_d))
+    for a in (dq, dout, dlo, dhi, dfirst, dcount):
+        a.free()
     return serving, nreq, total_lookups
 
 
