@@ -682,8 +682,15 @@ class Server:
         check(st)
         return ms.value
 
+    @staticmethod
+    def request_array(reqs):
+        """The requests as one ctypes array, built once for repeated replays
+        (building it costs about 0.7 us per request in Python)."""
+        return (Request * len(reqs))(*reqs)
+
     def replay(self, reqs, arrival_s):
-        arr = (Request * len(reqs))(*reqs)
+        """Open-loop replay of a list of Requests or a request_array()."""
+        arr = reqs if isinstance(reqs, C.Array) else (Request * len(reqs))(*reqs)
         a = np.ascontiguousarray(arrival_s, dtype=np.float64)
         lat = np.zeros(len(reqs), dtype=np.float64)
         dp = C.POINTER(C.c_double)
