@@ -348,8 +348,8 @@ bpt_status bpt_mlp_loss_grad(const float *h_params, uint32_t hidden, const float
  * is rejected with BPT_ERR_CAPACITY). Buffers stay the caller's until the
  * request completes. Point requests use `mode` (bpt_point_mode). The head
  * of the queue and the requests of its kind and buffer kind queued behind
- * it, in order (up to 2^20 queries, 512 requests), are answered in one
- * launch; each caller gets its own results.
+ * it, in order (up to 512 requests and 2^18 queries, 2^20 for device
+ * requests), are answered in one launch; each caller gets its own results.
  * flags = BPT_REQ_DEVICE: every buffer of the request is device memory of
  * the tree's device, aligned to its element width; there are no host copies
  * (a request alone is searched in place, a fused run is gathered into the
