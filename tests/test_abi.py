@@ -67,6 +67,34 @@ def test_python_bindings_cover_header():
     assert not missing, missing
 
 
+def test_binding_structs_match_header_layout(tmp_path):
+    """Every struct the ctypes binding passes across the ABI has the size,
+    field offsets and field sizes of its C declaration (gcc on the header)."""
+    import subprocess
+    import bptree
+    structs = {"BuildDesc": "bpt_build_desc", "TreeInfo": "bpt_tree_info", "LearnedConfig": "bpt_learned_config",
+               "LearnedStats": "bpt_learned_stats", "BatchConfig": "bpt_batch_config", "Request": "bpt_request",
+               "ServerStats": "bpt_server_stats"}
+    src = ["#include <stdio.h>", "#include <stddef.h>", '#include "bptree.h"', "int main(void) {"]
+    want = []
+    for py, c in structs.items():
+        cls = getattr(bptree, py)
+        src.append(f'printf("{py} %zu\\n", sizeof({c}));')
+        want.append(f"{py} {ctypes.sizeof(cls)}")
+        for f in cls._fields_:
+            src.append(f'printf("{py}.{f[0]} %zu %zu\\n", offsetof({c}, {f[0]}), sizeof((({c} *)0)->{f[0]}));')
+            fd = getattr(cls, f[0])
+            want.append(f"{py}.{f[0]} {fd.offset} {fd.size}")
+    src.append("return 0; }")
+    (tmp_path / "layout.c").write_text("\n".join(src))
+    exe = tmp_path / "layout"
+    r = subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(tmp_path / "layout.c"), "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    got = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    assert [g for g in got if g] == want
+
+
 def test_errors_without_device_are_reported_not_fatal():
     import bptree
     L = bptree.lib()
