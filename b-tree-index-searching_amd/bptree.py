@@ -60,6 +60,8 @@ SIGNATURES = [
     ("bpt_device_info", C.c_int, [C.c_int, C.POINTER(C.c_int), u64p, C.c_char_p, C.c_int]),
     ("bpt_device_alloc", C.c_int, [C.c_uint64, C.POINTER(vp)]),
     ("bpt_device_free", C.c_int, [vp]),
+    ("bpt_host_alloc", C.c_int, [C.c_uint64, C.POINTER(vp)]),
+    ("bpt_host_free", C.c_int, [vp]),
     ("bpt_memset", C.c_int, [vp, C.c_int, C.c_uint64, vp]),
     ("bpt_copy_h2d", C.c_int, [vp, vp, C.c_uint64, vp]),
     ("bpt_copy_d2h", C.c_int, [vp, vp, C.c_uint64, vp]),
@@ -214,6 +216,40 @@ class DeviceArray:
     def free(self):
         if getattr(self, "ptr", 0) and not _exiting:
             lib().bpt_device_free(self.ptr)
+        self.ptr = 0
+
+    __del__ = free
+
+    def __len__(self):
+        return self.n
+
+
+class HostArray:
+    """A 1-D array in pinned host memory that the device addresses at the
+    same pointer (bpt_host_alloc); `a` is its numpy view. It can stand in
+    for a DeviceArray in BPT_REQ_DEVICE serving requests."""
+
+    def __init__(self, n, dtype):
+        self.dtype = np.dtype(dtype)
+        self.n = int(n)
+        self.nbytes = self.n * self.dtype.itemsize
+        p = vp()
+        check(lib().bpt_host_alloc(self.nbytes, C.byref(p)))
+        self.ptr = p.value or 0
+        buf = (C.c_char * self.nbytes).from_address(self.ptr) if self.nbytes else bytearray(0)
+        self.a = np.frombuffer(buf, dtype=self.dtype, count=self.n)
+
+    @classmethod
+    def from_host(cls, a):
+        a = np.ascontiguousarray(a)
+        h = cls(a.size, a.dtype)
+        h.a[:] = a
+        return h
+
+    def free(self):
+        self.a = None
+        if getattr(self, "ptr", 0) and not _exiting:
+            lib().bpt_host_free(self.ptr)
         self.ptr = 0
 
     __del__ = free

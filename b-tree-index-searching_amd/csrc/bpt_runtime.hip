@@ -142,6 +142,29 @@ bpt_status bpt_device_free(void *d_ptr) {
     return BPT_OK;
 }
 
+bpt_status bpt_host_alloc(uint64_t bytes, void **h_ptr) {
+    BPT_CHECK(h_ptr, BPT_ERR_INVALID, "h_ptr is NULL");
+    *h_ptr = nullptr;
+    if (bytes == 0) return BPT_OK;
+    void *p = nullptr;
+    BPT_HIP(hipHostMalloc(&p, bytes, hipHostMallocMapped | hipHostMallocPortable));
+    void *d = nullptr;
+    const hipError_t e = hipHostGetDevicePointer(&d, p, 0);
+    if (e != hipSuccess || d != p) {  // the contract is one address for host and device
+        (void)hipHostFree(p);
+        if (e != hipSuccess) return bpt::hip_fail(e, "hipHostGetDevicePointer", __FILE__, __LINE__);
+        bpt::set_error("pinned memory maps to another device address");
+        return BPT_ERR_UNSUPPORTED;
+    }
+    *h_ptr = p;
+    return BPT_OK;
+}
+
+bpt_status bpt_host_free(void *h_ptr) {
+    if (h_ptr) BPT_HIP(hipHostFree(h_ptr));
+    return BPT_OK;
+}
+
 bpt_status bpt_memset(void *d_ptr, int value, uint64_t bytes, bpt_stream s) {
     if (bytes == 0) return BPT_OK;
     BPT_HIP(hipMemsetAsync(d_ptr, value, bytes, bpt::as_stream(s)));

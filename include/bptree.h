@@ -51,6 +51,12 @@ bpt_status bpt_device_count(int *count);
 bpt_status bpt_device_info(int device, int *cu_count, uint64_t *hbm_bytes, char *name, int name_len);
 bpt_status bpt_device_alloc(uint64_t bytes, void **d_ptr);
 bpt_status bpt_device_free(void *d_ptr);
+/* Pinned host memory mapped into the device's address space at the same
+ * address: the host reads and writes it through the pointer, and kernels
+ * (the serving loop's BPT_REQ_DEVICE requests, bpt_copy_*) can take the
+ * same pointer, reaching it over PCIe. */
+bpt_status bpt_host_alloc(uint64_t bytes, void **h_ptr);
+bpt_status bpt_host_free(void *h_ptr);
 bpt_status bpt_memset(void *d_ptr, int value, uint64_t bytes, bpt_stream s);
 /* Synchronous copies (the stream is synchronized before return). */
 bpt_status bpt_copy_h2d(void *d_dst, const void *h_src, uint64_t bytes, bpt_stream s);
@@ -350,8 +356,9 @@ bpt_status bpt_mlp_loss_grad(const float *h_params, uint32_t hidden, const float
  * of the queue and the requests of its kind and buffer kind queued behind
  * it, in order (up to 512 requests and 2^18 queries, 2^20 for device
  * requests), are answered in one launch; each caller gets its own results.
- * flags = BPT_REQ_DEVICE: every buffer of the request is device memory of
- * the tree's device, aligned to its element width; there are no host copies
+ * flags = BPT_REQ_DEVICE: every buffer of the request is memory the tree's
+ * device can address -- its device memory, or pinned host memory from
+ * bpt_host_alloc -- aligned to its element width; there are no host copies
  * (a request alone is searched in place, a fused run is gathered into the
  * pool and its results scattered back by two copy kernels), and the results
  * are in place when wait/replay returns. flags = 0: host buffers, staged

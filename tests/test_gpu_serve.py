@@ -278,6 +278,45 @@ def test_host_and_device_requests_in_one_stream(bpt, oracle, served_tree):
         assert np.array_equal(got, oracle.point_closed(keys, vals, q, SENT))
 
 
+@pytest.mark.parametrize("fused", [False, True])
+def test_pinned_host_requests_without_staging(bpt, oracle, served_tree, fused):
+    """BPT_REQ_DEVICE requests whose buffers are pinned host memory from
+    bpt_host_alloc: the search (alone) or the gather and scatter kernels
+    (fused) reach them over PCIe, with no staging copy. Every answer equals
+    the oracle."""
+    t, keys, vals = served_tree
+    rng = np.random.default_rng(80 + fused)
+    sizes = rng.integers(1, 3000, size=120)
+    q = np.where(rng.random(int(sizes.sum())) < 0.85, keys[rng.integers(0, len(keys), int(sizes.sum()))],
+                 rng.integers(0, 1 << 50, int(sizes.sum()), dtype=np.uint64)).astype(np.uint64)
+    s_ = rng.integers(0, len(keys), 40 * 64)
+    lo, hi = keys[s_].astype(np.uint64), keys[np.minimum(s_ + 500, len(keys) - 1)].astype(np.uint64)
+    hq, hout = bpt.HostArray.from_host(q), bpt.HostArray(len(q), np.uint64)
+    hlo, hhi = bpt.HostArray.from_host(lo), bpt.HostArray.from_host(hi)
+    hfirst, hcount = bpt.HostArray(len(lo), np.uint64), bpt.HostArray(len(lo), np.uint64)
+    hout.a[:] = 7
+    structs, at = [], 0
+    for m in sizes:
+        structs.append(bpt.Server.point_request_dev(hq, hout, at, int(m)))
+        at += int(m)
+    for r in range(40):
+        structs.insert(3 * r, bpt.Server.range_request_dev(hlo, hhi, hfirst, hcount, 64 * r, 64))
+    srv = bpt.Server(t, slots=4 if fused else 1, mem_budget=1 << 28, mode=0)
+    if fused:
+        srv.replay(structs, np.zeros(len(structs)))
+    else:
+        for st_ in structs:
+            srv.wait(srv.submit(st_))
+    st = srv.stats()
+    srv.close()
+    assert st["completed"] == len(structs)
+    assert np.array_equal(hout.a, oracle.point_closed(keys, vals, q, SENT))
+    f, c = oracle.range_closed(keys, lo, hi)
+    assert np.array_equal(hcount.a, c) and np.array_equal(hfirst.a, f)
+    for a in (hq, hout, hlo, hhi, hfirst, hcount):
+        a.free()
+
+
 def test_device_request_checks(bpt, served_tree):
     """Bad flags and device buffers not aligned to their element width are
     rejected at submit, before any launch."""
