@@ -473,13 +473,21 @@ def serving_run(tree, keys, n, miss_space, nreq, modes, workload):
     dlo = bptree.DeviceArray.from_host(np.concatenate([k[1] for k in rk]))
     dhi = bptree.DeviceArray.from_host(np.concatenate([k[2] for k in rk]))
     dfirst, dcount = bptree.DeviceArray(len(dlo), np.uint64), bptree.DeviceArray(len(dlo), np.uint64)
-    dreqs, pa, ra = [], 0, 0
+    # and with pinned host buffers that the device addresses directly
+    # (bpt_host_alloc): no staging copies, the data crosses PCIe in the kernels
+    hq, hout = bptree.HostArray.from_host(pq), bptree.HostArray(len(pq), vdt)
+    hlo = bptree.HostArray.from_host(np.concatenate([k[1] for k in rk]))
+    hhi = bptree.HostArray.from_host(np.concatenate([k[2] for k in rk]))
+    hfirst, hcount = bptree.HostArray(len(hlo), np.uint64), bptree.HostArray(len(hlo), np.uint64)
+    dreqs, hreqs, pa, ra = [], [], 0, 0
     for k in keep:
         if k[0] == "p":
             dreqs.append(bptree.Server.point_request_dev(dq, dout, pa, len(k[1])))
+            hreqs.append(bptree.Server.point_request_dev(hq, hout, pa, len(k[1])))
             pa += len(k[1])
         else:
             dreqs.append(bptree.Server.range_request_dev(dlo, dhi, dfirst, dcount, ra, len(k[1])))
+            hreqs.append(bptree.Server.range_request_dev(hlo, hhi, hfirst, hcount, ra, len(k[1])))
             ra += len(k[1])
 
     def measure(srv, rq):
@@ -519,12 +527,19 @@ def serving_run(tree, keys, n, miss_space, nreq, modes, workload):
         res_d = measure(srv, dreqs)
         res_d["point_mismatches_vs_one_batch"] = int((dout.to_host() != want_p).sum())
         res_d["note"] = "requests with device buffers (BPT_REQ_DEVICE): no host staging, results left on the device"
+        hout.a[:] = 0
+        res_h = measure(srv, hreqs)
+        res_h["point_mismatches_vs_one_batch"] = int((hout.a != want_p).sum())
+        res_h["note"] = ("requests in pinned host memory (bpt_host_alloc) passed as BPT_REQ_DEVICE: no staging "
+                         "copies, the kernels read and write them over PCIe")
         srv.close()
         serving[name] = res
         serving[name + "_device"] = res_d
+        serving[name + "_pinned_host"] = res_h
         log(f"serving {name}: " + json.dumps(res))
         log(f"serving {name}, device buffers: " + json.dumps(res_d))
-    for a in (dq, dout, dlo, dhi, dfirst, dcount):
+        log(f"serving {name}, pinned host buffers: " + json.dumps(res_h))
+    for a in (dq, dout, dlo, dhi, dfirst, dcount, hq, hout, hlo, hhi, hfirst, hcount):
         a.free()
     return serving, nreq, total_lookups
 
