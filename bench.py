@@ -743,6 +743,33 @@ def main():
         forms["plain_end_to_end"] = {"lookups_per_s": nq / wall_e, "ms": wall_e * 1e3,
                                      "identical_to_resident": bool(np.array_equal(host_out, gpu_out)),
                                      "note": "pageable host buffers; H2D 1 GiB + search + D2H 1 GiB"}
+        # the same from pinned host buffers (bpt_host_alloc): DMA copies in and
+        # out, and the search reading and writing the host buffers in place
+        hq, hout = bptree.HostArray.from_host(q), bptree.HostArray(nq, np.uint32)
+        e2e_dma, e2e_zc = [], []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            bptree.check(bptree.lib().bpt_copy_h2d(dq.ptr, hq.ptr, hq.nbytes, None))
+            tree.point_search_dev(dq, dout, bptree.PLAIN)
+            bptree.check(bptree.lib().bpt_copy_d2h(hout.ptr, dout.ptr, hout.nbytes, None))
+            e2e_dma.append(time.perf_counter() - t0)
+        same_dma = bool(np.array_equal(hout.a, gpu_out))
+        hout.a[:] = 0
+        for _ in range(3):
+            t0 = time.perf_counter()
+            tree.point_search_dev(hq, hout, bptree.PLAIN)
+            bptree.sync()
+            e2e_zc.append(time.perf_counter() - t0)
+        same_zc = bool(np.array_equal(hout.a, gpu_out))
+        hq.free()
+        hout.free()
+        w_dma, w_zc = float(np.median(e2e_dma)), float(np.median(e2e_zc))
+        forms["plain_end_to_end_pinned"] = {
+            "dma": {"lookups_per_s": nq / w_dma, "ms": w_dma * 1e3, "identical_to_resident": same_dma},
+            "in_place": {"lookups_per_s": nq / w_zc, "ms": w_zc * 1e3, "identical_to_resident": same_zc},
+            "note": "pinned host buffers (bpt_host_alloc): H2D 1 GiB + search + D2H 1 GiB by DMA, or the search "
+                    "reading the queries and writing the results in host memory over PCIe"}
+        log("end to end, pinned: " + json.dumps(forms["plain_end_to_end_pinned"]))
         del host_out
         wall_g, ev_g = time_form(tree, dq, dout, bptree.GROUPED, max(3, args.steps // 2), 1)
         # the grouped form's own algorithmic bytes per lookup: the histogram

@@ -317,6 +317,21 @@ def test_pinned_host_requests_without_staging(bpt, oracle, served_tree, fused):
         a.free()
 
 
+def test_point_search_in_pinned_host_memory(bpt, oracle, served_tree):
+    """bpt_point_search straight from and into pinned host memory (the
+    kernel reads the queries and writes the results over PCIe)."""
+    t, keys, vals = served_tree
+    rng = np.random.default_rng(90)
+    q = np.where(rng.random(100_003) < 0.9, keys[rng.integers(0, len(keys), 100_003)],
+                 rng.integers(0, 1 << 50, 100_003, dtype=np.uint64)).astype(np.uint64)
+    hq, hout = bpt.HostArray.from_host(q), bpt.HostArray(len(q), np.uint64)
+    t.point_search_dev(hq, hout, bpt.PLAIN)
+    bpt.sync()
+    assert np.array_equal(hout.a, oracle.point_closed(keys, vals, q, SENT))
+    hq.free()
+    hout.free()
+
+
 def test_device_request_checks(bpt, served_tree):
     """Bad flags and device buffers not aligned to their element width are
     rejected at submit, before any launch."""
