@@ -107,6 +107,11 @@ def test_errors_without_device_are_reported_not_fatal():
     d = bptree.BuildDesc(4, 4, 48, 16, 0, 0)
     assert L.bpt_tree_build(ctypes.byref(d), None, None, 0, ctypes.byref(h), None) == 1
     assert L.bpt_tree_free(None) == 0
+    # pinned host memory: NULL output rejected, empty and NULL frees are no-ops
+    p = ctypes.c_void_p(1)
+    assert L.bpt_host_alloc(16, None) == 1 and b"NULL" in L.bpt_last_error()
+    assert L.bpt_host_alloc(0, ctypes.byref(p)) == 0 and not p.value
+    assert L.bpt_host_free(None) == 0
 
 
 def test_bench_refuses_more_than_one_gpu():
