@@ -204,20 +204,27 @@ __global__ __launch_bounds__(256) void k_range_count_coop(TreeView t, const K *_
     extern __shared__ uint4 lds_raw[];
     uint8_t *lds = reinterpret_cast<uint8_t *>(lds_raw);
     stage_top_levels(t, lds);
-    const uint32_t lane = threadIdx.x & 63;
+    // 32 ranges per wave step: lane r < 32 descends for lower_bound(lo) of
+    // range r and lane r + 32 for lower_bound(hi + 1), so the G-lane groups
+    // of the two halves walk a range's two bounds in the same load
+    // instructions, and a node or leaf that both bounds reach (level 2 nearly
+    // always, often level 1 and the leaf) is one line request, not two.
+    const uint32_t lane = threadIdx.x & 63, r = lane & 31;
+    const bool upper = lane >= 32;
     const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const uint64_t stride = ((uint64_t)gridDim.x * blockDim.x) & ~63ull;
-    for (uint64_t base = wave * 64; base < nq; base += stride) {
-        const uint64_t j = base + lane;
+    const uint64_t stride = (((uint64_t)gridDim.x * blockDim.x) >> 6) * 32;
+    for (uint64_t base = wave * 32; base < nq; base += stride) {
+        const uint64_t j = base + r;
         const bool valid = j < nq;
         const uint64_t i = valid ? (order ? (uint64_t)order[j] : j) : 0;
         const K a = valid ? lo[i] : K(0), b = valid ? hi[i] : K(0);
         const bool top = b == key_max<K>();
-        const uint64_t pa = coop_lookup64<K, V, F, LF, false, kOpLowerBound, G, OFF32>(t, lds, a, V(0));
-        const uint64_t pb1 =
-            coop_lookup64<K, V, F, LF, false, kOpLowerBound, G, OFF32>(t, lds, top ? b : K(b + 1), V(0));
+        const K key = upper ? (top ? b : K(b + 1)) : a;
+        const uint64_t p = coop_lookup64<K, V, F, LF, false, kOpLowerBound, G, OFF32>(t, lds, key, V(0));
+        const uint64_t pa = __shfl(p, (int)r, 64);
+        const uint64_t pb1 = __shfl(p, (int)(r + 32), 64);
         const uint64_t pb = top ? t.n : pb1;
-        if (valid) {
+        if (valid && !upper) {
             if (first) first[i] = pa;
             count[i] = (a > b || pb < pa) ? 0 : pb - pa;
         }
