@@ -12,7 +12,8 @@
 //  * persistent kernel: each block takes the next item from an atomic
 //    counter, stages that tree's descriptor and LDS-resident top levels when
 //    the tree changes, and walks the item's queries with the cooperative
-//    8-lane descent; ranges take two lower bounds (lo and hi + 1);
+//    8-lane descent; ranges take two lower bounds (lo and hi + 1), walked in
+//    the same load instructions;
 //  * the counter reset and the launch are captured into a hipGraph and
 //    replayed (the reset is a kernel, not a memset node);
 //  * an autotuner times fanout x LDS budget x chunk and keeps the fastest.
@@ -185,16 +186,23 @@ __global__ __launch_bounds__(256) void k_forest_batch(const TreeView *__restrict
                 if (my < wi.end) __builtin_nontemporal_store(r, &out[my]);
             }
         } else {
-            for (uint64_t base = wi.begin + (uint64_t)w * 64; base < wi.end; base += (uint64_t)nw * 64) {
-                const uint64_t my = base + lane;
+            // 32 ranges per wave step: lanes 0-31 descend for the lower
+            // bounds, lanes 32-63 for the upper ones, so a range's two
+            // descents share their load instructions and a node both reach
+            // is one line request
+            const uint32_t r = lane & 31;
+            const bool upper = lane >= 32;
+            for (uint64_t base = wi.begin + (uint64_t)w * 32; base < wi.end; base += (uint64_t)nw * 32) {
+                const uint64_t my = base + r;
                 const bool ok = my < wi.end;
                 const uint64_t a = ok ? lo[my] : 0ull, b = ok ? hi[my] : 0ull;
-                const uint64_t pa = coop_lookup64<uint64_t, uint64_t, F, LF, false, kOpLowerBound, G>(tv, lds, a, 0ull);
                 const bool top = b == ~0ull;
-                const uint64_t pb0 =
-                    coop_lookup64<uint64_t, uint64_t, F, LF, false, kOpLowerBound, G>(tv, lds, top ? b : b + 1, 0ull);
+                const uint64_t key = upper ? (top ? b : b + 1) : a;
+                const uint64_t p = coop_lookup64<uint64_t, uint64_t, F, LF, false, kOpLowerBound, G>(tv, lds, key, 0ull);
+                const uint64_t pa = __shfl(p, (int)r, 64);
+                const uint64_t pb0 = __shfl(p, (int)(r + 32), 64);
                 const uint64_t pb = top ? tv.n : pb0;
-                if (ok) {
+                if (ok && !upper) {
                     first[my] = pa;
                     count[my] = (a > b || pb < pa) ? 0 : pb - pa;
                 }
