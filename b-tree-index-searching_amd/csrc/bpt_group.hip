@@ -382,6 +382,11 @@ uint64_t query_order_ws(uint64_t n) {
     return 256 + align256(n * 8) + sort_ws_bytes(n, 4);
 }
 
+// Clears the two maxima k_max2 accumulates into.
+__global__ void k_zero_max2(unsigned long long *mx) {
+    if (threadIdx.x < 2) mx[threadIdx.x] = 0;
+}
+
 template <class KT>
 bpt_status query_order(const KT *lo, const KT *hi, uint64_t n, int mode, void *ws, uint64_t ws_bytes,
                        uint32_t **order, hipStream_t s) {
@@ -395,7 +400,7 @@ bpt_status query_order(const KT *lo, const KT *hi, uint64_t n, int mode, void *w
     *order = nullptr;
     if (n == 0) return BPT_OK;
     const uint32_t g = (uint32_t)std::min<uint64_t>((n + 255) / 256, (uint64_t)device_cu_count() * 16);
-    BPT_HIP(hipMemsetAsync(mx, 0, 16, s));
+    k_zero_max2<<<1, 64, 0, s>>>(mx);  // a kernel, not a memset: callers may capture the search into a graph
     k_max2<KT><<<(uint32_t)std::min<uint64_t>(g, (uint64_t)device_cu_count() * 2), 256, 0, s>>>(lo, hi, n, mx);
     BPT_HIP(hipGetLastError());
     if (mode == BPT_GROUP_GRID2D) {
