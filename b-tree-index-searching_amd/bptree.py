@@ -226,8 +226,10 @@ class DeviceArray:
 
 class HostArray:
     """A 1-D array in pinned host memory that the device addresses at the
-    same pointer (bpt_host_alloc); `a` is its numpy view. It can stand in
-    for a DeviceArray in BPT_REQ_DEVICE serving requests."""
+    same pointer (bpt_host_alloc); `a` is its numpy view, valid until
+    free() (a view kept past it points at released memory). It can stand
+    in for a DeviceArray in BPT_REQ_DEVICE serving requests and in
+    point_search_dev."""
 
     def __init__(self, n, dtype):
         self.dtype = np.dtype(dtype)
